@@ -1,0 +1,227 @@
+"""ctypes binding of libakx.so (the C ABI of include/akx.h).
+
+`SolidMechanicsModel` mirrors the reference class of the same name
+(src/model/solid_mechanics/solid_mechanics_model.hh:58-543) for the explicit
+lumped-mass path: the constructor takes a mesh, materials are added as in the
+reference's material file, `init_full()` corresponds to
+``initFull(_analysis_method = _explicit_lumped_mass)``, and the nodal arrays
+are reached through get_/set_ accessors that copy to and from the GPU.
+
+There is no CPU fallback: importing this module raises if libakx.so was not
+built, and every call raises AkxError on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libakx.so")
+
+TRIANGLE_3, QUADRANGLE_4, TETRAHEDRON_4, HEXAHEDRON_8 = 0, 1, 2, 3
+ELASTIC, NEOHOOKEAN, NEOHOOKEAN_DAMAGE = 0, 1, 2
+FP64, MIXED = 0, 1
+FIELDS = {"displacement": 0, "velocity": 1, "acceleration": 2, "external_force": 3,
+          "internal_force": 4, "mass": 5, "blocked_dofs": 6, "previous_displacement": 7,
+          "displacement_increment": 8, "grad_u": 9, "stress": 10, "damage": 11,
+          "shapes_derivatives": 12, "jacobians": 13, "current_position": 14}
+ENERGIES = {"kinetic": 0, "potential": 1, "external work": 2}
+
+# Every entry point of include/akx.h, with its ctypes signature.
+_P, _I, _I64, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+_PD = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "akx_ctx_create": (_I, [_I, ctypes.c_uint, ctypes.POINTER(_P)]),
+    "akx_ctx_destroy": (_I, [_P]),
+    "akx_last_error": (ctypes.c_char_p, [_P]),
+    "akx_set_stream": (_I, [_P, _P]),
+    "akx_mesh_set": (_I, [_P, _I, _I, _I64, _P, _I64, _P]),
+    "akx_material_add": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(_I)]),
+    "akx_material_damage": (_I, [_P, _I, _D, _P]),
+    "akx_assign_materials": (_I, [_P, _P]),
+    "akx_init": (_I, [_P]),
+    "akx_field_upload": (_I, [_P, _I, _P, _SZ]),
+    "akx_field_download": (_I, [_P, _I, _P, _SZ]),
+    "akx_field_device_pointer": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "akx_node_numbering": (_I, [_P, _P]),
+    "akx_set_time_step": (_I, [_P, _D]),
+    "akx_solve_step": (_I, [_P, _I, _I]),
+    "akx_assemble_internal_forces": (_I, [_P, _I]),
+    "akx_stable_time_step": (_I, [_P, _PD]),
+    "akx_energy": (_I, [_P, _I, _PD]),
+    "akx_set_dirichlet_increment": (_I, [_P, _I64, _P, _I, _D]),
+    "akx_synchronize": (_I, [_P]),
+    "akx_enable_timing": (_I, [_P, _I]),
+    "akx_last_step_timing": (_I, [_P, _PD, _PD]),
+    "akx_set_launch": (_I, [_P, _I, _I, _I]),
+}
+
+
+class AkxError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libakx.so is not built (run __graft_entry__.build() or "
+                          "python akantu_amd/build.py); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """Owns one akx_ctx (device buffers + stream)."""
+
+    def __init__(self, device: int = 0, keep_numbering: bool = False):
+        h = _P()
+        rc = lib.akx_ctx_create(device, 1 if keep_numbering else 0, ctypes.byref(h))
+        if rc != 0:
+            raise AkxError("akx_ctx_create failed (%d): no usable HIP device %d" % (rc, device))
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            raise AkxError("akx error %d: %s" % (rc, lib.akx_last_error(self.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.akx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class SolidMechanicsModel:
+    """Explicit lumped-mass SolidMechanicsModel on one GPU (see module doc)."""
+
+    def __init__(self, mesh, device: int = 0, keep_numbering: bool = False):
+        self.mesh = mesh
+        self.dim = mesh.dim
+        self.ctx = Context(device, keep_numbering)
+        self._X = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
+        self._conn = np.ascontiguousarray(mesh.conn, dtype=np.uint32)
+        self.ctx.check(lib.akx_mesh_set(self.ctx.h, mesh.elem_type, mesh.dim, mesh.nb_nodes,
+                                        _ptr(self._X), mesh.nb_elements, _ptr(self._conn)))
+        self.materials = []
+        self.time_step = 0.0
+
+    # -- materials (MaterialFactory / input file `material elastic [...]`) --
+    def add_material(self, kind=ELASTIC, E=None, nu=0.0, rho=None, plane_stress=False,
+                     Sd=None, Yd=None):
+        mid = ctypes.c_int(-1)
+        self.ctx.check(lib.akx_material_add(self.ctx.h, kind, float(E), float(nu), float(rho),
+                                            int(bool(plane_stress)), ctypes.byref(mid)))
+        if kind == NEOHOOKEAN_DAMAGE:
+            yd = np.ascontiguousarray(Yd, dtype=np.float64)
+            self.ctx.check(lib.akx_material_damage(self.ctx.h, mid.value, float(Sd), _ptr(yd)))
+        self.materials.append(dict(kind=kind, E=E, nu=nu, rho=rho))
+        return mid.value
+
+    def assign_materials(self, element_material):
+        em = np.ascontiguousarray(element_material, dtype=np.int32)
+        self.ctx.check(lib.akx_assign_materials(self.ctx.h, _ptr(em)))
+
+    def init_full(self):
+        self.ctx.check(lib.akx_init(self.ctx.h))
+
+    # -- time stepping --
+    def get_stable_time_step(self):
+        dt = ctypes.c_double()
+        self.ctx.check(lib.akx_stable_time_step(self.ctx.h, ctypes.byref(dt)))
+        return dt.value
+
+    def set_time_step(self, dt):
+        self.time_step = float(dt)
+        self.ctx.check(lib.akx_set_time_step(self.ctx.h, float(dt)))
+
+    def solve_step(self, nsteps=1, precision=FP64):
+        self.ctx.check(lib.akx_solve_step(self.ctx.h, int(nsteps), int(precision)))
+
+    def assemble_internal_forces(self, precision=FP64):
+        self.ctx.check(lib.akx_assemble_internal_forces(self.ctx.h, int(precision)))
+
+    def get_energy(self, kind):
+        v = ctypes.c_double()
+        self.ctx.check(lib.akx_energy(self.ctx.h, ENERGIES[kind], ctypes.byref(v)))
+        return v.value
+
+    def set_dirichlet_increment(self, nodes, axis, increment):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        self.ctx.check(lib.akx_set_dirichlet_increment(self.ctx.h, nodes.size, _ptr(nodes),
+                                                       int(axis), float(increment)))
+
+    def synchronize(self):
+        self.ctx.check(lib.akx_synchronize(self.ctx.h))
+
+    def enable_timing(self, on=True):
+        self.ctx.check(lib.akx_enable_timing(self.ctx.h, int(bool(on))))
+
+    def last_step_timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(lib.akx_last_step_timing(self.ctx.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_launch(self, element_block=0, node_block=0, xcd_remap=-1):
+        self.ctx.check(lib.akx_set_launch(self.ctx.h, element_block, node_block, xcd_remap))
+
+    # -- fields --
+    def _shape(self, name):
+        m, d = self.mesh, self.dim
+        eq = m.nb_elements * m.nb_qp
+        if name in ("grad_u", "stress"):
+            return (eq, d * d)
+        if name in ("damage", "jacobians"):
+            return (eq,)
+        if name == "shapes_derivatives":
+            return (eq, d * m.conn.shape[1])
+        return (m.nb_nodes, d)
+
+    def get(self, name):
+        dtype = np.uint8 if name == "blocked_dofs" else np.float64
+        out = np.zeros(self._shape(name), dtype=dtype)
+        self.ctx.check(lib.akx_field_download(self.ctx.h, FIELDS[name], _ptr(out), out.nbytes))
+        if name == "blocked_dofs":
+            return out.astype(bool)
+        return out
+
+    def set(self, name, values):
+        dtype = np.uint8 if name == "blocked_dofs" else np.float64
+        arr = np.ascontiguousarray(np.broadcast_to(np.asarray(values).astype(dtype),
+                                                   self._shape(name)))
+        self.ctx.check(lib.akx_field_upload(self.ctx.h, FIELDS[name], _ptr(arr), arr.nbytes))
+
+    # reference-style accessors
+    def get_displacement(self):
+        return self.get("displacement")
+
+    def get_velocity(self):
+        return self.get("velocity")
+
+    def get_acceleration(self):
+        return self.get("acceleration")
+
+    def get_internal_force(self):
+        return self.get("internal_force")
+
+    def get_mass(self):
+        return self.get("mass")
+
+    def get_blocked_dofs(self):
+        return self.get("blocked_dofs")
+
+    def close(self):
+        self.ctx.close()

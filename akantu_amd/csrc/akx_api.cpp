@@ -1,0 +1,944 @@
+// akx_api.cpp -- the C ABI of include/akx.h: context, buffers, renumbering,
+// CSR build and the step driver that enqueues the HIP kernels of
+// akx_kernels.hip.
+//
+// Internal numbering.  Unless the context is created with
+// AKX_CTX_KEEP_NUMBERING, nodes and elements are renumbered along a Morton
+// (Z-order) curve of their positions / centroids when the mesh is set, so that
+// the elements that share a node, and the nodes of an element, sit close
+// together in memory (the node gather and the displacement gather then hit
+// the same cache lines).  Every host array crossing the ABI stays in the
+// caller's numbering; the node -> slot CSR keeps the caller's element order,
+// so the FP64 sums are unchanged by the renumbering.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/akx.h"
+#include "akx_internal.hpp"
+
+using namespace akx;
+
+namespace {
+
+int fail(akx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(c, expr)                                                            \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      return fail((c), AKX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// The mass CSR aliases the force CSR when both orders coincide: release the
+// alias first so that no buffer is freed twice.
+void free_csr(akx_ctx* c) {
+  if (c->mass_off == c->csr_off) c->mass_off = nullptr; else dfree(c->mass_off);
+  if (c->mass_slot == c->csr_slot) c->mass_slot = nullptr; else dfree(c->mass_slot);
+  dfree(c->csr_off);
+  dfree(c->csr_slot);
+}
+
+void free_derived(akx_ctx* c) {
+  dfree(c->B64); dfree(c->J64); dfree(c->B32); dfree(c->J32);
+  free_csr(c);
+  dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
+  dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
+  for (auto& g : c->groups) dfree(g.filter);
+  c->groups.clear();
+  c->initialized = false;
+}
+
+void free_mesh_buffers(akx_ctx* c) {
+  free_derived(c);
+  dfree(c->X); dfree(c->conn_soa);
+  dfree(c->u); dfree(c->v); dfree(c->a); dfree(c->fext); dfree(c->fint); dfree(c->mass);
+  dfree(c->prev); dfree(c->inc); dfree(c->blocked); dfree(c->work);
+}
+
+int elem_dim(int et) { return (et == AKX_TETRAHEDRON_4 || et == AKX_HEXAHEDRON_8) ? 3 : 2; }
+int elem_nodes(int et) { return et == AKX_TRIANGLE_3 ? 3 : et == AKX_HEXAHEDRON_8 ? 8 : 4; }
+int elem_qp(int et) { return et == AKX_QUADRANGLE_4 ? 4 : et == AKX_HEXAHEDRON_8 ? 8 : 1; }
+
+// Gauss points, weights, N and dN/dxi of the element (host FP64: glibc sqrt).
+// src/fe_engine/gauss_integration.cc:55-56,72-73,122-123 and the element classes
+// element_class_{triangle_3,quadrangle_4,tetrahedron_4,hexahedron_8}_inline_impl.hh.
+ElemTables make_tables(int et) {
+  ElemTables t;
+  std::memset(&t, 0, sizeof t);
+  const double g = 1. / std::sqrt(3.);
+  const int d = elem_dim(et), Q = elem_qp(et);
+  for (int q = 0; q < Q; ++q) {
+    double c[3] = {0, 0, 0};
+    double* D = t.dnds[q];
+    auto set = [&](int i, int a, double v) { D[i + a * d] = v; };
+    switch (et) {
+    case AKX_TRIANGLE_3:
+      c[0] = 1. / 3.; c[1] = 1. / 3.; t.w[q] = 1. / 2.;
+      set(0, 0, -1.); set(0, 1, 1.); set(0, 2, 0.); set(1, 0, -1.); set(1, 1, 0.); set(1, 2, 1.);
+      t.N[q][0] = 1 - c[0] - c[1]; t.N[q][1] = c[0]; t.N[q][2] = c[1];
+      break;
+    case AKX_QUADRANGLE_4:
+      c[0] = (q & 1) ? g : -g; c[1] = (q & 2) ? g : -g; t.w[q] = 1.;
+      set(0, 0, -.25 * (1 - c[1])); set(0, 1, .25 * (1 - c[1]));
+      set(0, 2, .25 * (1 + c[1])); set(0, 3, -.25 * (1 + c[1]));
+      set(1, 0, -.25 * (1 - c[0])); set(1, 1, -.25 * (1 + c[0]));
+      set(1, 2, .25 * (1 + c[0])); set(1, 3, .25 * (1 - c[0]));
+      t.N[q][0] = .25 * (1 - c[0]) * (1 - c[1]); t.N[q][1] = .25 * (1 + c[0]) * (1 - c[1]);
+      t.N[q][2] = .25 * (1 + c[0]) * (1 + c[1]); t.N[q][3] = .25 * (1 - c[0]) * (1 + c[1]);
+      break;
+    case AKX_TETRAHEDRON_4: {
+      c[0] = c[1] = c[2] = 1. / 4.; t.w[q] = 1. / 6.;
+      const double v[3][4] = {{-1, 1, 0, 0}, {-1, 0, 1, 0}, {-1, 0, 0, 1}};
+      for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 4; ++a) set(i, a, v[i][a]);
+      t.N[q][0] = 1 - c[0] - c[1] - c[2]; t.N[q][1] = c[1]; t.N[q][2] = c[2]; t.N[q][3] = c[0];
+    } break;
+    case AKX_HEXAHEDRON_8: {
+      c[0] = (q & 1) ? g : -g; c[1] = (q & 2) ? g : -g; c[2] = (q & 4) ? g : -g; t.w[q] = 1.;
+      const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+      for (int a = 0; a < 8; ++a) {
+        auto f = [&](int s, double x) { return s > 0 ? (1 + x) : (1 - x); };
+        set(0, a, (sx[a] > 0 ? .125 : -.125) * f(sy[a], c[1]) * f(sz[a], c[2]));
+        set(1, a, (sy[a] > 0 ? .125 : -.125) * f(sx[a], c[0]) * f(sz[a], c[2]));
+        set(2, a, (sz[a] > 0 ? .125 : -.125) * f(sx[a], c[0]) * f(sy[a], c[1]));
+        t.N[q][a] = .125 * f(sx[a], c[0]) * f(sy[a], c[1]) * f(sz[a], c[2]);
+      }
+    } break;
+    }
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Morton renumbering
+// ---------------------------------------------------------------------------
+uint64_t spread3(uint64_t v) {
+  v &= 0x1FFFFF;
+  v = (v | (v << 32)) & 0x1F00000000FFFFull;
+  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+uint64_t spread2(uint64_t v) {
+  v &= 0xFFFFFFFF;
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+
+// perm[new] = old, ordered by the Morton key of the points P (n x dim).
+std::vector<uint32_t> morton_order(const std::vector<double>& P, size_t n, int dim,
+                                   const double* lo, const double* hi) {
+  const double levels = dim == 3 ? double((1u << 21) - 1) : double(0xFFFFFFFFu);
+  std::vector<std::pair<uint64_t, uint32_t>> key(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t k = 0;
+    for (int j = 0; j < dim; ++j) {
+      const double span = hi[j] > lo[j] ? hi[j] - lo[j] : 1.;
+      double t = (P[i * dim + j] - lo[j]) / span;
+      t = std::min(1., std::max(0., t));
+      const uint64_t q = uint64_t(t * levels);
+      k |= (dim == 3 ? spread3(q) : spread2(q)) << j;
+    }
+    key[i] = {k, uint32_t(i)};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> perm(n);
+  for (size_t i = 0; i < n; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
+// node -> slot CSR in internal numbering; `order` lists caller element ids in
+// assembly order.
+void build_csr(const akx_ctx* c, const std::vector<uint32_t>& order, std::vector<uint32_t>& off,
+               std::vector<uint32_t>& slot) {
+  const int n = c->n;
+  const int64_t N = c->N;
+  off.assign(N + 1, 0);
+  for (uint32_t e : order)
+    for (int a = 0; a < n; ++a) off[c->node_inv[c->hconn[size_t(e) * n + a]] + 1]++;
+  for (int64_t i = 0; i < N; ++i) off[i + 1] += off[i];
+  slot.assign(off[N], 0);
+  std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+  for (uint32_t e : order) {
+    const uint32_t ei = c->elem_inv[e];
+    for (int a = 0; a < n; ++a) {
+      const uint32_t node = c->node_inv[c->hconn[size_t(e) * n + a]];
+      slot[fill[node]++] = ei * uint32_t(n) + uint32_t(a);
+    }
+  }
+}
+
+void update_params(HostMaterial& m, int dim) {
+  // material_elastic.cc:83-107, material_neohookean.cc:83-89
+  m.lambda = m.nu * m.E / ((1 + m.nu) * (1 - 2 * m.nu));
+  m.mu = m.E / (2 * (1 + m.nu));
+  if (dim == 2 && m.plane_stress && m.kind == AKX_MATERIAL_ELASTIC)
+    m.lambda = m.nu * m.E / ((1 + m.nu) * (1 - m.nu));
+}
+
+size_t field_elems(const akx_ctx* c, int field) {
+  switch (field) {
+  case AKX_GRAD_U:
+  case AKX_STRESS: return size_t(c->E) * c->Q * c->d * c->d;
+  case AKX_DAMAGE:
+  case AKX_JACOBIANS: return size_t(c->E) * c->Q;
+  case AKX_SHAPES_DERIVATIVES: return size_t(c->E) * c->Q * c->d * c->n;
+  default: return size_t(c->N) * c->d;
+  }
+}
+
+double* nodal_ptr(akx_ctx* c, int field) {
+  switch (field) {
+  case AKX_DISPLACEMENT: return c->u;
+  case AKX_VELOCITY: return c->v;
+  case AKX_ACCELERATION: return c->a;
+  case AKX_EXTERNAL_FORCE: return c->fext;
+  case AKX_INTERNAL_FORCE: return c->fint;
+  case AKX_PREVIOUS_DISPLACEMENT: return c->prev;
+  case AKX_DISPLACEMENT_INCREMENT: return c->inc;
+  default: return nullptr;
+  }
+}
+
+// caller (N, d) <-> internal (N, d)
+template <typename T>
+void to_internal_nodal(const akx_ctx* c, const T* src, T* dst) {
+  const int d = c->d;
+  for (int64_t k = 0; k < c->N; ++k)
+    for (int i = 0; i < d; ++i) dst[k * d + i] = src[size_t(c->node_perm[k]) * d + i];
+}
+template <typename T>
+void to_caller_nodal(const akx_ctx* c, const T* src, T* dst) {
+  const int d = c->d;
+  for (int64_t k = 0; k < c->N; ++k)
+    for (int i = 0; i < d; ++i) dst[size_t(c->node_perm[k]) * d + i] = src[k * d + i];
+}
+// per-point arrays with `comp` values per point, points element-major
+template <typename T>
+void to_caller_points(const akx_ctx* c, const T* src, T* dst, int comp) {
+  const size_t per = size_t(c->Q) * comp;
+  for (int64_t e = 0; e < c->E; ++e)
+    std::memcpy(dst + size_t(c->elem_perm[e]) * per, src + size_t(e) * per, per * sizeof(T));
+}
+template <typename T>
+void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
+  const size_t per = size_t(c->Q) * comp;
+  for (int64_t e = 0; e < c->E; ++e)
+    std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
+}
+
+ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
+  ElemArgs A;
+  A.count = g.count;
+  A.filter = g.filter;
+  A.E = c->E;
+  A.conn = c->conn_soa;
+  A.B = precision == AKX_FP64 ? static_cast<const void*>(c->B64) : c->B32;
+  A.J = precision == AKX_FP64 ? static_cast<const void*>(c->J64) : c->J32;
+  A.u = c->u;
+  A.slots = c->slots;
+  A.elem_mat = c->elem_mat_d;
+  A.mats = c->mats_d;
+  A.damage = c->damage;
+  A.Yd = c->Yd;
+  A.xcd_remap = c->xcd_remap;
+  return A;
+}
+
+NodeArgs node_args(akx_ctx* c, int mode) {
+  NodeArgs A;
+  A.N = c->N;
+  A.csr_off = c->csr_off;
+  A.csr_slot = c->csr_slot;
+  A.slots = c->slots;
+  A.u = c->u; A.v = c->v; A.a = c->a;
+  A.fext = c->fext;
+  A.mass = c->mass;
+  A.blocked = c->blocked;
+  A.fint_out = c->fint;
+  A.prev = c->prev;
+  A.inc = c->inc;
+  A.dt = c->dt;
+  for (int i = 0; i < 3; ++i) A.bc_inc[i] = c->bc_inc[i];
+  A.mode = mode;
+  A.xcd_remap = c->xcd_remap;
+  return A;
+}
+
+int check_ready(akx_ctx* c) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->initialized) return fail(c, AKX_ERR_STATE, "akx_init has not been called");
+  return AKX_OK;
+}
+
+int run_elements(akx_ctx* c, int precision) {
+  for (auto& g : c->groups) {
+    ElemArgs A = elem_args(c, g, precision);
+    HIP_TRY(c, launch_element(c->et, g.kind, precision, c->stream, A, c->elem_block));
+  }
+  return AKX_OK;
+}
+
+int read_blocked(akx_ctx* c, std::vector<uint8_t>& internal) {
+  const size_t nd = size_t(c->N) * c->d;
+  internal.resize(nd);
+  HIP_TRY(c, hipMemcpyAsync(internal.data(), c->blocked, nd, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+}  // namespace
+
+/* ======================================================================== */
+extern "C" {
+
+int akx_ctx_create(int device, unsigned flags, akx_ctx** out) {
+  if (!out) return AKX_ERR_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return AKX_ERR_HIP;
+  if (device < 0 || device >= count) return AKX_ERR_ARGUMENT;
+  auto* c = new (std::nothrow) akx_ctx();
+  if (!c) return AKX_ERR_ARGUMENT;
+  c->device = device;
+  c->reorder = !(flags & AKX_CTX_KEEP_NUMBERING);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return AKX_ERR_HIP;
+  }
+  c->own_stream = true;
+  *out = c;
+  return AKX_OK;
+}
+
+int akx_ctx_destroy(akx_ctx* c) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_mesh_buffers(c);
+  for (auto& e : c->pool) (void)hipEventDestroy(e);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AKX_OK;
+}
+
+const char* akx_last_error(const akx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int akx_set_stream(akx_ctx* c, void* s) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  if (s) {
+    c->stream = static_cast<hipStream_t>(s);
+    c->own_stream = false;
+  } else {
+    HIP_TRY(c, hipStreamCreate(&c->stream));
+    c->own_stream = true;
+  }
+  return AKX_OK;
+}
+
+int akx_mesh_set(akx_ctx* c, int et, int dim, int64_t nb_nodes, const double* nodes,
+                 int64_t nb_elements, const uint32_t* conn) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (et < AKX_TRIANGLE_3 || et > AKX_HEXAHEDRON_8)
+    return fail(c, AKX_ERR_ARGUMENT, "unknown element type");
+  if (dim != elem_dim(et)) return fail(c, AKX_ERR_ARGUMENT, "dimension does not match element type");
+  if (nb_nodes <= 0 || nb_elements < 0 || !nodes || (nb_elements > 0 && !conn))
+    return fail(c, AKX_ERR_ARGUMENT, "empty or null mesh arrays");
+  const int n = elem_nodes(et);
+  if (nb_nodes >= int64_t(std::numeric_limits<uint32_t>::max()) ||
+      nb_elements * n >= int64_t(std::numeric_limits<uint32_t>::max()))
+    return fail(c, AKX_ERR_ARGUMENT, "mesh too large for 32-bit slot indices");
+  for (int64_t k = 0; k < nb_elements * n; ++k)
+    if (int64_t(conn[k]) >= nb_nodes)
+      return fail(c, AKX_ERR_ARGUMENT, "connectivity refers to a missing node");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh_buffers(c);
+  c->et = et; c->d = dim; c->n = n; c->Q = elem_qp(et);
+  c->N = nb_nodes; c->E = nb_elements;
+  c->hX.assign(nodes, nodes + nb_nodes * dim);
+  c->hconn.assign(conn, conn + nb_elements * n);
+  c->elem_mat.assign(nb_elements, 0);
+  c->mats.clear();
+  c->has_fext = false;
+  c->has_bc_inc = false;
+  for (double& x : c->bc_inc) x = 0;
+  c->dt = 0;
+  c->mesh_set = true;
+  c->fint_valid = false;
+
+  // numbering
+  c->node_perm.resize(nb_nodes);
+  c->elem_perm.resize(nb_elements);
+  if (c->reorder && nb_elements > 0) {
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int j = 0; j < dim; ++j) {
+      lo[j] = std::numeric_limits<double>::max();
+      hi[j] = -std::numeric_limits<double>::max();
+    }
+    for (int64_t i = 0; i < nb_nodes; ++i)
+      for (int j = 0; j < dim; ++j) {
+        lo[j] = std::min(lo[j], nodes[i * dim + j]);
+        hi[j] = std::max(hi[j], nodes[i * dim + j]);
+      }
+    c->node_perm = morton_order(c->hX, size_t(nb_nodes), dim, lo, hi);
+    std::vector<double> cent(size_t(nb_elements) * dim, 0.);
+    for (int64_t e = 0; e < nb_elements; ++e)
+      for (int a = 0; a < n; ++a)
+        for (int j = 0; j < dim; ++j) cent[e * dim + j] += nodes[size_t(conn[e * n + a]) * dim + j] / n;
+    c->elem_perm = morton_order(cent, size_t(nb_elements), dim, lo, hi);
+  } else {
+    std::iota(c->node_perm.begin(), c->node_perm.end(), 0u);
+    std::iota(c->elem_perm.begin(), c->elem_perm.end(), 0u);
+  }
+  c->node_inv.resize(nb_nodes);
+  c->elem_inv.resize(nb_elements);
+  for (int64_t k = 0; k < nb_nodes; ++k) c->node_inv[c->node_perm[k]] = uint32_t(k);
+  for (int64_t k = 0; k < nb_elements; ++k) c->elem_inv[c->elem_perm[k]] = uint32_t(k);
+
+  const size_t nd = size_t(nb_nodes) * dim;
+  HIP_TRY(c, dalloc(&c->X, nd));
+  HIP_TRY(c, dalloc(&c->u, nd)); HIP_TRY(c, dalloc(&c->v, nd)); HIP_TRY(c, dalloc(&c->a, nd));
+  HIP_TRY(c, dalloc(&c->fext, nd)); HIP_TRY(c, dalloc(&c->fint, nd));
+  HIP_TRY(c, dalloc(&c->prev, nd)); HIP_TRY(c, dalloc(&c->inc, nd));
+  HIP_TRY(c, dalloc(&c->blocked, nd)); HIP_TRY(c, dalloc(&c->mass, size_t(nb_nodes)));
+  HIP_TRY(c, dalloc(&c->work, nd));
+  for (double* p : {c->u, c->v, c->a, c->fext, c->fint, c->prev, c->inc})
+    HIP_TRY(c, hipMemsetAsync(p, 0, nd * sizeof(double), c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->blocked, 0, nd, c->stream));
+  std::vector<double> Xi(nd);
+  to_internal_nodal(c, nodes, Xi.data());
+  HIP_TRY(c, hipMemcpyAsync(c->X, Xi.data(), nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  std::vector<uint32_t> soa(size_t(nb_elements) * n);
+  for (int64_t e = 0; e < nb_elements; ++e) {
+    const size_t eo = c->elem_perm[e];
+    for (int a = 0; a < n; ++a) soa[size_t(a) * nb_elements + e] = c->node_inv[conn[eo * n + a]];
+  }
+  HIP_TRY(c, dalloc(&c->conn_soa, soa.size()));
+  HIP_TRY(c, hipMemcpyAsync(c->conn_soa, soa.data(), soa.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_material_add(akx_ctx* c, int kind, double E, double nu, double rho, int plane_stress,
+                     int* mat_id) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "set the mesh before adding materials");
+  if (kind < AKX_MATERIAL_ELASTIC || kind > AKX_MATERIAL_NEOHOOKEAN_DAMAGE)
+    return fail(c, AKX_ERR_ARGUMENT, "unknown material kind");
+  if (!(rho > 0) || !(E > 0)) return fail(c, AKX_ERR_ARGUMENT, "E and rho must be positive");
+  if (kind != AKX_MATERIAL_ELASTIC && c->d == 2 && plane_stress)
+    return fail(c, AKX_ERR_UNSUPPORTED, "plane-stress finite-strain laws are not supported");
+  if (c->mats.size() >= 65535) return fail(c, AKX_ERR_UNSUPPORTED, "too many materials");
+  HostMaterial m;
+  m.kind = kind; m.E = E; m.nu = nu; m.rho = rho; m.plane_stress = plane_stress;
+  update_params(m, c->d);
+  c->mats.push_back(m);
+  c->initialized = false;
+  if (mat_id) *mat_id = int(c->mats.size()) - 1;
+  return AKX_OK;
+}
+
+int akx_material_damage(akx_ctx* c, int mat_id, double Sd, const double* Yd) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (mat_id < 0 || mat_id >= int(c->mats.size())) return fail(c, AKX_ERR_ARGUMENT, "bad material id");
+  if (c->mats[mat_id].kind != AKX_MATERIAL_NEOHOOKEAN_DAMAGE)
+    return fail(c, AKX_ERR_ARGUMENT, "material has no damage");
+  if (!(Sd > 0) || !Yd) return fail(c, AKX_ERR_ARGUMENT, "Sd must be positive and Yd given");
+  c->mats[mat_id].Sd = Sd;
+  c->mats[mat_id].Yd.assign(Yd, Yd + size_t(c->E) * c->Q);
+  c->initialized = false;
+  return AKX_OK;
+}
+
+int akx_assign_materials(akx_ctx* c, const int32_t* em) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  for (int64_t e = 0; e < c->E; ++e) {
+    const int k = em ? em[e] : 0;
+    if (k < 0 || k >= int(c->mats.size())) return fail(c, AKX_ERR_ARGUMENT, "bad material index");
+    c->elem_mat[e] = k;
+  }
+  c->initialized = false;
+  return AKX_OK;
+}
+
+int akx_init(akx_ctx* c) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  if (c->mats.empty()) return fail(c, AKX_ERR_STATE, "no material");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int64_t E = c->E, N = c->N;
+  const int n = c->n, Q = c->Q, d = c->d, dn = d * n;
+  for (int64_t e = 0; e < E; ++e)
+    if (c->elem_mat[e] >= int(c->mats.size()))
+      return fail(c, AKX_ERR_ARGUMENT, "element assigned to a missing material");
+  free_derived(c);
+
+  // assembly order: material by material (creation order), ascending element
+  // ids within a material (solid_mechanics_model.cc:396-399,
+  // material.cc:190-237), node order within an element.
+  const int nm = int(c->mats.size());
+  std::vector<std::vector<uint32_t>> filt(nm);
+  for (int64_t e = 0; e < E; ++e) filt[c->elem_mat[e]].push_back(uint32_t(e));
+  std::vector<uint32_t> order;
+  order.reserve(E);
+  for (auto& f : filt) order.insert(order.end(), f.begin(), f.end());
+  std::vector<uint32_t> off, slot;
+  build_csr(c, order, off, slot);
+  bool ascending = true;
+  for (int64_t e = 0; e < E && ascending; ++e) ascending = (order[e] == uint32_t(e));
+  HIP_TRY(c, dalloc(&c->csr_off, off.size()));
+  HIP_TRY(c, dalloc(&c->csr_slot, slot.size()));
+  HIP_TRY(c, hipMemcpy(c->csr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->csr_slot, slot.data(), slot.size() * 4, hipMemcpyHostToDevice));
+  if (ascending) {
+    c->mass_off = c->csr_off;
+    c->mass_slot = c->csr_slot;
+  } else {
+    // lumped mass assembles every element in ascending order (assembleFieldLumped)
+    std::vector<uint32_t> asc(E), moff, mslot;
+    std::iota(asc.begin(), asc.end(), 0u);
+    build_csr(c, asc, moff, mslot);
+    HIP_TRY(c, dalloc(&c->mass_off, moff.size()));
+    HIP_TRY(c, dalloc(&c->mass_slot, mslot.size()));
+    HIP_TRY(c, hipMemcpy(c->mass_off, moff.data(), moff.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->mass_slot, mslot.data(), mslot.size() * 4, hipMemcpyHostToDevice));
+  }
+
+  // launch groups: one per constitutive law present (internal element ids)
+  std::vector<int> kinds;
+  for (int k = 0; k < nm; ++k)
+    if (!filt[k].empty() &&
+        std::find(kinds.begin(), kinds.end(), c->mats[k].kind) == kinds.end())
+      kinds.push_back(c->mats[k].kind);
+  for (int kind : kinds) {
+    KindGroup g{kind, 0, nullptr};
+    std::vector<uint32_t> ids;
+    for (int k = 0; k < nm; ++k)
+      if (c->mats[k].kind == kind)
+        for (uint32_t e : filt[k]) ids.push_back(c->elem_inv[e]);
+    std::sort(ids.begin(), ids.end());
+    g.count = int64_t(ids.size());
+    if (kinds.size() > 1) {
+      HIP_TRY(c, dalloc(&g.filter, ids.size()));
+      HIP_TRY(c, hipMemcpy(g.filter, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->groups.push_back(g);
+  }
+
+  // materials
+  std::vector<MatParams> mp(nm);
+  bool any_damage = false;
+  for (int k = 0; k < nm; ++k) {
+    const auto& m = c->mats[k];
+    MatParams p;
+    std::memset(&p, 0, sizeof p);
+    p.lambda = m.lambda; p.mu = m.mu; p.rho = m.rho; p.Sd = m.Sd;
+    p.lambda_f = float(m.lambda); p.mu_f = float(m.mu); p.Sd_f = float(m.Sd);
+    p.inv_Sd_f = m.Sd > 0 ? float(1. / m.Sd) : 0.f;
+    p.kind = m.kind;
+    mp[k] = p;
+    if (m.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
+      any_damage = true;
+      if (m.Yd.size() != size_t(E) * Q)
+        return fail(c, AKX_ERR_STATE, "damage material without thresholds (akx_material_damage)");
+    }
+  }
+  HIP_TRY(c, dalloc(&c->mats_d, nm));
+  HIP_TRY(c, hipMemcpy(c->mats_d, mp.data(), nm * sizeof(MatParams), hipMemcpyHostToDevice));
+  if (nm > 1) {
+    std::vector<uint16_t> em(E);
+    for (int64_t e = 0; e < E; ++e) em[e] = uint16_t(c->elem_mat[c->elem_perm[e]]);
+    HIP_TRY(c, dalloc(&c->elem_mat_d, size_t(E)));
+    HIP_TRY(c, hipMemcpy(c->elem_mat_d, em.data(), size_t(E) * 2, hipMemcpyHostToDevice));
+  }
+  if (any_damage) {
+    std::vector<double> yd(size_t(E) * Q, 0.), ydi(size_t(E) * Q);
+    for (int64_t e = 0; e < E; ++e) {
+      const auto& m = c->mats[c->elem_mat[e]];
+      if (m.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE)
+        for (int q = 0; q < Q; ++q) yd[size_t(e) * Q + q] = m.Yd[size_t(e) * Q + q];
+    }
+    to_internal_points(c, yd.data(), ydi.data(), 1);
+    HIP_TRY(c, dalloc(&c->Yd, ydi.size()));
+    HIP_TRY(c, dalloc(&c->damage, ydi.size()));
+    HIP_TRY(c, hipMemcpy(c->Yd, ydi.data(), ydi.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->damage, 0, ydi.size() * 8));
+  }
+
+  // tables
+  const size_t bq = size_t(E) * Q * dn, jq = size_t(E) * Q;
+  HIP_TRY(c, dalloc(&c->B64, bq)); HIP_TRY(c, dalloc(&c->B32, bq));
+  HIP_TRY(c, dalloc(&c->J64, jq)); HIP_TRY(c, dalloc(&c->J32, jq));
+  HIP_TRY(c, hipMalloc(&c->slots, std::max<size_t>(1, size_t(E) * dn) * sizeof(double)));
+  HIP_TRY(c, dalloc(&c->flag, 1));
+  HIP_TRY(c, hipMemset(c->flag, 0, sizeof(int)));
+  c->partial_blocks = 4096;
+  HIP_TRY(c, dalloc(&c->partial, size_t(c->partial_blocks)));
+  HIP_TRY(c, dalloc(&c->scalar, 4));
+  const ElemTables tab = make_tables(c->et);
+  HIP_TRY(c, launch_precompute(c->et, c->stream, E, N, c->X, c->conn_soa, tab, c->B64, c->J64,
+                               c->B32, c->J32, c->flag));
+  int bad = 0;
+  HIP_TRY(c, hipMemcpyAsync(&bad, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (bad)
+    return fail(c, AKX_ERR_JACOBIAN,
+                "Negative jacobian computed, possible problem in the element node ordering");
+  HIP_TRY(c, launch_mass(c->et, c->stream, N, E, c->mass_off, c->mass_slot, tab, c->J64,
+                         c->elem_mat_d, c->mats_d, c->mass));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->initialized = true;
+  c->fint_valid = false;
+  return AKX_OK;
+}
+
+int akx_field_upload(akx_ctx* c, int field, const void* host, size_t bytes) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  if (!host) return fail(c, AKX_ERR_ARGUMENT, "null host pointer");
+  const size_t nd = size_t(c->N) * c->d;
+  (void)hipSetDevice(c->device);
+  if (field == AKX_BLOCKED_DOFS) {
+    if (bytes != nd) return fail(c, AKX_ERR_ARGUMENT, "blocked: size mismatch");
+    std::vector<uint8_t> cur, h(nd);
+    int rc = read_blocked(c, cur);
+    if (rc) return rc;
+    to_internal_nodal(c, static_cast<const uint8_t*>(host), h.data());
+    // bit0: blocked; bit1 (per-step Dirichlet increment) survives on DOFs that stay blocked
+    for (size_t k = 0; k < nd; ++k) cur[k] = uint8_t(h[k] ? (1 | (cur[k] & 2)) : 0);
+    HIP_TRY(c, hipMemcpy(c->blocked, cur.data(), nd, hipMemcpyHostToDevice));
+    return AKX_OK;
+  }
+  if (field == AKX_DAMAGE) {
+    if (!c->damage) return fail(c, AKX_ERR_STATE, "no damage material");
+    if (bytes != size_t(c->E) * c->Q * 8) return fail(c, AKX_ERR_ARGUMENT, "damage: size mismatch");
+    std::vector<double> tmp(size_t(c->E) * c->Q);
+    to_internal_points(c, static_cast<const double*>(host), tmp.data(), 1);
+    HIP_TRY(c, hipMemcpy(c->damage, tmp.data(), bytes, hipMemcpyHostToDevice));
+    return AKX_OK;
+  }
+  double* p = nodal_ptr(c, field);
+  if (!p || field == AKX_INTERNAL_FORCE || field == AKX_DISPLACEMENT_INCREMENT)
+    return fail(c, AKX_ERR_ARGUMENT, "field cannot be uploaded");
+  if (bytes != nd * 8) return fail(c, AKX_ERR_ARGUMENT, "size mismatch");
+  std::vector<double> tmp(nd);
+  to_internal_nodal(c, static_cast<const double*>(host), tmp.data());
+  HIP_TRY(c, hipMemcpyAsync(p, tmp.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  if (field == AKX_EXTERNAL_FORCE) {
+    c->has_fext = false;
+    for (size_t k = 0; k < nd && !c->has_fext; ++k) c->has_fext = (tmp[k] != 0.);
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_field_download(akx_ctx* c, int field, void* host, size_t bytes) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  if (!host) return fail(c, AKX_ERR_ARGUMENT, "null host pointer");
+  (void)hipSetDevice(c->device);
+  const size_t nd = size_t(c->N) * c->d;
+  if (field == AKX_BLOCKED_DOFS) {
+    if (bytes != nd) return fail(c, AKX_ERR_ARGUMENT, "blocked: size mismatch");
+    std::vector<uint8_t> cur;
+    int rc = read_blocked(c, cur);
+    if (rc) return rc;
+    for (auto& b : cur) b &= 1;
+    to_caller_nodal(c, cur.data(), static_cast<uint8_t*>(host));
+    return AKX_OK;
+  }
+  const size_t ne = field_elems(c, field);
+  if (bytes != ne * 8) return fail(c, AKX_ERR_ARGUMENT, "size mismatch");
+  if (field == AKX_MASS) {
+    if (!c->initialized) return fail(c, AKX_ERR_STATE, "akx_init has not been called");
+    std::vector<double> m(c->N);
+    HIP_TRY(c, hipMemcpyAsync(m.data(), c->mass, c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double* h = static_cast<double*>(host);
+    for (int64_t k = 0; k < c->N; ++k)
+      for (int i = 0; i < c->d; ++i) h[size_t(c->node_perm[k]) * c->d + i] = m[k];
+    return AKX_OK;
+  }
+  std::vector<double> tmp(ne);
+  if (field == AKX_CURRENT_POSITION) {
+    HIP_TRY(c, launch_add(c->stream, int64_t(nd), c->X, c->u, c->work));
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), c->work, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    to_caller_nodal(c, tmp.data(), static_cast<double*>(host));
+    return AKX_OK;
+  }
+  if (field == AKX_DAMAGE) {
+    if (!c->damage) return fail(c, AKX_ERR_STATE, "no damage material");
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), c->damage, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    to_caller_points(c, tmp.data(), static_cast<double*>(host), 1);
+    return AKX_OK;
+  }
+  if (field == AKX_SHAPES_DERIVATIVES || field == AKX_JACOBIANS) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const int comp = field == AKX_JACOBIANS ? 1 : c->d * c->n;
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), field == AKX_JACOBIANS ? c->J64 : c->B64, bytes,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> pts(ne);
+    const int64_t E = c->E;
+    for (int64_t e = 0; e < E; ++e)
+      for (int q = 0; q < c->Q; ++q)
+        for (int k = 0; k < comp; ++k)
+          pts[(size_t(e) * c->Q + q) * comp + k] = tmp[(size_t(q) * comp + k) * E + e];
+    to_caller_points(c, pts.data(), static_cast<double*>(host), comp);
+    return AKX_OK;
+  }
+  if (field == AKX_GRAD_U || field == AKX_STRESS) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->gradu) {
+      HIP_TRY(c, dalloc(&c->gradu, ne));
+      HIP_TRY(c, dalloc(&c->stress, ne));
+    }
+    for (auto& g : c->groups) {
+      ElemArgs A = elem_args(c, g, AKX_FP64);
+      HIP_TRY(c, launch_point_fields(c->et, g.kind, c->stream, A, c->gradu, c->stress));
+    }
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), field == AKX_GRAD_U ? c->gradu : c->stress, bytes,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    to_caller_points(c, tmp.data(), static_cast<double*>(host), c->d * c->d);
+    return AKX_OK;
+  }
+  double* p = nodal_ptr(c, field);
+  if (!p) return fail(c, AKX_ERR_ARGUMENT, "unknown field");
+  HIP_TRY(c, hipMemcpyAsync(tmp.data(), p, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  to_caller_nodal(c, tmp.data(), static_cast<double*>(host));
+  return AKX_OK;
+}
+
+int akx_field_device_pointer(akx_ctx* c, int field, void** ptr) {
+  if (!c || !ptr) return AKX_ERR_ARGUMENT;
+  if (field == AKX_BLOCKED_DOFS) { *ptr = c->blocked; return AKX_OK; }
+  if (field == AKX_MASS) { *ptr = c->mass; return AKX_OK; }
+  double* p = nodal_ptr(c, field);
+  if (!p) return fail(c, AKX_ERR_ARGUMENT, "field has no nodal device buffer");
+  *ptr = p;
+  return AKX_OK;
+}
+
+int akx_node_numbering(akx_ctx* c, int64_t* internal_to_caller) {
+  if (!c || !internal_to_caller) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  for (int64_t k = 0; k < c->N; ++k) internal_to_caller[k] = c->node_perm[k];
+  return AKX_OK;
+}
+
+int akx_set_time_step(akx_ctx* c, double dt) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(c, AKX_ERR_ARGUMENT, "time step must be > 0");
+  c->dt = dt;
+  return AKX_OK;
+}
+
+int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (precision != AKX_FP64 && precision != AKX_MIXED)
+    return fail(c, AKX_ERR_ARGUMENT, "unknown precision");
+  if (!(c->dt > 0)) return fail(c, AKX_ERR_STATE, "time step not set");
+  if (nsteps <= 0) return AKX_OK;
+  (void)hipSetDevice(c->device);
+  const int d = c->d;
+  const bool timing = c->timing;
+  if (timing) {
+    const size_t need = 2 * size_t(nsteps) + 1;
+    while (c->pool.size() < need) {
+      hipEvent_t e;
+      HIP_TRY(c, hipEventCreate(&e));
+      c->pool.push_back(e);
+    }
+  }
+  // predictor of step 1
+  {
+    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0));
+    HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
+  }
+  for (int s = 1; s <= nsteps; ++s) {
+    const bool last = (s == nsteps);
+    if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
+    rc = run_elements(c, precision);
+    if (rc) return rc;
+    if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
+    int mode = NODE_CORRECT;
+    if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
+    else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
+    NodeArgs A = node_args(c, mode);
+    HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
+    if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
+  }
+  if (timing) {
+    // element kernel of step s: pool[2s-2] -> pool[2s-1]; node kernel: pool[2s-1] -> pool[2s]
+    HIP_TRY(c, hipEventSynchronize(c->pool[2 * nsteps]));
+    double el = 0., nd = 0.;
+    for (int s = 1; s <= nsteps; ++s) {
+      float a = 0.f, b = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&a, c->pool[2 * s - 2], c->pool[2 * s - 1]));
+      HIP_TRY(c, hipEventElapsedTime(&b, c->pool[2 * s - 1], c->pool[2 * s]));
+      el += a;
+      nd += b;
+    }
+    c->last_elem_ms = el / nsteps;
+    c->last_node_ms = nd / nsteps;
+  }
+  c->fint_valid = true;
+  return AKX_OK;
+}
+
+int akx_assemble_internal_forces(akx_ctx* c, int precision) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (precision != AKX_FP64 && precision != AKX_MIXED)
+    return fail(c, AKX_ERR_ARGUMENT, "unknown precision");
+  (void)hipSetDevice(c->device);
+  rc = run_elements(c, precision);
+  if (rc) return rc;
+  NodeArgs A = node_args(c, NODE_GATHER | NODE_STORE_FINT);
+  HIP_TRY(c, launch_node(c->d, precision, c->has_fext, c->stream, A, c->node_block));
+  c->fint_valid = true;
+  return AKX_OK;
+}
+
+int akx_stable_time_step(akx_ctx* c, double* dt) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!dt) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, launch_stable_dt(c->et, c->stream, c->E, c->X, c->u, c->conn_soa, c->elem_mat_d,
+                              c->mats_d, c->partial, c->partial_blocks, c->scalar));
+  HIP_TRY(c, hipMemcpyAsync(dt, c->scalar, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_energy(akx_ctx* c, int kind, double* value) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!value) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  if (kind == AKX_ENERGY_KINETIC || kind == AKX_ENERGY_EXTERNAL_WORK) {
+    HIP_TRY(c, launch_energy(kind, c->et, 0, c->stream, c->N, c->E, c->u, c->v, c->mass, c->fext,
+                             c->fint, c->blocked, c->conn_soa, c->B64, c->J64, c->elem_mat_d,
+                             c->mats_d, c->damage, nullptr, 0, c->partial, c->partial_blocks,
+                             c->scalar));
+    double r = 0;
+    HIP_TRY(c, hipMemcpyAsync(&r, c->scalar, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *value = kind == AKX_ENERGY_KINETIC ? r * .5 : r * c->dt;
+    return AKX_OK;
+  }
+  if (kind != AKX_ENERGY_POTENTIAL) return fail(c, AKX_ERR_ARGUMENT, "unknown energy");
+  double total = 0.;
+  for (const auto& g : c->groups) {
+    HIP_TRY(c, launch_energy(kind, c->et, g.kind, c->stream, c->N, c->E, c->u, c->v, c->mass,
+                             c->fext, c->fint, c->blocked, c->conn_soa, c->B64, c->J64,
+                             c->elem_mat_d, c->mats_d, c->damage, g.filter, g.count, c->partial,
+                             c->partial_blocks, c->scalar + 1));
+    double r = 0;
+    HIP_TRY(c, hipMemcpyAsync(&r, c->scalar + 1, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    total += r;
+  }
+  *value = total;
+  return AKX_OK;
+}
+
+int akx_set_dirichlet_increment(akx_ctx* c, int64_t nb, const int64_t* nodes, int axis,
+                                double increment) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!c->mesh_set) return fail(c, AKX_ERR_STATE, "no mesh");
+  if (axis < 0 || axis >= c->d) return fail(c, AKX_ERR_ARGUMENT, "bad axis");
+  if (nb < 0 || (nb > 0 && !nodes)) return fail(c, AKX_ERR_ARGUMENT, "bad node list");
+  if (c->has_bc_inc && c->bc_inc[axis] != 0. && c->bc_inc[axis] != increment)
+    return fail(c, AKX_ERR_UNSUPPORTED, "one increment value per axis");
+  for (int64_t k = 0; k < nb; ++k)
+    if (nodes[k] < 0 || nodes[k] >= c->N) return fail(c, AKX_ERR_ARGUMENT, "node out of range");
+  (void)hipSetDevice(c->device);
+  std::vector<uint8_t> cur;
+  int rc = read_blocked(c, cur);
+  if (rc) return rc;
+  for (int64_t k = 0; k < nb; ++k) cur[size_t(c->node_inv[nodes[k]]) * c->d + axis] = 3;
+  HIP_TRY(c, hipMemcpy(c->blocked, cur.data(), cur.size(), hipMemcpyHostToDevice));
+  c->bc_inc[axis] = increment;
+  c->has_bc_inc = true;
+  return AKX_OK;
+}
+
+int akx_synchronize(akx_ctx* c) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_enable_timing(akx_ctx* c, int on) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  c->timing = on != 0;
+  return AKX_OK;
+}
+
+int akx_last_step_timing(akx_ctx* c, double* element_ms, double* node_ms) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (element_ms) *element_ms = c->last_elem_ms;
+  if (node_ms) *node_ms = c->last_node_ms;
+  return AKX_OK;
+}
+
+int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
+  if (!ok(element_block) || !ok(node_block))
+    return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
+  if (element_block) c->elem_block = element_block;
+  if (node_block) c->node_block = node_block;
+  if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;
+  return AKX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// akx_internal.hpp -- host-side context of libakx.so and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "akx_device.hpp"
+
+namespace akx {
+
+struct HostMaterial {
+  int kind = ELASTIC;
+  double E = 0, nu = 0, rho = 0, Sd = 0;
+  int plane_stress = 0;
+  double lambda = 0, mu = 0;
+  std::vector<double> Yd;  // per point of the whole mesh (only own points used)
+};
+
+// A contiguous launch of the element kernel: one constitutive law.
+struct KindGroup {
+  int kind;
+  int64_t count;        // elements in the group
+  uint32_t* filter;     // device element ids, nullptr = all elements 0..E-1
+};
+
+// Arguments of the element kernels.
+struct ElemArgs {
+  int64_t count;             // threads' elements
+  const uint32_t* filter;    // nullptr = identity
+  int64_t E;                 // total elements (SoA stride)
+  const uint32_t* conn;      // [a][E]
+  const void* B;             // [(q*dn+k)][E], double or float
+  const void* J;             // [q][E], double or float
+  const double* u;           // (N, d)
+  void* slots;               // [e][dn], double or float
+  const uint16_t* elem_mat;  // nullptr = material 0
+  const MatParams* mats;
+  double* damage;            // [e*Q + q]
+  const double* Yd;          // [e*Q + q]
+  int xcd_remap;
+};
+
+struct NodeArgs {
+  int64_t N;
+  const uint32_t* csr_off;
+  const uint32_t* csr_slot;
+  const void* slots;
+  double* u;
+  double* v;
+  double* a;
+  const double* fext;
+  const double* mass;     // one value per node
+  const uint8_t* blocked; // bit0 blocked, bit1 per-step Dirichlet increment
+  double* fint_out;
+  double* prev;
+  double* inc;
+  double dt;
+  double bc_inc[3];
+  int mode;
+  int xcd_remap;
+};
+
+enum NodeMode {
+  NODE_CORRECT = 1,     // gather forces, residual, lumped solve, corrector
+  NODE_PREDICT = 2,     // Dirichlet increments, save previous, predictor
+  NODE_STORE_FINT = 4,  // write the assembled internal force
+  NODE_SAVE_PREV = 8,   // previous = u before the predictor
+  NODE_STORE_INC = 16,  // increment = u - previous after the corrector
+  NODE_GATHER = 32      // gather forces only (with NODE_STORE_FINT)
+};
+
+// Launchers (akx_kernels.hip).  All enqueue on `stream` and return the launch status.
+hipError_t launch_precompute(int et, hipStream_t s, int64_t E, int64_t N, const double* X,
+                             const uint32_t* conn_soa, const ElemTables& tab, double* B64,
+                             double* J64, float* B32, float* J32, int* bad_jacobian);
+hipError_t launch_mass(int et, hipStream_t s, int64_t N, int64_t E, const uint32_t* off,
+                       const uint32_t* slot, const ElemTables& tab, const double* J64,
+                       const uint16_t* elem_mat, const MatParams* mats, double* mass);
+hipError_t launch_element(int et, int kind, int precision, hipStream_t s, const ElemArgs& A,
+                          int block);
+hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
+                       int block);
+hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
+                            const uint32_t* conn_soa, const uint16_t* elem_mat,
+                            const MatParams* mats, double* partial, int nblocks_max,
+                            double* out);
+hipError_t launch_energy(int kind_energy, int et, int mat_kind_uniform, hipStream_t s,
+                         int64_t N, int64_t E, const double* u, const double* v,
+                         const double* mass, const double* fext, const double* fint,
+                         const uint8_t* blocked, const uint32_t* conn_soa, const double* B64,
+                         const double* J64, const uint16_t* elem_mat, const MatParams* mats,
+                         const double* damage, const uint32_t* filter, int64_t count,
+                         double* partial, int nblocks_max, double* out);
+hipError_t launch_point_fields(int et, int kind, hipStream_t s, const ElemArgs& A, double* gradu,
+                               double* stress);
+hipError_t launch_add(hipStream_t s, int64_t n, const double* x, const double* y, double* out);
+
+}  // namespace akx
+
+// The context behind the opaque handle of include/akx.h.
+struct akx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  // mesh
+  int et = -1, d = 0, n = 0, Q = 0;
+  int64_t N = 0, E = 0;
+  std::vector<double> hX;
+  std::vector<uint32_t> hconn;
+
+  // materials
+  std::vector<akx::HostMaterial> mats;
+  std::vector<int32_t> elem_mat;
+  std::vector<akx::KindGroup> groups;
+
+  // device buffers
+  double* X = nullptr;
+  uint32_t* conn_soa = nullptr;
+  double* B64 = nullptr;
+  double* J64 = nullptr;
+  float* B32 = nullptr;
+  float* J32 = nullptr;
+  double *u = nullptr, *v = nullptr, *a = nullptr, *fext = nullptr, *fint = nullptr;
+  double* mass = nullptr;  // per node
+  double *prev = nullptr, *inc = nullptr;
+  uint8_t* blocked = nullptr;
+  uint32_t *csr_off = nullptr, *csr_slot = nullptr;
+  uint32_t *mass_off = nullptr, *mass_slot = nullptr;  // element-ascending order
+  void* slots = nullptr;
+  uint16_t* elem_mat_d = nullptr;
+  akx::MatParams* mats_d = nullptr;
+  double* damage = nullptr;
+  double* Yd = nullptr;
+  double* partial = nullptr;
+  double* scalar = nullptr;
+  int partial_blocks = 0;
+  double* gradu = nullptr;
+  double* stress = nullptr;
+  double* work = nullptr;  // N*d scratch (current position)
+  int* flag = nullptr;
+
+  bool has_fext = false;
+  bool has_bc_inc = false;
+  double bc_inc[3] = {0, 0, 0};
+  double dt = 0.;
+  bool mesh_set = false, initialized = false;
+  bool fint_valid = false;
+
+  int elem_block = 256, node_block = 256, xcd_remap = 1;
+  // internal numbering: perm[internal] = caller id, inv[caller id] = internal
+  bool reorder = true;
+  std::vector<uint32_t> node_perm, node_inv, elem_perm, elem_inv;
+
+  std::vector<hipEvent_t> pool;  // per-step timing events
+  bool timing = false;
+  double last_elem_ms = 0, last_node_ms = 0;
+};

@@ -1,0 +1,801 @@
+// akx_kernels.hip -- HIP kernels of the explicit central-difference step (gfx950).
+//
+// Per step two kernels run:
+//   element kernel  (one thread per element): gather u of the element's nodes,
+//     per quadrature point grad u -> stress -> B^T sigma, integrate with
+//     det(J) w, write the element's n*d force "slots" (AoS per element);
+//   node kernel     (one thread per node): gather the node's slots through the
+//     node -> slot CSR in assembly order (no atomics: fixed summation order),
+//     residual + lumped solve + corrector of this step, then Dirichlet
+//     increments + predictor of the next step.
+// Both are bandwidth-bound (a few flops per byte); the tables are SoA so that
+// consecutive threads read consecutive addresses.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "akx_internal.hpp"
+
+namespace akx {
+
+// ===========================================================================
+// precompute: B = J^{-1} dN/dxi and det(J) w
+//   ShapeLagrange::computeShapeDerivativesOnIntegrationPoints
+//   (shape_lagrange_inline_impl.hh:204-252, :62-82; element_class_tmpl.hh:282-355),
+//   IntegratorGauss::computeJacobiansOnIntegrationPoints + weights
+//   (integrator_gauss_inline_impl.hh:208-252, 396-423)
+// ===========================================================================
+template <int ET>
+__global__ void __launch_bounds__(256) k_precompute(int64_t E, const double* __restrict__ X,
+                                                    const uint32_t* __restrict__ conn, ElemTables tab,
+                                                    double* __restrict__ B64, double* __restrict__ J64,
+                                                    float* __restrict__ B32, float* __restrict__ J32,
+                                                    int* bad) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double xe[dn];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    const int64_t node = conn[a * E + e];
+#pragma unroll
+    for (int i = 0; i < d; ++i) xe[i + a * d] = X[node * d + i];
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    double J[d * d], Ji[d * d], B[dn];
+    matrix_matrixt<d, d, n>(tab.dnds[q], xe, J);
+    invn<d>(J, Ji);
+    matrix_matrix<d, n, d>(Ji, tab.dnds[q], B);
+    const double jw = detn<d>(J) * tab.w[q];
+#pragma unroll
+    for (int k = 0; k < dn; ++k) {
+      B64[(int64_t(q) * dn + k) * E + e] = B[k];
+      B32[(int64_t(q) * dn + k) * E + e] = float(B[k]);
+    }
+    J64[int64_t(q) * E + e] = jw;
+    J32[int64_t(q) * E + e] = float(jw);
+    if (jw < 0) *bad = 1;
+  }
+}
+
+// ===========================================================================
+// lumped mass by row sum (SolidMechanicsModel::assembleMassLumped,
+// solid_mechanics_model_mass.cc:58-102 -> assembleLumpedRowSum,
+// fe_engine_template_tmpl_field.hh:165-199, computeNtb
+// shape_lagrange_inline_impl.hh:495-533): per node, in element-ascending order,
+//   m += sum_q (rho N_a(q)) J_q
+// ===========================================================================
+template <int ET>
+__global__ void __launch_bounds__(256) k_mass(int64_t N, int64_t E, const uint32_t* __restrict__ off,
+                                              const uint32_t* __restrict__ slot, ElemTables tab,
+                                              const double* __restrict__ J64,
+                                              const uint16_t* __restrict__ elem_mat,
+                                              const MatParams* __restrict__ mats,
+                                              double* __restrict__ mass) {
+  using T = Elem<ET>;
+  constexpr int n = T::n, Q = T::Q;
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= N) return;
+  double local = 0.;
+  for (uint32_t p = off[node]; p < off[node + 1]; ++p) {
+    const uint32_t s = slot[p];
+    const int64_t e = s / n;
+    const int a = s % n;
+    const double rho = mats[elem_mat ? elem_mat[e] : 0].rho;
+    double m = 0.;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double ntb = 0.;
+      ntb += rho * tab.N[q][a];
+      m += ntb * J64[int64_t(q) * E + e];
+    }
+    local += m;
+  }
+  double r = 0.;
+  r += local;
+  mass[node] = r;
+}
+
+// ===========================================================================
+// element kernel, FP64, reference operation order
+// ===========================================================================
+template <int ET, int KIND>
+__global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
+  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t E = A.E;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+  const double* __restrict__ Bt = static_cast<const double*>(A.B);
+  const double* __restrict__ Jt = static_cast<const double*>(A.J);
+  // FEEngine::extractNodalToElementField (fe_engine_inline_impl.hh:104-143)
+  double ue[dn];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    const int64_t node = A.conn[a * E + e];
+#pragma unroll
+    for (int i = 0; i < d; ++i) ue[i + a * d] = A.u[node * d + i];
+  }
+  double fe[dn];
+#pragma unroll
+  for (int k = 0; k < dn; ++k) fe[k] = 0.;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    double Bq[dn];
+#pragma unroll
+    for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+    double G[d * d], S[d * d], R[dn];
+    // grad u = u_e B^T (shape_functions_inline_impl.hh:388, Matrix::mul<false, true>)
+    matrix_matrixt<d, d, n>(ue, Bq, G);
+    if constexpr (KIND == ELASTIC) {
+      elastic_stress_ref<d>(m, G, S);
+      // B^T sigma computed as sigma B (shape_lagrange_inline_impl.hh:421)
+      matrix_matrix<d, n, d>(S, Bq, R);
+    } else {
+      if constexpr (KIND == NEOHOOKEAN) {
+        neohookean_stress_ref<d>(m, G, S);
+      } else {
+        const int64_t eq = e * Q + q;
+        double dam = A.damage[eq];
+        nh_damage_stress_ref<d>(m, G, S, dam, A.Yd[eq]);
+        A.damage[eq] = dam;
+      }
+      finite_residual_ref<d, n>(Bq, G, S, R);
+    }
+    // IntegratorGauss::integrate (integrator_gauss_inline_impl.hh:446-452)
+    const double jq = Jt[int64_t(q) * E + e];
+#pragma unroll
+    for (int k = 0; k < dn; ++k) fe[k] += R[k] * jq;
+  }
+  double* __restrict__ out = static_cast<double*>(A.slots) + e * dn;
+#pragma unroll
+  for (int k = 0; k < dn; ++k) out[k] = fe[k];
+}
+
+// ===========================================================================
+// element kernel, FP32 element math (mixed precision)
+// ===========================================================================
+template <int ET, int KIND>
+__global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
+#pragma clang fp contract(fast)
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
+  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t E = A.E;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+  const float* __restrict__ Bt = static_cast<const float*>(A.B);
+  const float* __restrict__ Jt = static_cast<const float*>(A.J);
+  // displacement differences to node 0, formed in FP64 then rounded: sum_a B_a = 0
+  // (partition of unity), so grad u = sum_{a>0} (u_a - u_0) B_a^T exactly.
+  int64_t nodes[n];
+#pragma unroll
+  for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
+  double u0[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) u0[i] = A.u[nodes[0] * d + i];
+  float du[dn];
+#pragma unroll
+  for (int i = 0; i < d; ++i) du[i] = 0.f;
+#pragma unroll
+  for (int a = 1; a < n; ++a)
+#pragma unroll
+    for (int i = 0; i < d; ++i) du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
+  float fe[dn];
+#pragma unroll
+  for (int k = 0; k < dn; ++k) fe[k] = 0.f;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    float Bq[dn];
+#pragma unroll
+    for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+    const float jq = Jt[int64_t(q) * E + e];
+    float G[d * d];
+#pragma unroll
+    for (int j = 0; j < d; ++j)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float s = 0.f;
+#pragma unroll
+        for (int a = 1; a < n; ++a) s += du[i + a * d] * Bq[j + a * d];
+        G[i + j * d] = s;
+      }
+    float P[d * d];  // first Piola-Kirchhoff (Cauchy at small strain) times det(J) w
+    if constexpr (KIND == ELASTIC) {
+      float tr = 0.f;
+#pragma unroll
+      for (int i = 0; i < d; ++i) tr += G[i + i * d];
+#pragma unroll
+      for (int i = 0; i < d; ++i)
+#pragma unroll
+        for (int j = 0; j < d; ++j) {
+          float s = m.mu_f * (G[i + j * d] + G[j + i * d]);
+          if (i == j) s += m.lambda_f * tr;
+          P[i + j * d] = s * jq;
+        }
+    } else {
+      float S[d * d];
+      float W;
+      neohookean_stress_f32<d>(m.lambda_f, m.mu_f, G, S, KIND == NH_DAMAGE ? &W : nullptr);
+      float scale = jq;
+      if constexpr (KIND == NH_DAMAGE) {
+        const int64_t eq = e * Q + q;
+        double dam = A.damage[eq];
+        const double Fd = double(W) - A.Yd[eq] - m.Sd * dam;
+        if (Fd > 0) dam = (double(W) - A.Yd[eq]) / m.Sd;
+        dam = fmin(dam, 1.);
+        A.damage[eq] = dam;
+        scale *= float(1. - dam);
+      }
+      // P = F S = S + G S
+#pragma unroll
+      for (int i = 0; i < d; ++i)
+#pragma unroll
+        for (int j = 0; j < d; ++j) {
+          float s = S[i + j * d];
+#pragma unroll
+          for (int l = 0; l < d; ++l) s += G[i + l * d] * S[l + j * d];
+          P[i + j * d] = s * scale;
+        }
+    }
+    // f_a,i += sum_j P_ij dN_a/dX_j
+#pragma unroll
+    for (int a = 0; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float s = fe[i + a * d];
+#pragma unroll
+        for (int j = 0; j < d; ++j) s += P[i + j * d] * Bq[j + a * d];
+        fe[i + a * d] = s;
+      }
+  }
+  float* __restrict__ out = static_cast<float*>(A.slots) + e * dn;
+#pragma unroll
+  for (int k = 0; k < dn; ++k) out[k] = fe[k];
+}
+
+// ===========================================================================
+// node kernel: force gather (CSR, assembly order) + residual + lumped solve +
+// corrector of step n, then Dirichlet increments + predictor of step n+1.
+//   residual: r = 0; r += 1 f_ext; r += 1 f_int; r += M (-1 a)
+//     (TimeStepSolver::assembleResidual, solid_mechanics_model.cc:279-293,
+//      dof_manager_default.cc:271-287)
+//   solve:    delta = 1 (r / M)           (non_linear_solver_lumped.cc:85-98)
+//   correct:  u += 0 delta; v += (dt/2) delta; a += delta   (newmark-beta.cc:176-210)
+//   predict:  u += dt v + (dt/2)(dt a); v += dt a           (newmark-beta.cc:60-89)
+// f_int[node] = sum over the node's slots of -fe (DOFManager::
+// assembleElementalArrayLocalArray scale -1, dof_manager.cc:113-134).
+// FP64 slots: plain sum in slot order (bit-identical to the sequential
+// scatter).  FP32 slots: FP64 Neumaier-compensated sum.
+// ===========================================================================
+template <int D, typename TS, bool HAS_FEXT>
+__global__ void __launch_bounds__(256) k_node(NodeArgs A) {
+  const int64_t node = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
+  if (node >= A.N) return;
+  const int mode = A.mode;
+  const double dt = A.dt;
+  if (mode & (NODE_CORRECT | NODE_GATHER)) {
+    const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
+    double f[D], c[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) { f[i] = 0.; c[i] = 0.; }
+    const uint32_t p0 = A.csr_off[node], p1 = A.csr_off[node + 1];
+    auto add = [&](int i, TS raw) {
+      const double x = -1. * double(raw);
+      if constexpr (sizeof(TS) == 8) {
+        f[i] += x;
+      } else {
+        const double tsum = f[i] + x;
+        c[i] += (fabs(f[i]) >= fabs(x)) ? (f[i] - tsum) + x : (x - tsum) + f[i];
+        f[i] = tsum;
+      }
+    };
+    // Batches of CHUNK entries: all loads of a batch are issued before the
+    // (in-order) accumulation, so the gathers overlap instead of serialising.
+    constexpr int CHUNK = 8;
+    uint32_t p = p0;
+    for (; p + CHUNK <= p1; p += CHUNK) {
+      uint32_t s[CHUNK];
+#pragma unroll
+      for (int j = 0; j < CHUNK; ++j) s[j] = A.csr_slot[p + j];
+      TS x[CHUNK][D];
+#pragma unroll
+      for (int j = 0; j < CHUNK; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
+#pragma unroll
+      for (int j = 0; j < CHUNK; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) add(i, x[j][i]);
+    }
+    for (; p < p1; ++p) {
+      const int64_t s = A.csr_slot[p];
+#pragma unroll
+      for (int i = 0; i < D; ++i) add(i, slots[s * D + i]);
+    }
+    const double M = A.mass[node];
+    const double dcoef = 0.5 * dt;  // beta dt
+    const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int64_t k = node * D + i;
+      double fi = f[i];
+      if constexpr (sizeof(TS) != 8) fi = f[i] + c[i];
+      double r = 0.;
+      if constexpr (HAS_FEXT) r += A.fext[k];
+      r += fi;
+      double cache = 0.;
+      cache += -1. * A.a[k];
+      r += M * cache;
+      if (mode & NODE_STORE_FINT) A.fint_out[k] = fi;
+      if (!(mode & NODE_CORRECT)) continue;
+      if (!(A.blocked[k] & 1)) {
+        const double delta = r / M;
+        A.u[k] += ecoef * delta;
+        A.v[k] += dcoef * delta;
+        A.a[k] += delta;
+      }
+      if (mode & NODE_STORE_INC) {
+        double inc = A.u[k];
+        inc -= A.prev[k];
+        A.inc[k] = inc;
+      }
+    }
+  }
+  if (mode & NODE_PREDICT) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int64_t k = node * D + i;
+      const uint8_t b = A.blocked[k];
+      double u = A.u[k];
+      if (b & 2) u += A.bc_inc[i];
+      if (mode & NODE_SAVE_PREV) A.prev[k] = u;
+      if (!(b & 1)) {
+        const double v = A.v[k], acc = A.a[k];
+        const double dt_a_n = dt * acc;
+        u += dt * v + 0.5 * dt * dt_a_n;
+        A.v[k] = v + dt_a_n;
+      }
+      A.u[k] = u;
+    }
+  }
+}
+
+// ===========================================================================
+// stable time step: min_e inradius(X + u) / c_p   (solid_mechanics_model.cc:495-533)
+// ===========================================================================
+__device__ __forceinline__ double dist2d(const double* x, const double* y) {
+  return sqrt((y[0] - x[0]) * (y[0] - x[0]) + (y[1] - x[1]) * (y[1] - x[1]));
+}
+__device__ __forceinline__ double dist3d(const double* x, const double* y) {
+  return sqrt((y[0] - x[0]) * (y[0] - x[0]) + (y[1] - x[1]) * (y[1] - x[1]) +
+              (y[2] - x[2]) * (y[2] - x[2]));
+}
+// aka_math_tmpl.hh:568-587
+__device__ double triangle_inradius(const double* c1, const double* c2, const double* c3) {
+  double a = dist2d(c1, c2), b = dist2d(c2, c3), c = dist2d(c1, c3);
+  double s = (a + b + c) * 0.5;
+  return sqrt((s - a) * (s - b) * (s - c) / s);
+}
+// aka_math_tmpl.hh:596-679
+__device__ double tet_inradius(const double* c1, const double* c2, const double* c3,
+                               const double* c4) {
+  double l12 = dist3d(c1, c2), l13 = dist3d(c1, c3), l14 = dist3d(c1, c4);
+  double l23 = dist3d(c2, c3), l24 = dist3d(c2, c4), l34 = dist3d(c3, c4);
+  double s1 = (l12 + l23 + l13) * 0.5; s1 = sqrt(s1 * (s1 - l12) * (s1 - l23) * (s1 - l13));
+  double s2 = (l12 + l24 + l14) * 0.5; s2 = sqrt(s2 * (s2 - l12) * (s2 - l24) * (s2 - l14));
+  double s3 = (l23 + l34 + l24) * 0.5; s3 = sqrt(s3 * (s3 - l23) * (s3 - l34) * (s3 - l24));
+  double s4 = (l13 + l34 + l14) * 0.5; s4 = sqrt(s4 * (s4 - l13) * (s4 - l34) * (s4 - l14));
+  double xx[9], vol;
+  auto put = [&](const double* p, const double* q, const double* r) {
+    xx[0] = p[0]; xx[1] = p[1]; xx[2] = p[2];
+    xx[3] = q[0]; xx[4] = q[1]; xx[5] = q[2];
+    xx[6] = r[0]; xx[7] = r[1]; xx[8] = r[2];
+  };
+  put(c2, c3, c4); vol = det3(xx);
+  put(c1, c3, c4); vol -= det3(xx);
+  put(c1, c2, c4); vol += det3(xx);
+  put(c1, c2, c3); vol -= det3(xx);
+  vol /= 6;
+  return 3 * vol / (s1 + s2 + s3 + s4);
+}
+
+template <int ET>
+__device__ double element_h(const double* x) {
+  if constexpr (ET == TRI3) return 2. * triangle_inradius(x, x + 2, x + 4);
+  if constexpr (ET == TET4) return 2. * tet_inradius(x, x + 3, x + 6, x + 9);
+  if constexpr (ET == QUAD4) {
+    // element_class_quadrangle_4_inline_impl.hh:131-154
+    double a = dist2d(x + 0, x + 2), b = dist2d(x + 2, x + 4), c = dist2d(x + 4, x + 6),
+           d = dist2d(x + 6, x + 0);
+    return fmin(fmin(a, b), fmin(c, d));
+  }
+  if constexpr (ET == HEX8) {
+    // element_class_hexahedron_8_inline_impl.hh:218-246
+    const int ed[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                           {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    double p = dist3d(x, x + 3);
+    for (int k = 1; k < 12; ++k) p = fmin(p, dist3d(x + 3 * ed[k][0], x + 3 * ed[k][1]));
+    return p;
+  }
+  return 0.;
+}
+
+template <typename Op>
+__device__ __forceinline__ double block_reduce(double v, Op op, double init) {
+  __shared__ double sh[32];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_down(v, o, 64));
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  v = (threadIdx.x < nw) ? sh[threadIdx.x] : init;
+  if (wid == 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_down(v, o, 64));
+  }
+  return v;
+}
+
+struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };
+
+template <int ET>
+__global__ void __launch_bounds__(256) k_stable_dt(int64_t E, const double* __restrict__ X,
+                                                   const double* __restrict__ u,
+                                                   const uint32_t* __restrict__ conn,
+                                                   const uint16_t* __restrict__ elem_mat,
+                                                   const MatParams* __restrict__ mats,
+                                                   double* partial) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n;
+  double best = 1.7976931348623157e308;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    double x[d * n];
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      const int64_t node = conn[a * E + e];
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        double p = X[node * d + i];
+        p += u[node * d + i];
+        x[i + a * d] = p;
+      }
+    }
+    const MatParams m = mats[elem_mat ? elem_mat[e] : 0];
+    const double c = sqrt((m.lambda + 2 * m.mu) / m.rho);
+    best = fmin(best, element_h<ET>(x) / c);
+  }
+  best = block_reduce(best, OpMin(), 1.7976931348623157e308);
+  if (threadIdx.x == 0) partial[blockIdx.x] = best;
+}
+
+template <int IS_MIN>
+__global__ void __launch_bounds__(256) k_finish(const double* partial, int nb, double* out) {
+  double v = IS_MIN ? 1.7976931348623157e308 : 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    v = IS_MIN ? fmin(v, partial[i]) : v + partial[i];
+  if (IS_MIN) v = block_reduce(v, OpMin(), 1.7976931348623157e308);
+  else v = block_reduce(v, OpSum(), 0.);
+  if (threadIdx.x == 0) *out = v;
+}
+
+// ===========================================================================
+// energies (solid_mechanics_model.cc:536-582, 615-662; material.cc:743-757)
+// ===========================================================================
+template <int D>
+__global__ void __launch_bounds__(256) k_kinetic(int64_t N, const double* __restrict__ v,
+                                                 const double* __restrict__ mass, double* partial) {
+  double acc = 0.;
+  for (int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; node < N;
+       node += int64_t(gridDim.x) * blockDim.x) {
+    const double m = mass[node];
+    double mv2 = 0.;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double vi = v[node * D + i];
+      if (m > 2.220446049250313e-16) mv2 += vi * vi * m;
+    }
+    acc += mv2;
+  }
+  acc = block_reduce(acc, OpSum(), 0.);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_external_work(int64_t ndof, const double* __restrict__ v,
+                                                       const double* __restrict__ fext,
+                                                       const double* __restrict__ fint,
+                                                       const uint8_t* __restrict__ blocked,
+                                                       double* partial) {
+  double acc = 0.;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < ndof;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    if (blocked[k] & 1) acc -= fint[k] * v[k];
+    else acc += fext[k] * v[k];
+  }
+  acc = block_reduce(acc, OpSum(), 0.);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+template <int ET, int KIND>
+__global__ void __launch_bounds__(256) k_potential(int64_t count, const uint32_t* filter, int64_t E,
+                                                   const double* __restrict__ u,
+                                                   const uint32_t* __restrict__ conn,
+                                                   const double* __restrict__ B64,
+                                                   const double* __restrict__ J64,
+                                                   const uint16_t* __restrict__ elem_mat,
+                                                   const MatParams* __restrict__ mats,
+                                                   const double* __restrict__ damage,
+                                                   double* partial) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
+  double acc = 0.;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < count;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = filter ? int64_t(filter[t]) : t;
+    const MatParams m = mats[elem_mat ? elem_mat[e] : 0];
+    double ue[dn];
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      const int64_t node = conn[a * E + e];
+#pragma unroll
+      for (int i = 0; i < d; ++i) ue[i + a * d] = u[node * d + i];
+    }
+    double se = 0.;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double Bq[dn], G[d * d];
+#pragma unroll
+      for (int k = 0; k < dn; ++k) Bq[k] = B64[(int64_t(q) * dn + k) * E + e];
+      matrix_matrixt<d, d, n>(ue, Bq, G);
+      double w;
+      if constexpr (KIND == ELASTIC) {
+        double S[d * d];
+        elastic_stress_ref<d>(m, G, S);
+        double dd = 0.;
+#pragma unroll
+        for (int k = 0; k < d * d; ++k) dd += S[k] * G[k];
+        w = .5 * dd;
+      } else {
+        w = neohookean_energy_ref<d>(m, G);
+        if constexpr (KIND == NH_DAMAGE) w *= (1 - damage[e * Q + q]);
+      }
+      se += w * J64[int64_t(q) * E + e];
+    }
+    acc += se;
+  }
+  acc = block_reduce(acc, OpSum(), 0.);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// per-point grad u and stress on the current displacement (FP64, reference order;
+// the damage state is read, not advanced)
+template <int ET, int KIND>
+__global__ void __launch_bounds__(256) k_point_fields(ElemArgs A, double* __restrict__ gradu,
+                                                      double* __restrict__ stress) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t E = A.E;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+  const double* __restrict__ Bt = static_cast<const double*>(A.B);
+  double ue[dn];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    const int64_t node = A.conn[a * E + e];
+#pragma unroll
+    for (int i = 0; i < d; ++i) ue[i + a * d] = A.u[node * d + i];
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    double Bq[dn], G[d * d], S[d * d];
+#pragma unroll
+    for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+    matrix_matrixt<d, d, n>(ue, Bq, G);
+    if constexpr (KIND == ELASTIC) {
+      elastic_stress_ref<d>(m, G, S);
+    } else {
+      neohookean_stress_ref<d>(m, G, S);
+      if constexpr (KIND == NH_DAMAGE) {
+        const double dam = A.damage[e * Q + q];
+#pragma unroll
+        for (int k = 0; k < d * d; ++k) S[k] *= 1 - dam;
+      }
+    }
+    const int64_t eq = e * Q + q;
+#pragma unroll
+    for (int k = 0; k < d * d; ++k) {
+      gradu[eq * d * d + k] = G[k];
+      stress[eq * d * d + k] = S[k];
+    }
+  }
+}
+
+__global__ void k_add(int64_t n, const double* x, const double* y, double* out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double r = x[i];
+    r += y[i];
+    out[i] = r;
+  }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+// Clear any error left by an earlier runtime call so that the status returned
+// after a launch is the launch's own.
+static inline void clear_error() { (void)hipGetLastError(); }
+
+static inline unsigned grid_for(int64_t n, int block) {
+  return unsigned((n + block - 1) / block);
+}
+
+#define AKX_ET_SWITCH(et, F)       \
+  switch (et) {                    \
+  case TRI3: F(TRI3); break;       \
+  case QUAD4: F(QUAD4); break;     \
+  case TET4: F(TET4); break;       \
+  case HEX8: F(HEX8); break;       \
+  default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_precompute(int et, hipStream_t s, int64_t E, int64_t, const double* X,
+                             const uint32_t* conn, const ElemTables& tab, double* B64, double* J64,
+                             float* B32, float* J32, int* bad) {
+  clear_error();
+  if (E == 0) return hipSuccess;
+#define F(ET) k_precompute<ET><<<grid_for(E, 256), 256, 0, s>>>(E, X, conn, tab, B64, J64, B32, J32, bad)
+  AKX_ET_SWITCH(et, F)
+#undef F
+  return hipGetLastError();
+}
+
+hipError_t launch_mass(int et, hipStream_t s, int64_t N, int64_t E, const uint32_t* off,
+                       const uint32_t* slot, const ElemTables& tab, const double* J64,
+                       const uint16_t* elem_mat, const MatParams* mats, double* mass) {
+  clear_error();
+#define F(ET) k_mass<ET><<<grid_for(N, 256), 256, 0, s>>>(N, E, off, slot, tab, J64, elem_mat, mats, mass)
+  AKX_ET_SWITCH(et, F)
+#undef F
+  return hipGetLastError();
+}
+
+template <int ET>
+static hipError_t launch_element_et(int kind, int precision, hipStream_t s, const ElemArgs& A,
+                                    int block) {
+  const unsigned g = grid_for(A.count, block);
+  if (precision == 0) {
+    switch (kind) {
+    case ELASTIC: k_element_fp64<ET, ELASTIC><<<g, block, 0, s>>>(A); break;
+    case NEOHOOKEAN: k_element_fp64<ET, NEOHOOKEAN><<<g, block, 0, s>>>(A); break;
+    case NH_DAMAGE: k_element_fp64<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (kind) {
+    case ELASTIC: k_element_mixed<ET, ELASTIC><<<g, block, 0, s>>>(A); break;
+    case NEOHOOKEAN: k_element_mixed<ET, NEOHOOKEAN><<<g, block, 0, s>>>(A); break;
+    case NH_DAMAGE: k_element_mixed<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_element(int et, int kind, int precision, hipStream_t s, const ElemArgs& A,
+                          int block) {
+  clear_error();
+  if (A.count == 0) return hipSuccess;
+#define F(ET) return launch_element_et<ET>(kind, precision, s, A, block)
+  AKX_ET_SWITCH(et, F)
+#undef F
+  return hipSuccess;
+}
+
+template <int D>
+static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
+                                int block) {
+  const unsigned g = grid_for(A.N, block);
+  if (precision == 0) {
+    if (has_fext) k_node<D, double, true><<<g, block, 0, s>>>(A);
+    else k_node<D, double, false><<<g, block, 0, s>>>(A);
+  } else {
+    if (has_fext) k_node<D, float, true><<<g, block, 0, s>>>(A);
+    else k_node<D, float, false><<<g, block, 0, s>>>(A);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
+                       int block) {
+  clear_error();
+  if (A.N == 0) return hipSuccess;
+  if (d == 2) return launch_node_d<2>(precision, has_fext, s, A, block);
+  if (d == 3) return launch_node_d<3>(precision, has_fext, s, A, block);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
+                            const uint32_t* conn, const uint16_t* elem_mat, const MatParams* mats,
+                            double* partial, int nbmax, double* out) {
+  clear_error();
+  const int nb = int(std::min<int64_t>(nbmax, std::max<int64_t>(1, grid_for(E, 256))));
+#define F(ET) k_stable_dt<ET><<<nb, 256, 0, s>>>(E, X, u, conn, elem_mat, mats, partial)
+  AKX_ET_SWITCH(et, F)
+#undef F
+  k_finish<1><<<1, 256, 0, s>>>(partial, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(int kind_energy, int et, int mat_kind, hipStream_t s, int64_t N,
+                         int64_t E, const double* u, const double* v, const double* mass,
+                         const double* fext, const double* fint, const uint8_t* blocked,
+                         const uint32_t* conn, const double* B64, const double* J64,
+                         const uint16_t* elem_mat, const MatParams* mats, const double* damage,
+                         const uint32_t* filter, int64_t count, double* partial, int nbmax,
+                         double* out) {
+  clear_error();
+  const int d = (et == TET4 || et == HEX8) ? 3 : 2;
+  int nb = 1;
+  if (kind_energy == 0) {
+    nb = int(std::min<int64_t>(nbmax, std::max<int64_t>(1, grid_for(N, 256))));
+    if (d == 2) k_kinetic<2><<<nb, 256, 0, s>>>(N, v, mass, partial);
+    else k_kinetic<3><<<nb, 256, 0, s>>>(N, v, mass, partial);
+  } else if (kind_energy == 2) {
+    nb = int(std::min<int64_t>(nbmax, std::max<int64_t>(1, grid_for(N * d, 256))));
+    k_external_work<<<nb, 256, 0, s>>>(N * d, v, fext, fint, blocked, partial);
+  } else {
+    nb = int(std::min<int64_t>(nbmax, std::max<int64_t>(1, grid_for(count, 256))));
+#define G(ET, K) k_potential<ET, K><<<nb, 256, 0, s>>>(count, filter, E, u, conn, B64, J64, elem_mat, mats, damage, partial)
+#define F(ET)                                      \
+  switch (mat_kind) {                              \
+  case ELASTIC: G(ET, ELASTIC); break;             \
+  case NEOHOOKEAN: G(ET, NEOHOOKEAN); break;       \
+  case NH_DAMAGE: G(ET, NH_DAMAGE); break;         \
+  default: return hipErrorInvalidValue;            \
+  }
+    AKX_ET_SWITCH(et, F)
+#undef F
+#undef G
+  }
+  k_finish<0><<<1, 256, 0, s>>>(partial, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_fields(int et, int kind, hipStream_t s, const ElemArgs& A, double* gradu,
+                               double* stress) {
+  clear_error();
+  if (A.count == 0) return hipSuccess;
+  const unsigned g = grid_for(A.count, 256);
+#define G(ET, K) k_point_fields<ET, K><<<g, 256, 0, s>>>(A, gradu, stress)
+#define F(ET)                                      \
+  switch (kind) {                                  \
+  case ELASTIC: G(ET, ELASTIC); break;             \
+  case NEOHOOKEAN: G(ET, NEOHOOKEAN); break;       \
+  case NH_DAMAGE: G(ET, NH_DAMAGE); break;         \
+  default: return hipErrorInvalidValue;            \
+  }
+  AKX_ET_SWITCH(et, F)
+#undef F
+#undef G
+  return hipGetLastError();
+}
+
+hipError_t launch_add(hipStream_t s, int64_t n, const double* x, const double* y, double* out) {
+  clear_error();
+  if (n == 0) return hipSuccess;
+  k_add<<<grid_for(n, 256), 256, 0, s>>>(n, x, y, out);
+  return hipGetLastError();
+}
+
+}  // namespace akx

@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit central-difference step on one MI355X.
+
+Workload (default): 3D linear elastic unit cube, 100^3 hexes Kuhn-split into
+6,000,000 tetrahedron_4 (1,030,301 nodes), interior nodes jittered by +-1 % of h,
+E = 210 GPa, nu = 0.3, rho = 7800 kg/m^3, bottom face blocked, top face v_z = -1 m/s,
+dt = 0.8 x stable.  A "step" is one solveStep(): predictor, internal forces
+(gradient -> stress -> B^T sigma -> integrate -> assembly), residual, lumped
+solve, corrector.
+
+value = element-QP updates per second of the mixed path (FP32 element kernels,
+FP64 nodal state), inputs resident in HBM, timed over --steps steps between
+two stream synchronisations.  The FP64 path is timed the same way and reported
+beside it, with the relative nodal-displacement error of the mixed path after
+1000 steps.  The CPU baseline is the oracle (C++ FP64 restatement of the
+reference loops) timed on this host over a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "element-QP updates/s per explicit step, mixed vs FP64 at ≤1e-5 rel. error"
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False):
+    """Compulsory HBM bytes of one step, each array touched once.
+    pe: bytes of the element tables and force slots (4 mixed, 8 FP64)."""
+    elem = (E * n * 4            # connectivity
+            + E * Q * d * n * pe  # B = dN/dX
+            + E * Q * pe          # det(J) w
+            + N * d * 8           # displacement gather
+            + E * n * d * pe)     # element force slots (write)
+    node = ((N + 1) * 4 + E * n * 4  # node -> slot CSR
+            + E * n * d * pe         # force slots (read)
+            + 6 * N * d * 8          # u, v, a read + write
+            + N * 8                  # lumped mass (one value per node)
+            + N * d                  # blocked flags
+            + (N * d * 8 if has_fext else 0))
+    return elem, node
+
+
+def make_workload(n):
+    from akantu_amd import mesh as akm
+    m = akm.cube_tet4(n, 0.01, 1)
+    X = m.nodes
+    blocked = np.zeros_like(X, dtype=bool)
+    blocked[np.isclose(X[:, 2], 0.0)] = True
+    v0 = np.zeros_like(X)
+    v0[np.isclose(X[:, 2], 1.0), 2] = -1.0
+    return m, blocked, v0
+
+
+def reset(model, v0):
+    z = np.zeros_like(v0)
+    model.set("displacement", z)
+    model.set("velocity", v0)
+    model.set("acceleration", z)
+
+
+def timed(model, precision, steps, warmup, v0):
+    reset(model, v0)
+    if warmup:
+        model.solve_step(warmup, precision)
+    model.synchronize()
+    t0 = time.perf_counter()
+    model.solve_step(steps, precision)
+    model.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(model, precision, steps, v0):
+    reset(model, v0)
+    model.enable_timing(True)
+    model.solve_step(steps, precision)
+    model.synchronize()
+    el, nd = model.last_step_timing()
+    model.enable_timing(False)
+    return el, nd
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device hipMemcpy of `nbytes` (the runtime's copy kernel):
+    the measured streaming ceiling, 2 * nbytes moved per copy."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        return None
+    try:
+        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+        for _ in range(2):
+            hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)
+        hip.hipDeviceSynchronize()
+        t = (time.perf_counter() - t0) / reps
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return 2 * nbytes / t / 1e9
+
+
+def cpu_baseline(m, blocked, v0, dt, warm=2, steps=10, threads=1):
+    from oracle import oracle as orc
+    o = orc.OracleModel(m, [dict(E=210e9, nu=0.3, rho=7800.0)], threads=threads)
+    o.blocked[:] = blocked
+    o.nodal("v")[:] = v0
+    o.set_time_step(dt)
+    o.solve_step(warm)
+    t0 = time.perf_counter()
+    o.solve_step(steps)
+    t = time.perf_counter() - t0
+    return o, t / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=100, help="hexes per cube edge (6 n^3 tets)")
+    ap.add_argument("--error-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the 1 GiB device copy (calibration of the traffic counters)")
+    args = ap.parse_args()
+    if args.copy_only:
+        print(json.dumps({"copy_bytes": 1 << 30, "copy_gbs": copy_bandwidth(1 << 30, 3)}))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from akantu_amd import akx
+
+    log("building workload: cube %d^3 hexes -> tetrahedron_4" % args.n)
+    m, blocked, v0 = make_workload(args.n)
+    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 4, 1, 3
+    log("E=%d N=%d" % (E, N))
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(E=210e9, nu=0.3, rho=7800.0)
+    model.init_full()
+    model.set("blocked_dofs", blocked)
+    reset(model, v0)
+    dt = 0.8 * model.get_stable_time_step()
+    model.set_time_step(dt)
+    log("dt=%.6e" % dt)
+
+    res = {}
+    for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
+        t = timed(model, prec, args.steps, args.warmup, v0)
+        el_ms, nd_ms = kernel_times(model, prec, args.steps, v0)
+        res[name] = dict(seconds=t, ms_per_step=1e3 * t / args.steps,
+                         qp_per_s=E * Q * args.steps / t, element_ms=el_ms, node_ms=nd_ms)
+        log("%s: %.4f ms/step  (element %.4f ms, node %.4f ms)" %
+            (name, res[name]["ms_per_step"], el_ms, nd_ms))
+
+    # accuracy of the mixed path against the FP64 path after error-steps steps
+    log("error run: %d steps mixed and FP64" % args.error_steps)
+    reset(model, v0)
+    model.solve_step(args.error_steps, akx.MIXED)
+    u_mixed = model.get_displacement()
+    e_mixed = model.get_energy("kinetic") + model.get_energy("potential")
+    reset(model, v0)
+    model.solve_step(args.error_steps, akx.FP64)
+    u_fp64 = model.get_displacement()
+    e_fp64 = model.get_energy("kinetic") + model.get_energy("potential")
+    rel_u = float(np.linalg.norm(u_mixed - u_fp64) / np.linalg.norm(u_fp64))
+    rel_e = float(abs(e_mixed - e_fp64) / abs(e_fp64))
+    log("mixed vs FP64 after %d steps: rel u error %.3e, energy %.3e" %
+        (args.error_steps, rel_u, rel_e))
+
+    # roofline of the dominant kernel (mixed path)
+    eb, nb = algorithmic_bytes(E, N, n, Q, d, 4)
+    el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
+    if el_ms >= nd_ms:
+        kname, kbytes, kms = "element kernel (k_element_mixed<TET4, ELASTIC>)", eb, el_ms
+    else:
+        kname, kbytes, kms = "node kernel (k_node<3, float>)", nb, nd_ms
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c1_mixed.json")
+    if os.path.exists(tpath) and args.n == 100:
+        with open(tpath) as f:
+            tr = json.load(f)
+        traffic = tr.get("element" if el_ms >= nd_ms else "node")
+    step_bytes = eb + nb
+    step_gbs = step_bytes / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
+    eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8)
+    copy_gbs = copy_bandwidth()
+    log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
+
+    cpu = None
+    bitwise = None
+    if not args.no_cpu:
+        threads = 1
+        log("CPU baseline: oracle, %d thread, %d steps (+2 warm-up) on the full mesh"
+            % (threads, args.cpu_steps))
+        o, sec = cpu_baseline(m, blocked, v0, dt, 2, args.cpu_steps, threads)
+        cpu = {"value": E * Q / sec, "unit": "QP-updates/s", "cores": threads, "kind": "port",
+               "sample": "oracle (C++ FP64 restatement of the reference loops, g++ -O3 "
+                         "-ffp-contract=off), full 6M-element mesh, %d timed steps after 2 "
+                         "warm-up steps, %.3f s/step" % (args.cpu_steps, sec)}
+        log("CPU: %.3f s/step" % sec)
+        # the FP64 GPU path against the oracle on the full mesh, same step count
+        reset(model, v0)
+        model.solve_step(2 + args.cpu_steps, akx.FP64)
+        bitwise = bool(np.array_equal(model.get_displacement(), o.nodal("u")) and
+                       np.array_equal(model.get_velocity(), o.nodal("v")))
+        log("FP64 GPU == oracle after %d steps on the full mesh: %s" % (2 + args.cpu_steps, bitwise))
+        del o
+
+    out = {
+        "metric": METRIC,
+        "value": res["mixed"]["qp_per_s"],
+        "unit": "QP-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 element kernels / f64 nodal state",
+        "data": "synthetic",
+        "config": {"workload": "3D linear elastic cube, %d^3 hexes Kuhn-split into %d "
+                               "tetrahedron_4 (1 QP each), %d nodes, +-1%% interior jitter, "
+                               "E=210 GPa nu=0.3 rho=7800, bottom blocked, top v_z=-1 m/s, "
+                               "dt=0.8 x stable" % (args.n, E, N),
+                   "elements": E, "nodes": N, "qp_per_element": Q, "precision": "mixed",
+                   "parallelism": "1 GPU"},
+        "fp64": {"value": res["fp64"]["qp_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "element_ms": res["fp64"]["element_ms"], "node_ms": res["fp64"]["node_ms"],
+                 "step_gbs": (eb64 + nb64) / (res["fp64"]["ms_per_step"] * 1e-3) / 1e9},
+        "mixed_over_fp64": res["mixed"]["qp_per_s"] / res["fp64"]["qp_per_s"],
+        "accuracy": {"steps": args.error_steps, "rel_u_error_mixed_vs_fp64": rel_u,
+                     "rel_energy_error_mixed_vs_fp64": rel_e,
+                     "fp64_gpu_bitwise_equal_oracle": bitwise},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": kbytes, "avg_ms": kms},
+        "roofline_step": {"algorithmic_bytes": step_bytes, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "element_ms": el_ms, "node_ms": nd_ms,
+                          "element_gbs": eb / (el_ms * 1e-3) / 1e9,
+                          "node_gbs": nb / (nd_ms * 1e-3) / 1e9},
+        "copy_ceiling_gbs": copy_gbs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * akx.h -- C ABI of the MI355X explicit-dynamics engine (libakx.so).
+ *
+ * The engine replaces, on one GPU, the data-parallel hot path of Akantu's
+ * SolidMechanicsModel explicit central-difference step with lumped mass:
+ * FEEngine::gradientOnIntegrationPoints, Material::computeStress,
+ * FEEngine::computeBtD + integrate, DOFManager::assembleElementalArrayLocalArray
+ * and the NewmarkBeta(0, 1/2) predictor / lumped solve / corrector.
+ *
+ * Conventions (kept from the reference, src/common/aka_array.hh:139-224,
+ * aka_types.hh:879,1068):
+ *   - nodal arrays are (nb_nodes, dim) row-major float64 (blocked DOFs: uint8);
+ *   - connectivity is (nb_elements, nodes_per_element) uint32, reference local
+ *     node order;
+ *   - per-quadrature-point d x d tensors are column-major, points element-major
+ *     (index = element * nb_qp + q).
+ * Host pointers are owned by the caller; the context owns every device
+ * buffer.  All calls return AKX_OK (0) or a negative status; akx_last_error()
+ * holds the message (the reference throws debug::Exception, aka_error.hh:100).
+ * One host thread per context.  Calls are asynchronous on the context's stream
+ * except the ones that return host data, which synchronise.
+ */
+#ifndef AKX_H
+#define AKX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct akx_ctx akx_ctx;
+
+enum akx_status {
+  AKX_OK = 0,
+  AKX_ERR_ARGUMENT = -1,
+  AKX_ERR_STATE = -2,
+  AKX_ERR_HIP = -3,
+  AKX_ERR_JACOBIAN = -4,
+  AKX_ERR_UNSUPPORTED = -5
+};
+
+/* element types (src/common/aka_element_classes_info.hh.in) */
+enum akx_element_type {
+  AKX_TRIANGLE_3 = 0,
+  AKX_QUADRANGLE_4 = 1,
+  AKX_TETRAHEDRON_4 = 2,
+  AKX_HEXAHEDRON_8 = 3
+};
+
+/* constitutive laws */
+enum akx_material_kind {
+  AKX_MATERIAL_ELASTIC = 0,           /* MaterialElastic<dim>                 */
+  AKX_MATERIAL_NEOHOOKEAN = 1,        /* MaterialNeohookean<dim>              */
+  AKX_MATERIAL_NEOHOOKEAN_DAMAGE = 2  /* neo-Hookean + Marigo isotropic damage */
+};
+
+/* arithmetic of the element kernels */
+enum akx_precision {
+  AKX_FP64 = 0,  /* reference arithmetic, element-order sums (bit-exact path) */
+  AKX_MIXED = 1  /* FP32 element kernels, FP64 nodal state + compensated sums  */
+};
+
+/* fields for upload / download */
+enum akx_field {
+  AKX_DISPLACEMENT = 0,          /* (N,d) f64 */
+  AKX_VELOCITY = 1,              /* (N,d) f64 */
+  AKX_ACCELERATION = 2,          /* (N,d) f64 */
+  AKX_EXTERNAL_FORCE = 3,        /* (N,d) f64 */
+  AKX_INTERNAL_FORCE = 4,        /* (N,d) f64, of the last step / assembly */
+  AKX_MASS = 5,                  /* (N,d) f64, lumped (download only) */
+  AKX_BLOCKED_DOFS = 6,          /* (N,d) u8 */
+  AKX_PREVIOUS_DISPLACEMENT = 7, /* (N,d) f64, u before the last step's predictor */
+  AKX_DISPLACEMENT_INCREMENT = 8,/* (N,d) f64, u - previous after the last step */
+  AKX_GRAD_U = 9,                /* (E*Q, d*d) f64, on the current displacement */
+  AKX_STRESS = 10,               /* (E*Q, d*d) f64, Cauchy (small strain) or PK2 */
+  AKX_DAMAGE = 11,               /* (E*Q) f64 */
+  AKX_SHAPES_DERIVATIVES = 12,   /* (E*Q, d*n) f64, B = dN/dX, column-major d x n */
+  AKX_JACOBIANS = 13,            /* (E*Q) f64, det(J) * w */
+  AKX_CURRENT_POSITION = 14      /* (N,d) f64, X + u (download only) */
+};
+
+enum akx_energy_kind {
+  AKX_ENERGY_KINETIC = 0,   /* SolidMechanicsModel::getKineticEnergy      */
+  AKX_ENERGY_POTENTIAL = 1, /* Material::getPotentialEnergy summed        */
+  AKX_ENERGY_EXTERNAL_WORK = 2 /* SolidMechanicsModel::getExternalWork    */
+};
+
+/* akx_ctx_create flags */
+enum akx_ctx_flags {
+  /* Keep the caller's node and element numbering on the device.  By default
+   * both are renumbered along a Morton curve for memory locality; host arrays
+   * crossing the ABI are always in the caller's numbering. */
+  AKX_CTX_KEEP_NUMBERING = 1u
+};
+
+/* Context (device, stream, buffers).  Replaces the construction of
+ * SolidMechanicsModel (src/model/solid_mechanics/solid_mechanics_model.cc:71-103). */
+int akx_ctx_create(int device, unsigned flags, akx_ctx** out);
+int akx_ctx_destroy(akx_ctx* ctx);
+const char* akx_last_error(const akx_ctx* ctx);
+/* Use a caller-owned hipStream_t for all work of this context (NULL: the
+ * context's own stream). */
+int akx_set_stream(akx_ctx* ctx, void* hip_stream);
+
+/* Mesh upload.  Replaces Mesh::getNodes / getConnectivity as seen by
+ * FEEngineTemplate (src/mesh/mesh.hh:100-611); builds the node -> element
+ * slot CSR of MeshUtils::buildNode2Elements (src/mesh_utils/mesh_utils.cc:51-95)
+ * in assembly order. */
+int akx_mesh_set(akx_ctx* ctx, int element_type, int dim, int64_t nb_nodes,
+                 const double* nodes, int64_t nb_elements, const uint32_t* conn);
+
+/* Materials.  Replaces MaterialFactory instantiation + Material::initMaterial
+ * (src/model/solid_mechanics/material.hh:59-60,180; material_elastic.cc:83-107;
+ * material_neohookean.cc:83-89).  Returns the material index in *mat_id. */
+int akx_material_add(akx_ctx* ctx, int kind, double E, double nu, double rho,
+                     int plane_stress, int* mat_id);
+/* Damage parameters of a NEOHOOKEAN_DAMAGE material: Sd and the per-point
+ * thresholds Yd (E*Q values in global element order; only the material's own
+ * points are read).  Replaces MaterialMarigo's Sd / Yd random internal field
+ * (material_marigo.cc:39-57, random_internal_field_tmpl.hh:97-102). */
+int akx_material_damage(akx_ctx* ctx, int mat_id, double Sd, const double* Yd);
+/* Element -> material map (SolidMechanicsModel::assignMaterialToElements,
+ * solid_mechanics_model_material.cc:120-165).  NULL: every element -> 0. */
+int akx_assign_materials(akx_ctx* ctx, const int32_t* element_material);
+
+/* Precomputation: B = dN/dX and det(J) w at the reference configuration, lumped
+ * mass.  Replaces FEEngine::initShapeFunctions (fe_engine_template_tmpl.hh:154-167,
+ * shape_lagrange_inline_impl.hh:204-252, integrator_gauss_inline_impl.hh:208-252,
+ * 396-423) and SolidMechanicsModel::assembleMassLumped
+ * (solid_mechanics_model_mass.cc:58-102).  Fails with AKX_ERR_JACOBIAN on a
+ * negative Jacobian (integrator_gauss_inline_impl.hh:182-205). */
+int akx_init(akx_ctx* ctx);
+
+/* Host <-> device copies of the public arrays (SolidMechanicsModel::getX(),
+ * solid_mechanics_model.hh:363-397).  bytes must equal the field size. */
+int akx_field_upload(akx_ctx* ctx, int field, const void* host, size_t bytes);
+int akx_field_download(akx_ctx* ctx, int field, void* host, size_t bytes);
+/* Device address of a nodal field (for zero-copy interop on the same device).
+ * Device arrays are in the internal node numbering (see akx_node_numbering). */
+int akx_field_device_pointer(akx_ctx* ctx, int field, void** device_ptr);
+/* internal_to_caller[k] = caller id of internal node k (nb_nodes entries). */
+int akx_node_numbering(akx_ctx* ctx, int64_t* internal_to_caller);
+
+/* SolidMechanicsModel::setTimeStep (solid_mechanics_model.cc:438). */
+int akx_set_time_step(akx_ctx* ctx, double dt);
+
+/* nsteps x SolidMechanicsModel::solveStep() with the explicit lumped-mass
+ * solver: NonLinearSolverLumped::solve (non_linear_solver_lumped.cc:58-82) ->
+ * predictor (newmark-beta.cc:60-89) -> assembleResidual
+ * (solid_mechanics_model.cc:279-293, 375-414; integration_scheme_2nd_order.cc:79-102)
+ * -> solveLumped (non_linear_solver_lumped.cc:85-98) -> corrector
+ * (newmark-beta.cc:176-210).  Per-step Dirichlet increments registered with
+ * akx_set_dirichlet_increment are applied before each predictor. */
+int akx_solve_step(akx_ctx* ctx, int nsteps, int precision);
+
+/* SolidMechanicsModel::assembleInternalForces (solid_mechanics_model.cc:375-414)
+ * on the current displacement; the result is the AKX_INTERNAL_FORCE field. */
+int akx_assemble_internal_forces(akx_ctx* ctx, int precision);
+
+/* SolidMechanicsModel::getStableTimeStep (solid_mechanics_model.cc:482-533). */
+int akx_stable_time_step(akx_ctx* ctx, double* dt);
+
+/* SolidMechanicsModel::getEnergy (solid_mechanics_model.cc:665-683). */
+int akx_energy(akx_ctx* ctx, int kind, double* value);
+
+/* Blocks `axis` of the given nodes and adds `increment` to their displacement
+ * before every subsequent step: the per-step
+ * applyBC(BC::Dirichlet::IncrementValue(increment, axis), group) of a driver
+ * (boundary_condition_functor.hh:100-125, boundary_condition_tmpl.hh:62-88). */
+int akx_set_dirichlet_increment(akx_ctx* ctx, int64_t nb, const int64_t* nodes,
+                                int axis, double increment);
+
+/* Wait for all work queued on the context's stream. */
+int akx_synchronize(akx_ctx* ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every element and node kernel of akx_solve_step (off by default).
+ * akx_last_step_timing returns the averages over the last akx_solve_step, ms. */
+int akx_enable_timing(akx_ctx* ctx, int on);
+int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
+
+/* Launch shape used by the element and node kernels (threads per block, and
+ * the XCD-aware block mapping on/off).  0 keeps the current value. */
+int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKX_H */

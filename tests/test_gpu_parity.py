@@ -1,0 +1,250 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle.
+
+Bars:
+  * FP64 path: bit-identical to the oracle where the arithmetic is +,-,*,/ and
+    sqrt (elastic law, precompute, mass, stable dt); <= 1e-12 relative where the
+    law calls log() (neo-Hookean), whose last bit may differ between the device
+    library and glibc.
+  * mixed path (FP32 element kernels, FP64 nodal state): <= 1e-5 relative
+    nodal-displacement error after 1000 steps on the c0 bar; tolerances of the
+    other cases are written next to them.
+"""
+import numpy as np
+import pytest
+
+from akantu_amd import akx
+from akantu_amd import mesh as akm
+from tests import gpu_helpers as gh
+from tests import patch_helpers as ph
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("name", ph.PATCH_TYPES)
+def test_precompute_and_mass_bitwise(name):
+    m = ph.patch_mesh(name)
+    p = gh.Pair(m, [ph.STEEL])
+    B = p.gpu.get("shapes_derivatives").reshape(-1)
+    assert _same(B, p.cpu.field("B"))
+    assert _same(p.gpu.get("jacobians"), p.cpu.field("JxW"))
+    assert _same(p.gpu.get("mass"), p.cpu.nodal("mass"))
+    assert p.gpu.get_stable_time_step() == p.cpu.stable_time_step()
+
+
+@pytest.mark.parametrize("name", ph.PATCH_TYPES)
+@pytest.mark.parametrize("precision", [akx.FP64, akx.MIXED])
+def test_patch_linear_elastic_explicit_gpu(name, precision):
+    """patch_test_linear_elastic_explicit.cc:33-68 run on the GPU.  FP64 keeps
+    the reference tolerances; the mixed path keeps E_kin ~ 0 and the linear
+    field to FP32-level accuracy of the element forces."""
+    m = ph.patch_mesh(name)
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(**ph.STEEL)
+    model.init_full()
+    bn = akm.boundary_nodes(m)
+    blocked = np.zeros((m.nb_nodes, m.dim), dtype=bool)
+    blocked[bn] = True
+    model.set("blocked_dofs", blocked)
+    u0 = ph.linear_field(m.nodes)
+    model.set("displacement", u0)
+    model.set_time_step(0.8 * model.get_stable_time_step())
+    model.solve_step(100, precision)
+    u = model.get_displacement()
+    ekin = model.get_energy("kinetic")
+    if precision == akx.FP64:
+        assert abs(ekin) < 1e-16
+        assert np.abs(u - u0).max() < 1e-15
+        G = model.get("grad_u").reshape(-1, m.dim, m.dim).transpose(0, 2, 1)
+        pg = ph.prescribed_gradient(m.dim)
+        assert (np.abs(G - pg[None]).max(axis=(1, 2)) / np.abs(G).max(axis=(1, 2))).max() < 1e-13
+        S = model.get("stress").reshape(-1, m.dim, m.dim).transpose(0, 2, 1)
+        ps = ph.plane_strain_stress(pg)
+        assert (np.abs(S - ps[None]).max(axis=(1, 2)) / np.abs(ps).max()).max() < 1e-13
+        fint = model.get_internal_force()
+        fext = ph.boundary_traction_forces(m, S[0])
+        assert np.abs(fint + fext).max() / np.abs(fint).max() < 1e-9
+    else:
+        # FP32 element forces leave a residual of ~1e-7 of |f_e| at equilibrium:
+        # the interior nodes stay on the linear field to ~1e-6 of |u|.
+        assert np.abs(u - u0).max() < 1e-6 * np.abs(u0).max()
+        assert abs(ekin) < 1e-12 * model.get_energy("potential")
+
+
+def test_bar_c0_fp64_bitwise_1000_steps():
+    """c0 (4,000 triangle_3, 2,121 nodes), 1000 steps: FP64 GPU == oracle."""
+    p = gh.bar_pair()
+    p.step(1000, akx.FP64)
+    for g, c in (("displacement", "u"), ("velocity", "v"), ("acceleration", "a"),
+                 ("internal_force", "f_int")):
+        assert _same(p.gpu.get(g), p.cpu.nodal(c)), g
+    assert _same(p.gpu.get("displacement_increment"), p.cpu.nodal("increment"))
+    for kind in ("kinetic", "potential", "external work"):
+        assert abs(p.gpu.get_energy(kind) - p.cpu.energy(kind)) <= 1e-12 * abs(p.cpu.energy(kind))
+
+
+def test_bar_c0_mixed_1000_steps():
+    """Mixed path on c0 after 1000 steps: <= 1e-5 relative nodal displacement
+    error and <= 1e-5 relative error of E_kin + E_pot against the oracle."""
+    p = gh.bar_pair()
+    p.step(1000, akx.MIXED)
+    err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    assert err <= 1e-5, err
+    eg = p.gpu.get_energy("kinetic") + p.gpu.get_energy("potential")
+    ec = p.cpu.energy("kinetic") + p.cpu.energy("potential")
+    assert abs(eg - ec) / ec <= 1e-5
+
+
+def test_cube_tet4_fp64_bitwise_and_mixed():
+    """reduced c1 (10^3 hexes -> 6,000 jittered tetrahedron_4), 200 steps."""
+    p = gh.cube_pair(10)
+    p.step(200, akx.FP64)
+    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    assert _same(p.gpu.get_velocity(), p.cpu.nodal("v"))
+    q = gh.cube_pair(10)
+    q.step(200, akx.MIXED)
+    err = gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u"))
+    assert err <= 1e-5, err
+
+
+def test_cube_tet4_mixed_1000_steps():
+    """The 1000-step bar of the mixed path on tetrahedron_4 (8^3 cube)."""
+    p = gh.cube_pair(8)
+    p.step(1000, akx.MIXED)
+    err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    assert err <= 1e-5, err
+
+
+def test_hex8_neohookean_prescribed_velocity():
+    """reduced c2 (6^3 hexahedron_8, 8 points each), neo-Hookean, top face
+    pushed at 0.5 m/s: FP64 <= 1e-12, mixed <= 1e-5 after 300 steps."""
+    p = gh.hex_pair(6)
+    p.gpu.solve_step(300, akx.FP64)
+    gh.oracle_hex_step(p, 300)
+    err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    assert err <= 1e-12, err
+    assert gh.rel(p.gpu.get_velocity(), p.cpu.nodal("v")) <= 1e-12
+    epg, epc = p.gpu.get_energy("potential"), p.cpu.energy("potential")
+    assert abs(epg - epc) <= 1e-12 * abs(epc)
+    q = gh.hex_pair(6)
+    q.gpu.solve_step(300, akx.MIXED)
+    gh.oracle_hex_step(q, 300)
+    err = gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u"))
+    assert err <= 1e-5, err
+
+
+def test_hex8_damage_law():
+    """Neo-Hookean + damage on the reduced c2 cube with per-point thresholds:
+    FP64 damage and displacements match the oracle (<= 1e-12; the damage
+    branch decisions agree everywhere)."""
+    m = akm.cube_hex8(5)
+    rng = np.random.default_rng(3)
+    # thresholds around the energy density of ~5 % compression (E = 10 MPa)
+    Yd = rng.weibull(5.0, size=m.nb_elements * 8) * 5e3
+    p = gh.hex_pair(5, kind=akx.NEOHOOKEAN_DAMAGE, Yd=Yd, Sd=1e5, speed=0.5)
+    p.gpu.solve_step(150, akx.FP64)
+    gh.oracle_hex_step(p, 150)
+    dg, dc = p.gpu.get("damage"), p.cpu.field("damage")
+    assert (dc > 0).any()
+    assert np.array_equal(dg > 0, dc > 0)
+    assert np.abs(dg - dc).max() <= 1e-12
+    assert gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u")) <= 1e-12
+
+
+def test_two_materials_assembly_order():
+    """Two materials interleaved by element parity: the reference assembles
+    material by material (material.cc:190-237), so the FP64 sums must follow
+    that order and stay bit-identical."""
+    m = akm.bar_tri3(30, 6)
+    em = (np.arange(m.nb_elements) % 2).astype(np.int32)
+    mats = [dict(E=210e9, nu=0.3, rho=7800.0), dict(E=70e9, nu=0.33, rho=2700.0)]
+    p = gh.Pair(m, mats, em)
+    v = np.zeros_like(m.nodes)
+    v[:, 1] = 0.02 * np.sin(np.pi * m.nodes[:, 0])
+    p.set_nodal("velocity", v)
+    mask = np.zeros_like(m.nodes, dtype=bool)
+    mask[np.isclose(m.nodes[:, 0], 0.0)] = True
+    p.block(mask)
+    assert _same(p.gpu.get("mass"), p.cpu.nodal("mass"))
+    p.set_time_step(0.8 * p.cpu.stable_time_step())
+    p.step(200)
+    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+
+
+def test_external_force_and_internal_force_assembly():
+    """f_ext enters the residual (r += 1 f_ext) and assembleInternalForces on a
+    given displacement matches the oracle bit for bit."""
+    p = gh.cube_pair(6)
+    rng = np.random.default_rng(5)
+    f = rng.normal(size=p.mesh.nodes.shape) * 1e3
+    p.set_nodal("external_force", f)
+    p.step(50)
+    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    u = rng.normal(size=p.mesh.nodes.shape) * 1e-4
+    p.set_nodal("displacement", u)
+    p.gpu.assemble_internal_forces()
+    p.cpu.assemble_internal_forces()
+    assert _same(p.gpu.get_internal_force(), p.cpu.nodal("f_int"))
+
+
+def test_mixed_is_deterministic():
+    """No float atomics: two identical mixed runs are bit-identical."""
+    runs = []
+    for _ in range(2):
+        p = gh.cube_pair(8)
+        p.gpu.solve_step(100, akx.MIXED)
+        runs.append(p.gpu.get_displacement())
+    assert _same(runs[0], runs[1])
+
+
+def test_negative_jacobian_raises():
+    """integrator_gauss_inline_impl.hh:182-205: an inverted element is an error."""
+    m = akm.bar_tri3(4, 2)
+    conn = m.conn.copy()
+    conn[0] = conn[0][[0, 2, 1]]
+    bad = akm.Mesh(m.elem_type, m.nodes, conn)
+    model = akx.SolidMechanicsModel(bad)
+    model.add_material(E=1.0, nu=0.3, rho=1.0)
+    with pytest.raises(akx.AkxError, match="jacobian"):
+        model.init_full()
+
+
+def test_internal_renumbering_is_invisible():
+    """The Morton renumbering of nodes and elements inside the library changes
+    memory order only: FP64 results are bit-identical with and without it, and
+    every array comes back in the caller's numbering."""
+    outs = []
+    for keep in (False, True):
+        m = akm.cube_tet4(7, 0.01, 4)
+        model = akx.SolidMechanicsModel(m, keep_numbering=keep)
+        model.add_material(E=210e9, nu=0.3, rho=7800.0)
+        model.init_full()
+        X = m.nodes
+        blocked = np.zeros_like(X, dtype=bool)
+        blocked[np.isclose(X[:, 2], 0.0)] = True
+        model.set("blocked_dofs", blocked)
+        v = np.zeros_like(X)
+        v[:, 0] = np.sin(3 * X[:, 1]) * X[:, 2]
+        model.set("velocity", v)
+        model.set_time_step(0.8 * model.get_stable_time_step())
+        model.solve_step(60, akx.FP64)
+        outs.append((model.get_displacement(), model.get("stress"), model.get("mass"),
+                     model.get_blocked_dofs()))
+    for a, b in zip(*outs):
+        assert _same(a, b)
+
+
+def test_launch_shapes_do_not_change_results():
+    """Block sizes and the XCD-aware mapping change the schedule, not the sums."""
+    ref = None
+    for eb, nb, x in ((256, 256, 1), (64, 128, 0), (128, 64, 1)):
+        p = gh.cube_pair(6)
+        p.gpu.set_launch(eb, nb, x)
+        p.gpu.solve_step(50, akx.MIXED)
+        u = p.gpu.get_displacement()
+        if ref is None:
+            ref = u
+        assert _same(u, ref)
