@@ -66,6 +66,11 @@ void free_csr(akx_ctx* c) {
 void free_derived(akx_ctx* c) {
   dfree(c->B64); dfree(c->J64); dfree(c->B32); dfree(c->J32);
   free_csr(c);
+  dfree(c->slot_pos);
+  dfree(c->csr2_off); dfree(c->csr2_slot); dfree(c->slot2_pos);
+  dfree(c->pnode_off); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
+  dfree(c->lconn); dfree(c->lrank);
+  c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
   for (auto& g : c->groups) dfree(g.filter);
@@ -274,14 +279,28 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.damage = c->damage;
   A.Yd = c->Yd;
   A.xcd_remap = c->xcd_remap;
+  A.recompute = c->mixed_recompute;
+  A.X = c->X;
+  A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
+  A.patch = (precision == AKX_MIXED && c->node_ordered == 2 && c->patch_ok && !g.filter) ? 1 : 0;
+  A.patch_max_nodes = c->patch_max_nodes;
+  A.patch_lds = c->patch_lds;
+  A.patch_blocks = c->patch_blocks;
+  A.pnode_off = c->pnode_off;
+  A.pnode_ids = c->pnode_ids;
+  A.run_start = c->run_start;
+  A.run_base = c->run_base;
+  A.lconn = c->lconn;
+  A.lrank = c->lrank;
+  A.tabf = c->tabf;
   return A;
 }
 
-NodeArgs node_args(akx_ctx* c, int mode) {
+NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   NodeArgs A;
   A.N = c->N;
-  A.csr_off = c->csr_off;
-  A.csr_slot = c->csr_slot;
+  A.csr_off = precision == AKX_MIXED ? c->csr2_off : c->csr_off;
+  A.csr_slot = precision == AKX_MIXED ? c->csr2_slot : c->csr_slot;
   A.slots = c->slots;
   A.u = c->u; A.v = c->v; A.a = c->a;
   A.fext = c->fext;
@@ -294,6 +313,7 @@ NodeArgs node_args(akx_ctx* c, int mode) {
   for (int i = 0; i < 3; ++i) A.bc_inc[i] = c->bc_inc[i];
   A.mode = mode;
   A.xcd_remap = c->xcd_remap;
+  A.node_ordered = c->node_ordered;
   return A;
 }
 
@@ -308,6 +328,90 @@ int run_elements(akx_ctx* c, int precision) {
     ElemArgs A = elem_args(c, g, precision);
     HIP_TRY(c, launch_element(c->et, g.kind, precision, c->stream, A, c->elem_block));
   }
+  return AKX_OK;
+}
+
+// Patches of PATCH consecutive internal elements for k_element_patch: the
+// distinct nodes of each patch, where each node's run of mixed-CSR slots
+// starts, and for every (e, a) its patch-local node and rank in the run.
+// `slot2` is the mixed-path CSR (node-major, element id ascending).  Leaves
+// c->patch_ok false when a mesh does not fit the 16/8-bit local indices or
+// the LDS budget.
+constexpr int PATCH = 256;
+int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
+  c->patch_ok = false;
+  const int64_t E = c->E;
+  const int n = c->n, d = c->d;
+  if (E == 0) return AKX_OK;
+  // patches need one constitutive law over all elements (a single launch group)
+  for (int64_t e = 1; e < E; ++e)
+    if (c->mats[c->elem_mat[e]].kind != c->mats[c->elem_mat[0]].kind) return AKX_OK;
+  std::vector<uint32_t> pos(size_t(E) * n);  // CSR position of (e, a)
+  for (size_t p = 0; p < slot2.size(); ++p) pos[size_t(slot2[p] % n) * E + slot2[p] / n] = uint32_t(p);
+  const int64_t nb = (E + PATCH - 1) / PATCH;
+  std::vector<uint32_t> poff(nb + 1, 0), pids, rstart;
+  std::vector<uint16_t> rbase, lconn(size_t(E) * n);
+  std::vector<uint8_t> lrank(size_t(E) * n);
+  int maxm = 0;
+  std::vector<uint32_t> nodes;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t e0 = b * PATCH, e1 = std::min<int64_t>(E, e0 + PATCH);
+    nodes.clear();
+    for (int64_t e = e0; e < e1; ++e)
+      for (int a = 0; a < n; ++a)
+        nodes.push_back(c->node_inv[c->hconn[size_t(c->elem_perm[e]) * n + a]]);
+    std::sort(nodes.begin(), nodes.end());
+    nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+    const int m = int(nodes.size());
+    if (m > 65535) return AKX_OK;
+    maxm = std::max(maxm, m);
+    std::vector<uint32_t> lo(m, std::numeric_limits<uint32_t>::max()), hi(m, 0), cnt(m, 0);
+    for (int64_t e = e0; e < e1; ++e)
+      for (int a = 0; a < n; ++a) {
+        const uint32_t node = c->node_inv[c->hconn[size_t(c->elem_perm[e]) * n + a]];
+        const int k = int(std::lower_bound(nodes.begin(), nodes.end(), node) - nodes.begin());
+        const uint32_t p = pos[size_t(a) * E + e];
+        lo[k] = std::min(lo[k], p);
+        hi[k] = std::max(hi[k], p);
+        cnt[k]++;
+        lconn[size_t(a) * E + e] = uint16_t(k);
+      }
+    uint32_t acc = 0;
+    for (int k = 0; k < m; ++k) {
+      if (hi[k] - lo[k] + 1 != cnt[k] || cnt[k] > 255) return AKX_OK;  // run not contiguous
+      pids.push_back(nodes[k]);
+      rstart.push_back(lo[k]);
+      rbase.push_back(uint16_t(acc));
+      acc += cnt[k];
+    }
+    rbase.push_back(uint16_t(acc));
+    for (int64_t e = e0; e < e1; ++e)
+      for (int a = 0; a < n; ++a) {
+        const int k = lconn[size_t(a) * E + e];
+        lrank[size_t(a) * E + e] = uint8_t(pos[size_t(a) * E + e] - lo[k]);
+      }
+    poff[b + 1] = poff[b] + uint32_t(m);
+  }
+  const size_t lds = ((size_t(maxm) * (2 * d * 8 + 4) + size_t(maxm + 1) * 2 + 15) & ~size_t(15)) +
+                     size_t(PATCH) * n * d * 4;
+  if (lds > 64 * 1024) return AKX_OK;
+  HIP_TRY(c, dalloc(&c->pnode_off, poff.size()));
+  HIP_TRY(c, dalloc(&c->pnode_ids, pids.size()));
+  HIP_TRY(c, dalloc(&c->run_start, rstart.size()));
+  HIP_TRY(c, dalloc(&c->run_base, rbase.size()));
+  HIP_TRY(c, dalloc(&c->lconn, lconn.size()));
+  HIP_TRY(c, dalloc(&c->lrank, lrank.size()));
+  HIP_TRY(c, hipMemcpy(c->pnode_off, poff.data(), poff.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->pnode_ids, pids.data(), pids.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->run_start, rstart.data(), rstart.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->run_base, rbase.data(), rbase.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->lconn, lconn.data(), lconn.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->lrank, lrank.data(), lrank.size(), hipMemcpyHostToDevice));
+  c->patch_max_nodes = maxm;
+  c->patch_nodes_total = int64_t(pids.size());
+  c->patch_lds = int(lds);
+  c->patch_blocks = nb;
+  c->patch_ok = true;
   return AKX_OK;
 }
 
@@ -529,6 +633,31 @@ int akx_init(akx_ctx* c) {
   HIP_TRY(c, dalloc(&c->csr_slot, slot.size()));
   HIP_TRY(c, hipMemcpy(c->csr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->csr_slot, slot.data(), slot.size() * 4, hipMemcpyHostToDevice));
+  auto upload_pos = [&](const std::vector<uint32_t>& sl, uint32_t** dst) -> int {
+    // inverse map for the node-ordered slot layout: CSR position of (e, a)
+    std::vector<uint32_t> pos(size_t(E) * n);
+    for (size_t p = 0; p < sl.size(); ++p) {
+      const uint32_t s = sl[p];
+      pos[size_t(s % n) * E + s / n] = uint32_t(p);
+    }
+    HIP_TRY(c, dalloc(dst, pos.size()));
+    HIP_TRY(c, hipMemcpy(*dst, pos.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    return AKX_OK;
+  };
+  if (int rc = upload_pos(slot, &c->slot_pos)) return rc;
+  {
+    // The mixed path needs a fixed summation order but not the reference's:
+    // its CSR lists each node's contributions by internal element id, so that
+    // neighbouring elements of one block write neighbouring slots.
+    std::vector<uint32_t> off2, slot2;
+    build_csr(c, c->elem_perm, off2, slot2);
+    HIP_TRY(c, dalloc(&c->csr2_off, off2.size()));
+    HIP_TRY(c, dalloc(&c->csr2_slot, slot2.size()));
+    HIP_TRY(c, hipMemcpy(c->csr2_off, off2.data(), off2.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->csr2_slot, slot2.data(), slot2.size() * 4, hipMemcpyHostToDevice));
+    if (int rc = upload_pos(slot2, &c->slot2_pos)) return rc;
+    if (int rc = build_patches(c, slot2)) return rc;
+  }
   if (ascending) {
     c->mass_off = c->csr_off;
     c->mass_slot = c->csr_slot;
@@ -615,6 +744,10 @@ int akx_init(akx_ctx* c) {
   HIP_TRY(c, dalloc(&c->partial, size_t(c->partial_blocks)));
   HIP_TRY(c, dalloc(&c->scalar, 4));
   const ElemTables tab = make_tables(c->et);
+  for (int q = 0; q < 8; ++q) {
+    for (int k = 0; k < 24; ++k) c->tabf.dnds[q][k] = float(tab.dnds[q][k]);
+    c->tabf.w[q] = float(tab.w[q]);
+  }
   HIP_TRY(c, launch_precompute(c->et, c->stream, E, N, c->X, c->conn_soa, tab, c->B64, c->J64,
                                c->B32, c->J32, c->flag));
   int bad = 0;
@@ -798,7 +931,7 @@ int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
   }
   // predictor of step 1
   {
-    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0));
+    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0), precision);
     HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
   }
   for (int s = 1; s <= nsteps; ++s) {
@@ -810,7 +943,7 @@ int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
     int mode = NODE_CORRECT;
     if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
     else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
-    NodeArgs A = node_args(c, mode);
+    NodeArgs A = node_args(c, mode, precision);
     HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
   }
@@ -840,7 +973,7 @@ int akx_assemble_internal_forces(akx_ctx* c, int precision) {
   (void)hipSetDevice(c->device);
   rc = run_elements(c, precision);
   if (rc) return rc;
-  NodeArgs A = node_args(c, NODE_GATHER | NODE_STORE_FINT);
+  NodeArgs A = node_args(c, NODE_GATHER | NODE_STORE_FINT, precision);
   HIP_TRY(c, launch_node(c->d, precision, c->has_fext, c->stream, A, c->node_block));
   c->fint_valid = true;
   return AKX_OK;
@@ -930,7 +1063,100 @@ int akx_last_step_timing(akx_ctx* c, double* element_ms, double* node_ms) {
   return AKX_OK;
 }
 
-int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap) {
+int akx_query(akx_ctx* c, const char* key, double* value) {
+  if (!c || !key || !value) return AKX_ERR_ARGUMENT;
+  const std::string k(key);
+  if (k == "nb_nodes") *value = double(c->N);
+  else if (k == "nb_elements") *value = double(c->E);
+  else if (k == "csr_entries") *value = double(c->E) * c->n;
+  else if (k == "patch_blocks") *value = double(c->patch_ok ? c->patch_blocks : 0);
+  else if (k == "patch_nodes_total") *value = double(c->patch_ok ? c->patch_nodes_total : 0);
+  else if (k == "patch_max_nodes") *value = double(c->patch_ok ? c->patch_max_nodes : 0);
+  else if (k == "patch_available") *value = c->patch_ok ? 1. : 0.;
+  else if (k == "element_block") *value = c->elem_block;
+  else if (k == "node_block") *value = c->node_block;
+  else if (k == "xcd_remap") *value = c->xcd_remap;
+  else if (k == "mixed_recompute") *value = c->mixed_recompute;
+  else if (k == "slot_layout") *value = c->node_ordered;
+  else return fail(c, AKX_ERR_ARGUMENT, "unknown query key " + k);
+  return AKX_OK;
+}
+
+int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* best_ms) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (precision != AKX_FP64 && precision != AKX_MIXED)
+    return fail(c, AKX_ERR_ARGUMENT, "unknown precision");
+  if (!(c->dt > 0)) return fail(c, AKX_ERR_STATE, "time step not set");
+  if (nsteps <= 0) nsteps = 10;
+  (void)hipSetDevice(c->device);
+  // save the state every step touches
+  const size_t nd = size_t(c->N) * c->d;
+  std::vector<std::pair<double*, double*>> saved;
+  std::vector<std::pair<double*, size_t>> arrays = {{c->u, nd}, {c->v, nd}, {c->a, nd},
+                                                    {c->fint, nd}, {c->prev, nd}, {c->inc, nd}};
+  if (c->damage) arrays.push_back({c->damage, size_t(c->E) * c->Q});
+  for (auto& pa : arrays) {
+    double* copy = nullptr;
+    HIP_TRY(c, dalloc(&copy, pa.second));
+    HIP_TRY(c, hipMemcpyAsync(copy, pa.first, pa.second * 8, hipMemcpyDeviceToDevice, c->stream));
+    saved.push_back({pa.first, copy});
+  }
+  auto restore = [&]() -> int {
+    for (size_t k = 0; k < saved.size(); ++k)
+      HIP_TRY(c, hipMemcpyAsync(saved[k].first, saved[k].second, arrays[k].second * 8,
+                                hipMemcpyDeviceToDevice, c->stream));
+    return AKX_OK;
+  };
+  hipEvent_t t0, t1;
+  HIP_TRY(c, hipEventCreate(&t0));
+  HIP_TRY(c, hipEventCreate(&t1));
+  const int keep[5] = {c->elem_block, c->node_block, c->xcd_remap, c->mixed_recompute,
+                       c->node_ordered};
+  int best[5] = {keep[0], keep[1], keep[2], keep[3], keep[4]};
+  float best_t = std::numeric_limits<float>::max();
+  const bool was_timing = c->timing;
+  c->timing = false;
+  const int blocks[3] = {64, 128, 256};
+  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 3 : 2;
+  for (int lay = 0; lay < nlay && rc == AKX_OK; ++lay)
+  for (int rcmp = 0; rcmp < (precision == AKX_MIXED ? 2 : 1) && rc == AKX_OK; ++rcmp)
+    for (int x = 0; x < 2 && rc == AKX_OK; ++x)
+      for (int eb : blocks)
+        for (int nb : blocks) {
+          c->elem_block = eb; c->node_block = nb; c->xcd_remap = x; c->node_ordered = lay;
+          c->mixed_recompute = precision == AKX_MIXED ? rcmp : keep[3];
+          if ((rc = restore())) break;
+          if ((rc = akx_solve_step(c, 1, precision))) break;  // warm-up
+          if (hipEventRecord(t0, c->stream) != hipSuccess) { rc = AKX_ERR_HIP; break; }
+          if ((rc = akx_solve_step(c, nsteps, precision))) break;
+          if (hipEventRecord(t1, c->stream) != hipSuccess ||
+              hipEventSynchronize(t1) != hipSuccess) { rc = AKX_ERR_HIP; break; }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, t0, t1);
+          if (ms < best_t) {
+            best_t = ms;
+            best[0] = eb; best[1] = nb; best[2] = x; best[3] = c->mixed_recompute;
+            best[4] = lay;
+          }
+        }
+  c->timing = was_timing;
+  c->elem_block = best[0]; c->node_block = best[1]; c->xcd_remap = best[2];
+  c->mixed_recompute = best[3]; c->node_ordered = best[4];
+  int rc2 = restore();
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& s : saved) dfree(s.second);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (chosen) for (int k = 0; k < 5; ++k) chosen[k] = best[k];
+  if (best_ms) *best_ms = double(best_t) / nsteps;
+  return AKX_OK;
+}
+
+int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
+                   int mixed_recompute, int slot_layout) {
   if (!c) return AKX_ERR_ARGUMENT;
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
@@ -938,6 +1164,9 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap)
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;
+  if (mixed_recompute >= 0) c->mixed_recompute = mixed_recompute ? 1 : 0;
+  if (slot_layout > 2) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1 or 2");
+  if (slot_layout >= 0) c->node_ordered = slot_layout;
   return AKX_OK;
 }
 

@@ -43,6 +43,12 @@ struct ElemTables {
   double w[8];
 };
 
+// FP32 copy of dN/dxi and the weights for the mixed kernels that rebuild B.
+struct ElemTablesF {
+  float dnds[8][24];
+  float w[8];
+};
+
 // ---------------------------------------------------------------------------
 // XCD-aware block index: consecutive logical blocks land on the same XCD (the
 // hardware deals workgroups round-robin over the 8 XCDs), so neighbouring

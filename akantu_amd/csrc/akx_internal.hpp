@@ -41,6 +41,21 @@ struct ElemArgs {
   double* damage;            // [e*Q + q]
   const double* Yd;          // [e*Q + q]
   int xcd_remap;
+  int recompute;             // mixed path: rebuild B, det(J) w from X
+  const double* X;           // (N, d) reference coordinates
+  const uint32_t* slot_pos;  // [a][E] CSR position of (e, a); nullptr = element-ordered slots
+  // LDS-patch variant of the mixed kernel (k_element_patch)
+  int patch;                 // 1: use it
+  int patch_max_nodes;       // most distinct nodes of one patch
+  int patch_lds;             // dynamic LDS bytes per block
+  int64_t patch_blocks;      // number of patches
+  const uint32_t* pnode_off; // [patch] -> first entry in pnode_ids / run_start
+  const uint32_t* pnode_ids; // patch nodes (internal ids, ascending)
+  const uint32_t* run_start; // first mixed-CSR position of the patch's run for the node
+  const uint16_t* run_base;  // [pnode_off[b] + b + k]: prefix of run lengths (m+1 per patch)
+  const uint16_t* lconn;     // [a][E] patch-local node index
+  const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
+  ElemTablesF tabf;
 };
 
 struct NodeArgs {
@@ -61,6 +76,7 @@ struct NodeArgs {
   double bc_inc[3];
   int mode;
   int xcd_remap;
+  int node_ordered;       // slots stored in CSR order (contiguous per node)
 };
 
 enum NodeMode {
@@ -151,7 +167,19 @@ struct akx_ctx {
   bool mesh_set = false, initialized = false;
   bool fint_valid = false;
 
-  int elem_block = 256, node_block = 256, xcd_remap = 1;
+  int elem_block = 256, node_block = 256, xcd_remap = 1, mixed_recompute = 1;
+  int node_ordered = 0;             // slot layout (see store_slots)
+  uint32_t* slot_pos = nullptr;     // [a][E] CSR position of (e, a)
+  // mixed path: CSR by internal element id (any fixed order will do there)
+  uint32_t *csr2_off = nullptr, *csr2_slot = nullptr, *slot2_pos = nullptr;
+  // LDS element patches (slot layout 2)
+  bool patch_ok = false;
+  int patch_max_nodes = 0, patch_lds = 0;
+  int64_t patch_blocks = 0, patch_nodes_total = 0;
+  uint32_t *pnode_off = nullptr, *pnode_ids = nullptr, *run_start = nullptr;
+  uint16_t *run_base = nullptr, *lconn = nullptr;
+  uint8_t* lrank = nullptr;
+  akx::ElemTablesF tabf;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal
   bool reorder = true;
   std::vector<uint32_t> node_perm, node_inv, elem_perm, elem_inv;

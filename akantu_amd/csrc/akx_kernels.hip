@@ -99,6 +99,48 @@ __global__ void __launch_bounds__(256) k_mass(int64_t N, int64_t E, const uint32
 }
 
 // ===========================================================================
+// element force slots.  Two layouts:
+//   element-ordered (slot_pos == nullptr): element e's n*d values at
+//     slots[e*n*d ...], written as 16-byte vectors; the node kernel gathers
+//     them through the CSR slot ids;
+//   node-ordered: node a of element e goes to CSR position slot_pos[a][e], so
+//     that every node finds its contributions contiguous, in assembly order.
+// ===========================================================================
+template <int d, int n, typename T>
+__device__ __forceinline__ void store_slots(const ElemArgs& A, int64_t e, const T* fe) {
+  constexpr int dn = d * n;
+  if (A.slot_pos) {
+    T* __restrict__ out = static_cast<T*>(A.slots);
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      const int64_t p = A.slot_pos[a * A.E + e];
+#pragma unroll
+      for (int i = 0; i < d; ++i) out[p * d + i] = fe[i + a * d];
+    }
+    return;
+  }
+  T* __restrict__ out = static_cast<T*>(A.slots) + e * dn;
+  constexpr int per = 16 / sizeof(T);  // values per 16-byte store
+  if constexpr (dn % per == 0) {
+    using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    V* __restrict__ vo = reinterpret_cast<V*>(out);
+#pragma unroll
+    for (int k = 0; k < dn / per; ++k) {
+      V v;
+      if constexpr (sizeof(T) == 8) {
+        v.x = fe[2 * k]; v.y = fe[2 * k + 1];
+      } else {
+        v.x = fe[4 * k]; v.y = fe[4 * k + 1]; v.z = fe[4 * k + 2]; v.w = fe[4 * k + 3];
+      }
+      vo[k] = v;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < dn; ++k) out[k] = fe[k];
+  }
+}
+
+// ===========================================================================
 // element kernel, FP64, reference operation order
 // ===========================================================================
 template <int ET, int KIND>
@@ -151,50 +193,93 @@ __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
 #pragma unroll
     for (int k = 0; k < dn; ++k) fe[k] += R[k] * jq;
   }
-  double* __restrict__ out = static_cast<double*>(A.slots) + e * dn;
-#pragma unroll
-  for (int k = 0; k < dn; ++k) out[k] = fe[k];
+  store_slots<d, n>(A, e, fe);
 }
 
 // ===========================================================================
 // element kernel, FP32 element math (mixed precision)
+//   RECOMPUTE = false: B and det(J) w read from the FP32 tables;
+//   RECOMPUTE = true:  both rebuilt per point from the element's coordinates
+//     (differences to node 0 formed in FP64, then rounded), which replaces the
+//     E*Q*(d*n+1) table reads by one more gather of the (cache-resident) nodes.
 // ===========================================================================
-template <int ET, int KIND>
-__global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
+// Element forces of the mixed path from the element's displacement and
+// coordinate differences to node 0 (du, dX; dX unused when the FP32 tables are
+// read).  Shared by the gather kernel and the LDS-patch kernel.
+template <int ET, int KIND, bool RECOMPUTE>
+__device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t e,
+                                                     const MatParams& m, const float* du,
+                                                     const float* dX, float* fe) {
 #pragma clang fp contract(fast)
   using T = Elem<ET>;
   constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
-  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
-  if (t >= A.count) return;
+  constexpr bool simplex = (ET == TRI3 || ET == TET4);
   const int64_t E = A.E;
-  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
-  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
   const float* __restrict__ Bt = static_cast<const float*>(A.B);
   const float* __restrict__ Jt = static_cast<const float*>(A.J);
-  // displacement differences to node 0, formed in FP64 then rounded: sum_a B_a = 0
-  // (partition of unity), so grad u = sum_{a>0} (u_a - u_0) B_a^T exactly.
-  int64_t nodes[n];
-#pragma unroll
-  for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
-  double u0[d];
-#pragma unroll
-  for (int i = 0; i < d; ++i) u0[i] = A.u[nodes[0] * d + i];
-  float du[dn];
-#pragma unroll
-  for (int i = 0; i < d; ++i) du[i] = 0.f;
-#pragma unroll
-  for (int a = 1; a < n; ++a)
-#pragma unroll
-    for (int i = 0; i < d; ++i) du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
-  float fe[dn];
 #pragma unroll
   for (int k = 0; k < dn; ++k) fe[k] = 0.f;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     float Bq[dn];
+    float jq;
+    if constexpr (!RECOMPUTE) {
 #pragma unroll
-    for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
-    const float jq = Jt[int64_t(q) * E + e];
+      for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+      jq = Jt[int64_t(q) * E + e];
+    } else {
+      // J(i,j) = sum_a dN_a/dxi_i x_a,j (column-major), with x_0 = 0
+      float J[d * d];
+#pragma unroll
+      for (int j = 0; j < d; ++j)
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          if constexpr (simplex) {
+            J[i + j * d] = dX[j + (i + 1) * d];
+          } else {
+            float s = 0.f;
+#pragma unroll
+            for (int a = 1; a < n; ++a) s += A.tabf.dnds[q][i + a * d] * dX[j + a * d];
+            J[i + j * d] = s;
+          }
+        }
+      const float det = detn<d>(J);
+      const float idet = 1.f / det;
+      float Ji[d * d];
+      if constexpr (d == 2) {
+        Ji[0] = J[3] * idet; Ji[1] = -J[1] * idet; Ji[2] = -J[2] * idet; Ji[3] = J[0] * idet;
+      } else {
+        Ji[0] = (J[4] * J[8] - J[7] * J[5]) * idet;
+        Ji[1] = (J[2] * J[7] - J[8] * J[1]) * idet;
+        Ji[2] = (J[1] * J[5] - J[4] * J[2]) * idet;
+        Ji[3] = (J[5] * J[6] - J[8] * J[3]) * idet;
+        Ji[4] = (J[0] * J[8] - J[6] * J[2]) * idet;
+        Ji[5] = (J[2] * J[3] - J[5] * J[0]) * idet;
+        Ji[6] = (J[3] * J[7] - J[6] * J[4]) * idet;
+        Ji[7] = (J[1] * J[6] - J[7] * J[0]) * idet;
+        Ji[8] = (J[0] * J[4] - J[3] * J[1]) * idet;
+      }
+      // B(i,a) = sum_l Ji(i,l) dN_a/dxi_l
+#pragma unroll
+      for (int a = 0; a < n; ++a)
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          float s = 0.f;
+          if constexpr (simplex) {
+            if (a == 0) {
+#pragma unroll
+              for (int l = 0; l < d; ++l) s -= Ji[i + l * d];
+            } else {
+              s = Ji[i + (a - 1) * d];
+            }
+          } else {
+#pragma unroll
+            for (int l = 0; l < d; ++l) s += Ji[i + l * d] * A.tabf.dnds[q][l + a * d];
+          }
+          Bq[i + a * d] = s;
+        }
+      jq = det * A.tabf.w[q];
+    }
     float G[d * d];
 #pragma unroll
     for (int j = 0; j < d; ++j)
@@ -254,9 +339,132 @@ __global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
         fe[i + a * d] = s;
       }
   }
-  float* __restrict__ out = static_cast<float*>(A.slots) + e * dn;
+}
+
+template <int ET, int KIND, bool RECOMPUTE>
+__global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, dn = d * n;
+  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t E = A.E;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+  // displacement differences to node 0, formed in FP64 then rounded: sum_a B_a = 0
+  // (partition of unity), so grad u = sum_{a>0} (u_a - u_0) B_a^T exactly.
+  int64_t nodes[n];
 #pragma unroll
-  for (int k = 0; k < dn; ++k) out[k] = fe[k];
+  for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
+  double u0[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) u0[i] = A.u[nodes[0] * d + i];
+  float du[dn];
+#pragma unroll
+  for (int i = 0; i < d; ++i) du[i] = 0.f;
+#pragma unroll
+  for (int a = 1; a < n; ++a)
+#pragma unroll
+    for (int i = 0; i < d; ++i) du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
+  float dX[RECOMPUTE ? dn : 1];
+  if constexpr (RECOMPUTE) {
+    double x0[d];
+#pragma unroll
+    for (int i = 0; i < d; ++i) x0[i] = A.X[nodes[0] * d + i];
+#pragma unroll
+    for (int i = 0; i < d; ++i) dX[i] = 0.f;
+#pragma unroll
+    for (int a = 1; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) dX[i + a * d] = float(A.X[nodes[a] * d + i] - x0[i]);
+  }
+  float fe[dn];
+  mixed_element_forces<ET, KIND, RECOMPUTE>(A, e, m, du, dX, fe);
+  store_slots<d, n>(A, e, fe);
+}
+
+// ===========================================================================
+// element kernel, mixed precision, LDS-staged element patches
+//   One block = one patch of up to PATCH consecutive (Morton-ordered)
+//   elements.  The block
+//     1. loads the patch's distinct nodes (u and X in FP64, the start of the
+//        node's run of slots) once, coalesced, into LDS;
+//     2. computes every element from LDS through 16-bit patch-local node ids
+//        (same arithmetic as k_element_mixed<..., true>);
+//     3. stages the n*d forces of each element in LDS at the element's place
+//        in the patch's per-node runs and writes the runs out contiguously.
+//   Valid with the mixed-path CSR, which lists each node's slots by element
+//   id: the patch's contributions to one node are then one contiguous run.
+// ===========================================================================
+template <int ET, int KIND>
+__global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, dn = d * n;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t b = logical_block(A.xcd_remap);
+  const int64_t e0 = b * int64_t(blockDim.x);
+  if (e0 >= A.count) return;  // uniform over the block
+  const int ne = int(min<int64_t>(blockDim.x, A.count - e0));
+  const uint32_t k0 = A.pnode_off[b], m_nodes = A.pnode_off[b + 1] - k0;
+  // LDS: u (FP64), X (FP64), run start (u32), run base (u16, m+1), forces (f32)
+  double* s_u = reinterpret_cast<double*>(smem);
+  double* s_x = s_u + size_t(A.patch_max_nodes) * d;
+  uint32_t* s_rs = reinterpret_cast<uint32_t*>(s_x + size_t(A.patch_max_nodes) * d);
+  uint16_t* s_rb = reinterpret_cast<uint16_t*>(s_rs + A.patch_max_nodes);
+  float* s_f = reinterpret_cast<float*>(
+      smem + ((size_t(A.patch_max_nodes) * (2 * d * 8 + 4) + size_t(A.patch_max_nodes + 1) * 2 + 15) &
+              ~size_t(15)));
+  for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
+    const int64_t node = A.pnode_ids[k0 + k];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      s_u[k * d + i] = A.u[node * d + i];
+      s_x[k * d + i] = A.X[node * d + i];
+    }
+    s_rs[k] = A.run_start[k0 + k];
+    s_rb[k] = A.run_base[k0 + b + k];
+  }
+  if (threadIdx.x == 0) s_rb[m_nodes] = A.run_base[k0 + b + m_nodes];
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < ne) {
+    const int64_t e = e0 + t;
+    const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+    int ln[n];
+#pragma unroll
+    for (int a = 0; a < n; ++a) ln[a] = A.lconn[a * A.E + e];
+    float du[dn], dX[dn];
+#pragma unroll
+    for (int i = 0; i < d; ++i) { du[i] = 0.f; dX[i] = 0.f; }
+#pragma unroll
+    for (int a = 1; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        du[i + a * d] = float(s_u[ln[a] * d + i] - s_u[ln[0] * d + i]);
+        dX[i + a * d] = float(s_x[ln[a] * d + i] - s_x[ln[0] * d + i]);
+      }
+    float fe[dn];
+    mixed_element_forces<ET, KIND, true>(A, e, m, du, dX, fe);
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      const int q = s_rb[ln[a]] + A.lrank[a * A.E + e];
+#pragma unroll
+      for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
+    }
+  }
+  __syncthreads();
+  // contiguous runs: output q belongs to the node k with rb[k] <= q < rb[k+1]
+  float* __restrict__ out = static_cast<float*>(A.slots);
+  const int total = ne * n;
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    int lo = 0, hi = int(m_nodes);  // s_rb[lo] <= q < s_rb[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_rb[mid] <= q) lo = mid; else hi = mid;
+    }
+    const int64_t p = int64_t(s_rs[lo]) + (q - s_rb[lo]);
+#pragma unroll
+    for (int i = 0; i < d; ++i) out[p * d + i] = s_f[q * d + i];
+  }
 }
 
 // ===========================================================================
@@ -275,93 +483,139 @@ __global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
 // ===========================================================================
 template <int D, typename TS, bool HAS_FEXT>
 __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
-  const int64_t node = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
-  if (node >= A.N) return;
+  // The block's nodal state is staged through LDS: global loads and stores of
+  // u, v, a, f_ext and the blocked flags are contiguous runs of doubles (fully
+  // coalesced, each array written once), while each thread works on one node.
+  constexpr int NBMAX = 256;
+  __shared__ double s_u[NBMAX * D], s_v[NBMAX * D], s_a[NBMAX * D], s_f[HAS_FEXT ? NBMAX * D : 1];
+  __shared__ uint8_t s_b[NBMAX * D];
+  const int nb = blockDim.x;
+  const int64_t n0 = logical_block(A.xcd_remap) * nb;
+  if (n0 >= A.N) return;  // uniform over the block
+  const int cnt = int(min<int64_t>(nb, A.N - n0));
+  const int64_t k0 = n0 * D;
+  const int nk = cnt * D;
   const int mode = A.mode;
+  const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
+  for (int t = threadIdx.x; t < nk; t += nb) {
+    s_u[t] = A.u[k0 + t];
+    s_v[t] = A.v[k0 + t];
+    s_a[t] = A.a[k0 + t];
+    s_b[t] = A.blocked[k0 + t];
+    if constexpr (HAS_FEXT) s_f[t] = A.fext[k0 + t];
+  }
+  __syncthreads();
+  const int ln = threadIdx.x;
+  const int64_t node = n0 + ln;
   const double dt = A.dt;
-  if (mode & (NODE_CORRECT | NODE_GATHER)) {
-    const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
-    double f[D], c[D];
+  if (ln < cnt) {
+    if (mode & (NODE_CORRECT | NODE_GATHER)) {
+      const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
+      double f[D], c[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) { f[i] = 0.; c[i] = 0.; }
-    const uint32_t p0 = A.csr_off[node], p1 = A.csr_off[node + 1];
-    auto add = [&](int i, TS raw) {
-      const double x = -1. * double(raw);
-      if constexpr (sizeof(TS) == 8) {
-        f[i] += x;
-      } else {
-        const double tsum = f[i] + x;
-        c[i] += (fabs(f[i]) >= fabs(x)) ? (f[i] - tsum) + x : (x - tsum) + f[i];
-        f[i] = tsum;
+      for (int i = 0; i < D; ++i) { f[i] = 0.; c[i] = 0.; }
+      const uint32_t p0 = A.csr_off[node], p1 = A.csr_off[node + 1];
+      auto add = [&](int i, TS raw) {
+        const double x = -1. * double(raw);
+        if constexpr (sizeof(TS) == 8) {
+          f[i] += x;
+        } else {
+          const double tsum = f[i] + x;
+          c[i] += (fabs(f[i]) >= fabs(x)) ? (f[i] - tsum) + x : (x - tsum) + f[i];
+          f[i] = tsum;
+        }
+      };
+      // Batches of CHUNK entries: all loads of a batch are issued before the
+      // (in-order) accumulation, so the gathers overlap instead of serialising.
+      constexpr int CHUNK = 8;
+      uint32_t p = p0;
+      if (A.node_ordered) {
+        // the node's contributions are contiguous: slots[p*D ...] for p in [p0, p1)
+        for (; p + CHUNK <= p1; p += CHUNK) {
+          TS x[CHUNK * D];
+#pragma unroll
+          for (int j = 0; j < CHUNK * D; ++j) x[j] = slots[int64_t(p) * D + j];
+#pragma unroll
+          for (int j = 0; j < CHUNK; ++j)
+#pragma unroll
+            for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
+        }
+        for (; p < p1; ++p)
+#pragma unroll
+          for (int i = 0; i < D; ++i) add(i, slots[int64_t(p) * D + i]);
       }
-    };
-    // Batches of CHUNK entries: all loads of a batch are issued before the
-    // (in-order) accumulation, so the gathers overlap instead of serialising.
-    constexpr int CHUNK = 8;
-    uint32_t p = p0;
-    for (; p + CHUNK <= p1; p += CHUNK) {
-      uint32_t s[CHUNK];
+      for (; p + CHUNK <= p1; p += CHUNK) {
+        uint32_t s[CHUNK];
 #pragma unroll
-      for (int j = 0; j < CHUNK; ++j) s[j] = A.csr_slot[p + j];
-      TS x[CHUNK][D];
+        for (int j = 0; j < CHUNK; ++j) s[j] = A.csr_slot[p + j];
+        TS x[CHUNK][D];
 #pragma unroll
-      for (int j = 0; j < CHUNK; ++j)
+        for (int j = 0; j < CHUNK; ++j)
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
+          for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
 #pragma unroll
-      for (int j = 0; j < CHUNK; ++j)
+        for (int j = 0; j < CHUNK; ++j)
 #pragma unroll
-        for (int i = 0; i < D; ++i) add(i, x[j][i]);
-    }
-    for (; p < p1; ++p) {
-      const int64_t s = A.csr_slot[p];
-#pragma unroll
-      for (int i = 0; i < D; ++i) add(i, slots[s * D + i]);
-    }
-    const double M = A.mass[node];
-    const double dcoef = 0.5 * dt;  // beta dt
-    const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      const int64_t k = node * D + i;
-      double fi = f[i];
-      if constexpr (sizeof(TS) != 8) fi = f[i] + c[i];
-      double r = 0.;
-      if constexpr (HAS_FEXT) r += A.fext[k];
-      r += fi;
-      double cache = 0.;
-      cache += -1. * A.a[k];
-      r += M * cache;
-      if (mode & NODE_STORE_FINT) A.fint_out[k] = fi;
-      if (!(mode & NODE_CORRECT)) continue;
-      if (!(A.blocked[k] & 1)) {
-        const double delta = r / M;
-        A.u[k] += ecoef * delta;
-        A.v[k] += dcoef * delta;
-        A.a[k] += delta;
+          for (int i = 0; i < D; ++i) add(i, x[j][i]);
       }
-      if (mode & NODE_STORE_INC) {
-        double inc = A.u[k];
-        inc -= A.prev[k];
-        A.inc[k] = inc;
+      for (; p < p1; ++p) {
+        const int64_t s = A.csr_slot[p];
+#pragma unroll
+        for (int i = 0; i < D; ++i) add(i, slots[s * D + i]);
+      }
+      const double M = A.mass[node];
+      const double dcoef = 0.5 * dt;            // beta dt
+      const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const int l = ln * D + i;
+        double fi = f[i];
+        if constexpr (sizeof(TS) != 8) fi = f[i] + c[i];
+        if (mode & NODE_STORE_FINT) A.fint_out[node * D + i] = fi;
+        if (!correct) continue;
+        double r = 0.;
+        if constexpr (HAS_FEXT) r += s_f[l];
+        r += fi;
+        double cache = 0.;
+        cache += -1. * s_a[l];
+        r += M * cache;
+        if (!(s_b[l] & 1)) {
+          const double delta = r / M;
+          s_u[l] += ecoef * delta;
+          s_v[l] += dcoef * delta;
+          s_a[l] += delta;
+        }
+        if (mode & NODE_STORE_INC) {
+          double inc = s_u[l];
+          inc -= A.prev[node * D + i];
+          A.inc[node * D + i] = inc;
+        }
+      }
+    }
+    if (predict) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const int l = ln * D + i;
+        const uint8_t b = s_b[l];
+        double u = s_u[l];
+        if (b & 2) u += A.bc_inc[i];
+        if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = u;
+        if (!(b & 1)) {
+          const double v = s_v[l], acc = s_a[l];
+          const double dt_a_n = dt * acc;
+          u += dt * v + 0.5 * dt * dt_a_n;
+          s_v[l] = v + dt_a_n;
+        }
+        s_u[l] = u;
       }
     }
   }
-  if (mode & NODE_PREDICT) {
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      const int64_t k = node * D + i;
-      const uint8_t b = A.blocked[k];
-      double u = A.u[k];
-      if (b & 2) u += A.bc_inc[i];
-      if (mode & NODE_SAVE_PREV) A.prev[k] = u;
-      if (!(b & 1)) {
-        const double v = A.v[k], acc = A.a[k];
-        const double dt_a_n = dt * acc;
-        u += dt * v + 0.5 * dt * dt_a_n;
-        A.v[k] = v + dt_a_n;
-      }
-      A.u[k] = u;
+  __syncthreads();
+  if (correct || predict) {
+    for (int t = threadIdx.x; t < nk; t += nb) {
+      A.u[k0 + t] = s_u[t];
+      A.v[k0 + t] = s_v[t];
+      if (correct) A.a[k0 + t] = s_a[t];
     }
   }
 }
@@ -681,11 +935,29 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
     case NH_DAMAGE: k_element_fp64<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
     default: return hipErrorInvalidValue;
     }
+  } else if (A.patch) {
+    // one block per patch of 256 elements (the patches are built for that size)
+    const unsigned gp = unsigned(A.patch_blocks);
+    switch (kind) {
+    case ELASTIC: k_element_patch<ET, ELASTIC><<<gp, 256, A.patch_lds, s>>>(A); break;
+    case NEOHOOKEAN: k_element_patch<ET, NEOHOOKEAN><<<gp, 256, A.patch_lds, s>>>(A); break;
+    case NH_DAMAGE: k_element_patch<ET, NH_DAMAGE><<<gp, 256, A.patch_lds, s>>>(A); break;
+    default: return hipErrorInvalidValue;
+    }
   } else {
     switch (kind) {
-    case ELASTIC: k_element_mixed<ET, ELASTIC><<<g, block, 0, s>>>(A); break;
-    case NEOHOOKEAN: k_element_mixed<ET, NEOHOOKEAN><<<g, block, 0, s>>>(A); break;
-    case NH_DAMAGE: k_element_mixed<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
+    case ELASTIC:
+      if (A.recompute) k_element_mixed<ET, ELASTIC, true><<<g, block, 0, s>>>(A);
+      else k_element_mixed<ET, ELASTIC, false><<<g, block, 0, s>>>(A);
+      break;
+    case NEOHOOKEAN:
+      if (A.recompute) k_element_mixed<ET, NEOHOOKEAN, true><<<g, block, 0, s>>>(A);
+      else k_element_mixed<ET, NEOHOOKEAN, false><<<g, block, 0, s>>>(A);
+      break;
+    case NH_DAMAGE:
+      if (A.recompute) k_element_mixed<ET, NH_DAMAGE, true><<<g, block, 0, s>>>(A);
+      else k_element_mixed<ET, NH_DAMAGE, false><<<g, block, 0, s>>>(A);
+      break;
     default: return hipErrorInvalidValue;
     }
   }

@@ -1,0 +1,675 @@
+// akantu.cc -- the akantu:: façade over the C ABI of libakx.so.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <random>
+#include <set>
+#include <sstream>
+
+#include "akantu/solid_mechanics_model.hh"
+#include "akx.h"
+
+namespace akantu {
+
+/* ------------------------------------------------------------------------ */
+/* common                                                                    */
+/* ------------------------------------------------------------------------ */
+UInt getSpatialDimension(ElementType t) {
+  return (t == _tetrahedron_4 || t == _hexahedron_8) ? 3 : 2;
+}
+UInt getNbNodesPerElement(ElementType t) {
+  return t == _triangle_3 ? 3 : t == _hexahedron_8 ? 8 : 4;
+}
+UInt getNbIntegrationPoints(ElementType t) {
+  return t == _quadrangle_4 ? 4 : t == _hexahedron_8 ? 8 : 1;
+}
+
+Parser& getStaticParser() {
+  static Parser p;
+  return p;
+}
+
+void initialize(int& argc, char**& argv) {
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a.rfind("--aka_seed=", 0) == 0) getStaticParser().setSeed(std::stol(a.substr(11)));
+    else if (a == "--aka_seed" && i + 1 < argc) getStaticParser().setSeed(std::stol(argv[++i]));
+  }
+}
+
+void initialize(const std::string& input_file, int& argc, char**& argv) {
+  if (!input_file.empty()) getStaticParser().parse(input_file);
+  initialize(argc, argv);
+}
+
+void finalize() {}
+
+/* ------------------------------------------------------------------------ */
+/* Parser: `material <type> [ key = value ... ]` sections, `seed = N`         */
+/* ------------------------------------------------------------------------ */
+static std::string trim(const std::string& s) {
+  size_t b = s.find_first_not_of(" \t\r\n"), e = s.find_last_not_of(" \t\r\n");
+  return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+}
+
+void Parser::parse(const std::string& filename) {
+  std::ifstream f(filename);
+  if (!f) AKANTU_EXCEPTION("cannot open input file " << filename);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  parseString(ss.str());
+}
+
+void Parser::parseString(const std::string& text) {
+  // strip comments
+  std::string clean;
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    auto h = line.find('#');
+    if (h != std::string::npos) line = line.substr(0, h);
+    clean += line + "\n";
+  }
+  size_t pos = 0;
+  while (pos < clean.size()) {
+    size_t m = clean.find("material", pos);
+    // top-level `seed = N` before the next section
+    std::string head = clean.substr(pos, m == std::string::npos ? std::string::npos : m - pos);
+    std::istringstream hs(head);
+    while (std::getline(hs, line)) {
+      auto eq = line.find('=');
+      if (eq != std::string::npos && trim(line.substr(0, eq)) == "seed")
+        seed_ = std::stol(trim(line.substr(eq + 1)));
+    }
+    if (m == std::string::npos) break;
+    size_t open = clean.find('[', m);
+    if (open == std::string::npos) AKANTU_EXCEPTION("material section without '['");
+    ParserSection sec;
+    sec.type = trim(clean.substr(m + 8, open - m - 8));
+    // matching ']' (random parameters have their own brackets)
+    int depth = 0;
+    size_t close = open;
+    for (; close < clean.size(); ++close) {
+      if (clean[close] == '[') ++depth;
+      if (clean[close] == ']' && --depth == 0) break;
+    }
+    if (close >= clean.size()) AKANTU_EXCEPTION("material section without ']'");
+    std::istringstream body(clean.substr(open + 1, close - open - 1));
+    while (std::getline(body, line)) {
+      auto eq = line.find('=');
+      if (eq == std::string::npos) continue;
+      sec.params[trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+    }
+    sections_.push_back(sec);
+    pos = close + 1;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh                                                                      */
+/* ------------------------------------------------------------------------ */
+Mesh::Mesh(UInt spatial_dimension, const ID& id) : dim_(spatial_dimension), id_(id) {
+  nodes_.resize(0, spatial_dimension);
+}
+
+void Mesh::setNodes(const Array<Real>& nodes) {
+  if (nodes.getNbComponent() != dim_) AKANTU_EXCEPTION("nodes must have " << dim_ << " components");
+  nodes_.copy(nodes);
+}
+
+void Mesh::setConnectivity(ElementType type, const Array<UInt>& conn) {
+  if (getSpatialDimension(type) != dim_) AKANTU_EXCEPTION("element type of another dimension");
+  if (conn.getNbComponent() != getNbNodesPerElement(type)) AKANTU_EXCEPTION("bad connectivity width");
+  type_ = type;
+  conn_.copy(conn);
+}
+
+UInt Mesh::getNbElement(ElementType type) const { return type == type_ ? conn_.size() : 0; }
+
+const Array<UInt>& Mesh::getConnectivity(ElementType type) const {
+  if (type != type_) AKANTU_EXCEPTION("mesh has no elements of type " << int(type));
+  return conn_;
+}
+
+std::vector<UInt>& Mesh::createNodeGroup(const std::string& name) { return groups_[name]; }
+
+const std::vector<UInt>& Mesh::getNodeGroup(const std::string& name) const {
+  auto it = groups_.find(name);
+  if (it == groups_.end()) AKANTU_EXCEPTION("no node group " << name);
+  return it->second;
+}
+
+Real Mesh::getLowerBound(UInt axis) const {
+  Real v = std::numeric_limits<Real>::max();
+  for (UInt n = 0; n < nodes_.size(); ++n) v = std::min(v, nodes_(n, axis));
+  return v;
+}
+Real Mesh::getUpperBound(UInt axis) const {
+  Real v = -std::numeric_limits<Real>::max();
+  for (UInt n = 0; n < nodes_.size(); ++n) v = std::max(v, nodes_(n, axis));
+  return v;
+}
+
+void Mesh::createGroupsFromBoundingBox(Real tol) {
+  const char* names[3] = {"x", "y", "z"};
+  for (UInt k = 0; k < dim_; ++k) {
+    Real lo = getLowerBound(k), hi = getUpperBound(k), span = std::max(hi - lo, Real(1));
+    auto& gl = groups_[std::string(names[k]) + "_min"];
+    auto& gh = groups_[std::string(names[k]) + "_max"];
+    gl.clear();
+    gh.clear();
+    for (UInt n = 0; n < nodes_.size(); ++n) {
+      if (std::abs(nodes_(n, k) - lo) <= tol * span) gl.push_back(n);
+      if (std::abs(nodes_(n, k) - hi) <= tol * span) gh.push_back(n);
+    }
+  }
+}
+
+void Mesh::createBoundaryGroup() {
+  static const std::vector<std::vector<int>> facets[4] = {
+      {{0, 1}, {1, 2}, {2, 0}},
+      {{0, 1}, {1, 2}, {2, 3}, {3, 0}},
+      {{0, 2, 1}, {1, 2, 3}, {2, 0, 3}, {0, 1, 3}},
+      {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}}};
+  std::map<std::vector<UInt>, int> count;
+  const auto& fl = facets[type_];
+  for (UInt e = 0; e < conn_.size(); ++e)
+    for (auto& f : fl) {
+      std::vector<UInt> key;
+      for (int a : f) key.push_back(conn_(e, a));
+      std::sort(key.begin(), key.end());
+      count[key]++;
+    }
+  std::set<UInt> nodes;
+  for (auto& kv : count)
+    if (kv.second == 1) nodes.insert(kv.first.begin(), kv.first.end());
+  groups_["boundary"].assign(nodes.begin(), nodes.end());
+}
+
+void Mesh::read(const std::string& filename) {
+  std::ifstream f(filename);
+  if (!f) AKANTU_EXCEPTION("cannot open mesh file " << filename);
+  std::string line;
+  std::vector<long> ids;
+  std::vector<std::array<Real, 3>> coords;
+  struct El { int type, phys; std::vector<long> nodes; };
+  std::vector<El> els;
+  while (std::getline(f, line)) {
+    line = trim(line);
+    if (line == "$Nodes") {
+      std::getline(f, line);
+      long n = std::stol(line);
+      for (long k = 0; k < n; ++k) {
+        std::getline(f, line);
+        std::istringstream s(line);
+        long id;
+        std::array<Real, 3> c{0, 0, 0};
+        s >> id >> c[0] >> c[1] >> c[2];
+        ids.push_back(id);
+        coords.push_back(c);
+      }
+    } else if (line == "$Elements") {
+      std::getline(f, line);
+      long n = std::stol(line);
+      for (long k = 0; k < n; ++k) {
+        std::getline(f, line);
+        std::istringstream s(line);
+        long id, t, ntags;
+        s >> id >> t >> ntags;
+        std::vector<long> tags(ntags);
+        for (auto& x : tags) s >> x;
+        El el{int(t), ntags ? int(tags[0]) : 0, {}};
+        long v;
+        while (s >> v) el.nodes.push_back(v);
+        els.push_back(el);
+      }
+    }
+  }
+  // gmsh codes (mesh_io_msh.cc:181-199): 2 tri3, 3 quad4, 4 tet4, 5 hex8
+  std::map<int, ElementType> code = {{2, _triangle_3}, {3, _quadrangle_4}, {4, _tetrahedron_4},
+                                     {5, _hexahedron_8}};
+  std::map<long, UInt> id_map;
+  for (size_t k = 0; k < ids.size(); ++k) id_map[ids[k]] = UInt(k);
+  int found = -1;
+  for (auto& el : els)
+    if (code.count(el.type) && getSpatialDimension(code[el.type]) == dim_) {
+      if (found >= 0 && found != el.type) AKANTU_EXCEPTION("mixed element types are not supported");
+      found = el.type;
+    }
+  if (found < 0) AKANTU_EXCEPTION("no elements of dimension " << dim_ << " in " << filename);
+  type_ = code[found];
+  nodes_.resize(UInt(ids.size()), dim_);
+  for (size_t k = 0; k < ids.size(); ++k)
+    for (UInt j = 0; j < dim_; ++j) nodes_(UInt(k), j) = coords[k][j];
+  const UInt nn = getNbNodesPerElement(type_);
+  std::vector<UInt> flat;
+  for (auto& el : els) {
+    if (el.type == found)
+      for (UInt a = 0; a < nn; ++a) flat.push_back(id_map.at(el.nodes[a]));
+    if (el.phys) {
+      auto& g = groups_["phys_" + std::to_string(el.phys)];
+      for (long v : el.nodes) g.push_back(id_map.at(v));
+    }
+  }
+  for (auto& kv : groups_) {
+    std::sort(kv.second.begin(), kv.second.end());
+    kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+  }
+  conn_.resize(UInt(flat.size() / nn), nn);
+  std::copy(flat.begin(), flat.end(), conn_.storage());
+}
+
+/* ------------------------------------------------------------------------ */
+/* Materials                                                                 */
+/* ------------------------------------------------------------------------ */
+Material::Material(SolidMechanicsModel& model, const ID& id, int kind)
+    : model_(model), id_(id), name_(id), kind_(kind) {
+  params_ = {{"rho", 0.}, {"E", 0.}, {"nu", 0.}, {"Plane_Stress", 0.}};
+  if (kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
+    params_["Sd"] = 5000.;  // material_marigo.cc:44
+    params_["Yd"] = 50.;
+  }
+}
+
+Real Material::getParam(const std::string& key) const {
+  auto it = params_.find(key);
+  if (it == params_.end()) AKANTU_EXCEPTION("material " << name_ << " has no parameter " << key);
+  return it->second;
+}
+
+void Material::setParam(const std::string& key, Real value) {
+  if (!params_.count(key)) AKANTU_EXCEPTION("material " << name_ << " has no parameter " << key);
+  params_[key] = value;
+}
+
+void Material::setRandomParam(const std::string& key, Real base, const std::string& dist, Real p0,
+                              Real p1) {
+  if (!params_.count(key)) AKANTU_EXCEPTION("material " << name_ << " has no parameter " << key);
+  params_[key] = base;
+  random_[key] = RandomSpec{base, p0, p1, dist};
+}
+
+Real Material::getLambda() const {
+  const Real E = getParam("E"), nu = getParam("nu");
+  if (model_.getSpatialDimension() == 2 && getParam("Plane_Stress") != 0 && kind_ == AKX_MATERIAL_ELASTIC)
+    return nu * E / ((1 + nu) * (1 - nu));
+  return nu * E / ((1 + nu) * (1 - 2 * nu));
+}
+Real Material::getMu() const { return getParam("E") / (2 * (1 + getParam("nu"))); }
+Real Material::getPushWaveSpeed() const {
+  return std::sqrt((getLambda() + 2 * getMu()) / getParam("rho"));
+}
+
+const Array<Real>& Material::getStress(ElementType) {
+  model_.downloadPointField(AKX_STRESS, stress_, *this);
+  return stress_;
+}
+const Array<Real>& Material::getGradU(ElementType) {
+  model_.downloadPointField(AKX_GRAD_U, gradu_, *this);
+  return gradu_;
+}
+const Array<Real>& Material::getDamage(ElementType) {
+  model_.downloadPointField(AKX_DAMAGE, damage_, *this);
+  return damage_;
+}
+Real Material::getPotentialEnergy() {
+  if (model_.getNbMaterials() != 1)
+    AKANTU_EXCEPTION("per-material potential energy needs a single material");
+  return model_.getEnergy("potential");
+}
+Real Material::getEnergy(const std::string& id) {
+  if (id == "potential") return getPotentialEnergy();
+  return 0.;
+}
+
+MaterialFactory::MaterialFactory() {
+  auto make = [](int kind) {
+    return [kind](const ID& id, SolidMechanicsModel& m) {
+      return std::unique_ptr<Material>(new Material(m, id, kind));
+    };
+  };
+  allocators_["elastic"] = make(AKX_MATERIAL_ELASTIC);
+  allocators_["neohookean"] = make(AKX_MATERIAL_NEOHOOKEAN);
+  allocators_["neohookean_damage"] = make(AKX_MATERIAL_NEOHOOKEAN_DAMAGE);
+}
+
+MaterialFactory& MaterialFactory::getInstance() {
+  static MaterialFactory f;
+  return f;
+}
+
+void MaterialFactory::registerAllocator(const std::string& name, Allocator alloc) {
+  allocators_[name] = std::move(alloc);
+}
+
+std::unique_ptr<Material> MaterialFactory::allocate(const std::string& name, const ID& id,
+                                                    SolidMechanicsModel& model) const {
+  auto it = allocators_.find(name);
+  if (it == allocators_.end()) AKANTU_EXCEPTION("unknown material type " << name);
+  return it->second(id, model);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SolidMechanicsModel                                                       */
+/* ------------------------------------------------------------------------ */
+void SolidMechanicsModel::check(int rc, const char* what) const {
+  if (rc != AKX_OK)
+    AKANTU_EXCEPTION(what << " failed (" << rc << "): " << akx_last_error(ctx_));
+}
+
+SolidMechanicsModel::SolidMechanicsModel(Mesh& mesh, UInt spatial_dimension, const ID& id,
+                                         int device)
+    : mesh_(mesh), dim_(spatial_dimension ? spatial_dimension : mesh.getSpatialDimension()),
+      id_(id) {
+  if (dim_ != mesh.getSpatialDimension()) AKANTU_EXCEPTION("dimension mismatch with the mesh");
+  if (mesh.getElementType() == _not_defined) AKANTU_EXCEPTION("the mesh has no elements");
+  if (akx_ctx_create(device, 0, &ctx_) != AKX_OK || !ctx_)
+    AKANTU_EXCEPTION("no usable HIP device " << device << " for libakx");
+  const UInt N = mesh.getNbNodes(), E = mesh.getNbElement();
+  check(akx_mesh_set(ctx_, int(mesh.getElementType()), int(dim_), N, mesh.getNodes().storage(), E,
+                     mesh.getConnectivity(mesh.getElementType()).storage()),
+        "akx_mesh_set");
+  for (Array<Real>* a : {&u_, &v_, &a_, &fext_, &fint_, &mass_, &pos_, &inc_, &prev_})
+    a->resize(N, dim_), a->clear();
+  blocked_.resize(N, dim_);
+  blocked_.clear();
+  element_material_.resize(E, 1);
+  element_material_.clear();
+}
+
+SolidMechanicsModel::~SolidMechanicsModel() {
+  if (ctx_) akx_ctx_destroy(ctx_);
+}
+
+Material& SolidMechanicsModel::registerNewMaterial(const std::string& type,
+                                                   const std::string& name) {
+  auto m = MaterialFactory::getInstance().allocate(type, id_ + ":" + std::to_string(materials_.size()),
+                                                   *this);
+  m->setName(name);
+  materials_.push_back(std::move(m));
+  return *materials_.back();
+}
+
+void SolidMechanicsModel::instantiateMaterials() {
+  // SolidMechanicsModel::instantiateMaterials (solid_mechanics_model_material.cc:44-107)
+  for (const auto& sec : getStaticParser().getMaterialSections()) {
+    Material& m = registerNewMaterial(sec.type, sec.type);
+    for (const auto& kv : sec.params) {
+      std::string key = kv.first, val = kv.second;
+      if (key == "name") { m.setName(val); continue; }
+      if (key == "finite_deformation") continue;
+      std::string lower = val;
+      std::transform(lower.begin(), lower.end(), lower.begin(), ::tolower);
+      if (lower == "true" || lower == "false") {
+        m.setParam(key, lower == "true" ? 1. : 0.);
+        continue;
+      }
+      auto br = val.find('[');
+      if (br != std::string::npos) {
+        // `base dist [p0, p1]`
+        std::istringstream s(val.substr(0, br));
+        Real base;
+        std::string dist;
+        s >> base >> dist;
+        std::string args = val.substr(br + 1, val.find(']') - br - 1);
+        std::replace(args.begin(), args.end(), ',', ' ');
+        std::istringstream as(args);
+        Real p0 = 0, p1 = 0;
+        as >> p0 >> p1;
+        m.setRandomParam(key, base, dist, p0, p1);
+        continue;
+      }
+      m.setParam(key, std::stod(val));
+    }
+  }
+}
+
+void SolidMechanicsModel::setMaterialIndex(const Array<UInt>& em) {
+  if (em.size() != mesh_.getNbElement()) AKANTU_EXCEPTION("one material index per element");
+  element_material_.copy(em);
+  custom_assignment_ = true;
+}
+
+void SolidMechanicsModel::initFull(const AnalysisMethodArg& method) {
+  if (method.value != _explicit_lumped_mass)
+    AKANTU_EXCEPTION("only _explicit_lumped_mass is implemented on the GPU path");
+  if (materials_.empty()) instantiateMaterials();
+  if (materials_.empty()) AKANTU_EXCEPTION("no material defined (input file or registerNewMaterial)");
+  const UInt E = mesh_.getNbElement(), Q = getNbIntegrationPoints(mesh_.getElementType());
+  for (auto& m : materials_) m->filter_.resize(0, 1);
+  std::vector<std::vector<UInt>> filt(materials_.size());
+  for (UInt e = 0; e < E; ++e) {
+    UInt k = element_material_(e);
+    if (k >= materials_.size()) AKANTU_EXCEPTION("element " << e << " has no material " << k);
+    filt[k].push_back(e);
+  }
+  // std::default_random_engine seeded as the reference's RandomGenerator
+  // (aka_random_generator.hh:45-96); draws in element-filter then point order.
+  std::default_random_engine gen(static_cast<unsigned>(getStaticParser().getSeed()));
+  for (size_t k = 0; k < materials_.size(); ++k) {
+    Material& m = *materials_[k];
+    m.filter_.resize(UInt(filt[k].size()), 1);
+    std::copy(filt[k].begin(), filt[k].end(), m.filter_.storage());
+    int mid = -1;
+    check(akx_material_add(ctx_, m.kind_, m.getParam("E"), m.getParam("nu"), m.getParam("rho"),
+                           int(m.getParam("Plane_Stress") != 0), &mid),
+          "akx_material_add");
+    if (m.kind_ == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
+      std::vector<Real> yd(size_t(E) * Q, 0.);
+      auto it = m.random_.find("Yd");
+      for (UInt e : filt[k])
+        for (UInt q = 0; q < Q; ++q) {
+          Real v = m.getParam("Yd");
+          if (it != m.random_.end()) {
+            const auto& r = it->second;
+            if (r.dist == "weibull")
+              v = r.base + std::weibull_distribution<Real>(r.p1, r.p0)(gen);
+            else if (r.dist == "uniform")
+              v = r.base + std::uniform_real_distribution<Real>(r.p0, r.p1)(gen);
+            else
+              AKANTU_EXCEPTION("unsupported random distribution " << r.dist);
+          }
+          yd[size_t(e) * Q + q] = v;
+        }
+      check(akx_material_damage(ctx_, mid, m.getParam("Sd"), yd.data()), "akx_material_damage");
+    }
+  }
+  std::vector<int32_t> em(E);
+  for (UInt e = 0; e < E; ++e) em[e] = int32_t(element_material_(e));
+  check(akx_assign_materials(ctx_, em.data()), "akx_assign_materials");
+  check(akx_init(ctx_), "akx_init");
+  initialized_ = true;
+  for (bool& s : host_stale_) s = true;
+  host_stale_[kFext] = host_stale_[kBlocked] = false;
+  for (bool& d : host_dirty_) d = false;
+}
+
+static int field_of(int id) {
+  static const int map[] = {AKX_DISPLACEMENT,   AKX_VELOCITY, AKX_ACCELERATION, AKX_EXTERNAL_FORCE,
+                            AKX_INTERNAL_FORCE, AKX_BLOCKED_DOFS, AKX_MASS, AKX_CURRENT_POSITION,
+                            AKX_DISPLACEMENT_INCREMENT, AKX_PREVIOUS_DISPLACEMENT};
+  return map[id];
+}
+
+void SolidMechanicsModel::pull(NodalId id) {
+  if (!host_stale_[id]) return;
+  const size_t nd = size_t(mesh_.getNbNodes()) * dim_;
+  if (id == kBlocked) {
+    std::vector<uint8_t> b(nd);
+    check(akx_field_download(ctx_, AKX_BLOCKED_DOFS, b.data(), nd), "akx_field_download");
+    for (size_t k = 0; k < nd; ++k) blocked_[k] = b[k] != 0;
+  } else {
+    Array<Real>* arr[] = {&u_, &v_, &a_, &fext_, &fint_, nullptr, &mass_, &pos_, &inc_, &prev_};
+    check(akx_field_download(ctx_, field_of(id), arr[id]->storage(), nd * 8), "akx_field_download");
+  }
+  host_stale_[id] = false;
+}
+
+void SolidMechanicsModel::pushDirty() {
+  const size_t nd = size_t(mesh_.getNbNodes()) * dim_;
+  Array<Real>* arr[] = {&u_, &v_, &a_, &fext_};
+  for (int id : {kU, kV, kA, kFext}) {
+    if (!host_dirty_[id]) continue;
+    check(akx_field_upload(ctx_, field_of(id), arr[id]->storage(), nd * 8), "akx_field_upload");
+    host_dirty_[id] = false;
+  }
+  if (host_dirty_[kBlocked]) {
+    std::vector<uint8_t> b(nd);
+    for (size_t k = 0; k < nd; ++k) b[k] = blocked_[k] ? 1 : 0;
+    check(akx_field_upload(ctx_, AKX_BLOCKED_DOFS, b.data(), nd), "akx_field_upload");
+    host_dirty_[kBlocked] = false;
+  }
+  // a new displacement changes the current position
+  host_stale_[kPos] = true;
+}
+
+Array<Real>& SolidMechanicsModel::getDisplacement() { pull(kU); host_dirty_[kU] = true; return u_; }
+Array<Real>& SolidMechanicsModel::getVelocity() { pull(kV); host_dirty_[kV] = true; return v_; }
+Array<Real>& SolidMechanicsModel::getAcceleration() { pull(kA); host_dirty_[kA] = true; return a_; }
+Array<Real>& SolidMechanicsModel::getExternalForce() {
+  pull(kFext);
+  host_dirty_[kFext] = true;
+  return fext_;
+}
+Array<bool>& SolidMechanicsModel::getBlockedDOFs() {
+  pull(kBlocked);
+  host_dirty_[kBlocked] = true;
+  return blocked_;
+}
+const Array<Real>& SolidMechanicsModel::getInternalForce() { pull(kFint); return fint_; }
+const Array<Real>& SolidMechanicsModel::getMass() {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pull(kMass);
+  return mass_;
+}
+const Array<Real>& SolidMechanicsModel::getCurrentPosition() {
+  pushDirty();
+  host_stale_[kPos] = true;
+  pull(kPos);
+  return pos_;
+}
+const Array<Real>& SolidMechanicsModel::getIncrement() { pull(kInc); return inc_; }
+const Array<Real>& SolidMechanicsModel::getPreviousDisplacement() { pull(kPrev); return prev_; }
+
+Real SolidMechanicsModel::getStableTimeStep() {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  Real dt = 0;
+  check(akx_stable_time_step(ctx_, &dt), "akx_stable_time_step");
+  return dt;
+}
+
+void SolidMechanicsModel::setTimeStep(Real dt) {
+  check(akx_set_time_step(ctx_, dt), "akx_set_time_step");
+  time_step_ = dt;
+}
+
+void SolidMechanicsModel::solveStep() { solveStep(1); }
+
+void SolidMechanicsModel::solveStep(UInt nsteps) {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  check(akx_solve_step(ctx_, int(nsteps), int(precision_)), "akx_solve_step");
+  for (int id : {kU, kV, kA, kFint, kPos, kInc, kPrev}) host_stale_[id] = true;
+}
+
+void SolidMechanicsModel::assembleInternalForces() {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  check(akx_assemble_internal_forces(ctx_, int(precision_)), "akx_assemble_internal_forces");
+  host_stale_[kFint] = true;
+}
+
+Real SolidMechanicsModel::getEnergy(const std::string& id) {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  int kind = id == "kinetic" ? AKX_ENERGY_KINETIC
+             : id == "potential" ? AKX_ENERGY_POTENTIAL
+             : id == "external work" ? AKX_ENERGY_EXTERNAL_WORK : -1;
+  if (kind < 0) AKANTU_EXCEPTION("unknown energy " << id);
+  Real v = 0;
+  check(akx_energy(ctx_, kind, &v), "akx_energy");
+  return v;
+}
+
+void SolidMechanicsModel::applyBC(const BC::Dirichlet::FixedValue& f, const std::string& group) {
+  auto& u = getDisplacement();
+  auto& b = getBlockedDOFs();
+  for (UInt n : mesh_.getNodeGroup(group)) {
+    b(n, f.axis) = true;
+    u(n, f.axis) = f.value;
+  }
+}
+
+void SolidMechanicsModel::applyBC(const BC::Dirichlet::FlagOnly& f, const std::string& group) {
+  auto& b = getBlockedDOFs();
+  for (UInt n : mesh_.getNodeGroup(group)) b(n, f.axis) = true;
+}
+
+void SolidMechanicsModel::applyBC(const BC::Dirichlet::IncrementValue& f,
+                                  const std::string& group) {
+  auto& u = getDisplacement();
+  auto& b = getBlockedDOFs();
+  for (UInt n : mesh_.getNodeGroup(group)) {
+    b(n, f.axis) = true;
+    u(n, f.axis) += f.value;
+  }
+}
+
+void SolidMechanicsModel::applyBC(const BC::Neumann::FromTraction& f, const std::string& group) {
+  if (f.traction.size() != dim_) AKANTU_EXCEPTION("traction must have " << dim_ << " components");
+  auto& fe = getExternalForce();
+  for (UInt n : mesh_.getNodeGroup(group))
+    for (UInt i = 0; i < dim_; ++i) fe(n, i) += f.traction[i];
+}
+
+void SolidMechanicsModel::setDirichletIncrement(const std::string& group, SpatialDirection axis,
+                                                Real increment) {
+  pushDirty();
+  const auto& g = mesh_.getNodeGroup(group);
+  std::vector<int64_t> nodes(g.begin(), g.end());
+  check(akx_set_dirichlet_increment(ctx_, int64_t(nodes.size()), nodes.data(), int(axis), increment),
+        "akx_set_dirichlet_increment");
+  host_stale_[kBlocked] = true;
+}
+
+Material& SolidMechanicsModel::getMaterial(UInt i) {
+  if (i >= materials_.size()) AKANTU_EXCEPTION("no material " << i);
+  return *materials_[i];
+}
+
+Material& SolidMechanicsModel::getMaterial(const std::string& name) {
+  for (auto& m : materials_)
+    if (m->getName() == name) return *m;
+  AKANTU_EXCEPTION("no material named " << name);
+}
+
+void SolidMechanicsModel::autotune(UInt nsteps) {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  check(akx_autotune(ctx_, int(precision_), int(nsteps), nullptr, nullptr), "akx_autotune");
+}
+
+void SolidMechanicsModel::getLastStepTiming(Real& element_ms, Real& node_ms) {
+  check(akx_last_step_timing(ctx_, &element_ms, &node_ms), "akx_last_step_timing");
+}
+
+void SolidMechanicsModel::downloadPointField(int field, Array<Real>& out, const Material& m) {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  pushDirty();
+  const ElementType t = mesh_.getElementType();
+  const UInt E = mesh_.getNbElement(), Q = getNbIntegrationPoints(t);
+  const UInt comp = field == AKX_DAMAGE ? 1 : dim_ * dim_;
+  std::vector<Real> all(size_t(E) * Q * comp);
+  check(akx_field_download(ctx_, field, all.data(), all.size() * 8), "akx_field_download");
+  const auto& f = m.getElementFilter();
+  out.resize(f.size() * Q, comp);
+  for (UInt k = 0; k < f.size(); ++k)
+    std::memcpy(out.storage() + size_t(k) * Q * comp, all.data() + size_t(f(k)) * Q * comp,
+                sizeof(Real) * Q * comp);
+}
+
+}  // namespace akantu

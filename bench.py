@@ -36,15 +36,30 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False):
+def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
+                      patch_nodes=None, patch_blocks=0):
     """Compulsory HBM bytes of one step, each array touched once.
-    pe: bytes of the element tables and force slots (4 mixed, 8 FP64)."""
-    elem = (E * n * 4            # connectivity
-            + E * Q * d * n * pe  # B = dN/dX
-            + E * Q * pe          # det(J) w
-            + N * d * 8           # displacement gather
-            + E * n * d * pe)     # element force slots (write)
-    node = ((N + 1) * 4 + E * n * 4  # node -> slot CSR
+    pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
+    tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
+    gather instead of the two tables); node_ordered: the CSR slot positions
+    are read by the element kernel (scatter) instead of the node kernel;
+    patch_nodes: LDS-patch element kernel, which reads 16-bit local
+    connectivity + 8-bit run ranks per (e, a) and, per patch node, its id,
+    run start and run offset (4 + 4 + 2 bytes)."""
+    if patch_nodes is not None:
+        elem = (E * n * (2 + 1)                            # local connectivity, run ranks
+                + patch_nodes * (4 + 4 + 2) + patch_blocks * (4 + 2)  # patch node lists
+                + 2 * N * d * 8                            # u and X of every node
+                + E * n * d * pe)                          # force slots (write)
+        node_ordered = True
+    else:
+        elem = (E * n * 4                                   # connectivity
+                + (E * Q * d * n * pe if tables else 0)      # B = dN/dX
+                + (E * Q * pe if tables else N * d * 8)      # det(J) w, or X gather
+                + N * d * 8                                  # displacement gather
+                + (E * n * 4 if node_ordered else 0)         # slot positions
+                + E * n * d * pe)                            # element force slots (write)
+    node = ((N + 1) * 4 + (0 if node_ordered else E * n * 4)  # node -> slot CSR
             + E * n * d * pe         # force slots (read)
             + 6 * N * d * 8          # u, v, a read + write
             + N * 8                  # lumped mass (one value per node)
@@ -141,6 +156,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 1 GiB device copy (calibration of the traffic counters)")
+    ap.add_argument("--launch-mixed", default=None,
+                    help="element_block,node_block,xcd_remap,mixed_recompute,slot_layout "
+                         "(skips the autotuner for the mixed path)")
+    ap.add_argument("--launch-fp64", default=None, help="same, for the FP64 path")
     args = ap.parse_args()
     if args.copy_only:
         print(json.dumps({"copy_bytes": 1 << 30, "copy_gbs": copy_bandwidth(1 << 30, 3)}))
@@ -166,21 +185,35 @@ def main():
     log("dt=%.6e" % dt)
 
     res = {}
+    shapes = {}
+    keys = ("element_block", "node_block", "xcd_remap", "mixed_recompute", "slot_layout")
+    for name, prec, fixed in (("mixed", akx.MIXED, args.launch_mixed),
+                              ("fp64", akx.FP64, args.launch_fp64)):
+        if fixed:
+            shapes[name] = dict(zip(keys, (int(x) for x in fixed.split(","))))
+            log("%s: fixed launch %s" % (name, shapes[name]))
+        else:
+            shapes[name], tuned_ms = model.autotune(prec, 10)
+            log("%s: autotuned launch %s (%.4f ms/step)" % (name, shapes[name], tuned_ms))
     for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
+        model.set_launch(**shapes[name])
         t = timed(model, prec, args.steps, args.warmup, v0)
         el_ms, nd_ms = kernel_times(model, prec, args.steps, v0)
         res[name] = dict(seconds=t, ms_per_step=1e3 * t / args.steps,
-                         qp_per_s=E * Q * args.steps / t, element_ms=el_ms, node_ms=nd_ms)
+                         qp_per_s=E * Q * args.steps / t, element_ms=el_ms, node_ms=nd_ms,
+                         launch=shapes[name])
         log("%s: %.4f ms/step  (element %.4f ms, node %.4f ms)" %
             (name, res[name]["ms_per_step"], el_ms, nd_ms))
 
     # accuracy of the mixed path against the FP64 path after error-steps steps
     log("error run: %d steps mixed and FP64" % args.error_steps)
     reset(model, v0)
+    model.set_launch(**shapes["mixed"])
     model.solve_step(args.error_steps, akx.MIXED)
     u_mixed = model.get_displacement()
     e_mixed = model.get_energy("kinetic") + model.get_energy("potential")
     reset(model, v0)
+    model.set_launch(**shapes["fp64"])
     model.solve_step(args.error_steps, akx.FP64)
     u_fp64 = model.get_displacement()
     e_fp64 = model.get_energy("kinetic") + model.get_energy("potential")
@@ -190,22 +223,35 @@ def main():
         (args.error_steps, rel_u, rel_e))
 
     # roofline of the dominant kernel (mixed path)
-    eb, nb = algorithmic_bytes(E, N, n, Q, d, 4)
+    patch = shapes["mixed"]["slot_layout"] == 2
+    eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
+                               tables=not shapes["mixed"]["mixed_recompute"],
+                               node_ordered=bool(shapes["mixed"]["slot_layout"]),
+                               patch_nodes=model.query("patch_nodes_total") if patch else None,
+                               patch_blocks=model.query("patch_blocks") if patch else 0)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
+    rc_flag = "true" if shapes["mixed"]["mixed_recompute"] else "false"
     if el_ms >= nd_ms:
-        kname, kbytes, kms = "element kernel (k_element_mixed<TET4, ELASTIC>)", eb, el_ms
+        kname = ("void akx::k_element_patch<2, 0>(akx::ElemArgs)" if patch else
+                 "void akx::k_element_mixed<2, 0, %s>(akx::ElemArgs)" % rc_flag)
+        kbytes, kms = eb, el_ms
     else:
-        kname, kbytes, kms = "node kernel (k_node<3, float>)", nb, nd_ms
+        kname, kbytes, kms = "void akx::k_node<3, float, false>(akx::NodeArgs)", nb, nd_ms
     achieved = kbytes / (kms * 1e-3) / 1e9
+    # HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # runs, valid for the launch shape they were taken with
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_c1_mixed.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_c1.json")
     if os.path.exists(tpath) and args.n == 100:
         with open(tpath) as f:
             tr = json.load(f)
-        traffic = tr.get("element" if el_ms >= nd_ms else "node")
+        k = tr.get("kernels", {}).get(kname)
+        if k and tr.get("launch_mixed") == shapes["mixed"]:
+            traffic = k["fetch_bytes"] + k["write_bytes"]
     step_bytes = eb + nb
     step_gbs = step_bytes / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
-    eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8)
+    eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
+                                   node_ordered=bool(shapes["fp64"]["slot_layout"]))
     copy_gbs = copy_bandwidth()
     log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
 

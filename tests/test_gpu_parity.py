@@ -74,9 +74,12 @@ def test_patch_linear_elastic_explicit_gpu(name, precision):
         assert abs(ekin) < 1e-12 * model.get_energy("potential")
 
 
-def test_bar_c0_fp64_bitwise_1000_steps():
-    """c0 (4,000 triangle_3, 2,121 nodes), 1000 steps: FP64 GPU == oracle."""
+@pytest.mark.parametrize("layout", [0, 1])
+def test_bar_c0_fp64_bitwise_1000_steps(layout):
+    """c0 (4,000 triangle_3, 2,121 nodes), 1000 steps: FP64 GPU == oracle, with
+    element-ordered (0) and node-ordered (1) force slots."""
     p = gh.bar_pair()
+    p.gpu.set_launch(slot_layout=layout)
     p.step(1000, akx.FP64)
     for g, c in (("displacement", "u"), ("velocity", "v"), ("acceleration", "a"),
                  ("internal_force", "f_int")):
@@ -86,10 +89,13 @@ def test_bar_c0_fp64_bitwise_1000_steps():
         assert abs(p.gpu.get_energy(kind) - p.cpu.energy(kind)) <= 1e-12 * abs(p.cpu.energy(kind))
 
 
-def test_bar_c0_mixed_1000_steps():
-    """Mixed path on c0 after 1000 steps: <= 1e-5 relative nodal displacement
-    error and <= 1e-5 relative error of E_kin + E_pot against the oracle."""
+@pytest.mark.parametrize("recompute", [0, 1])
+def test_bar_c0_mixed_1000_steps(recompute):
+    """Mixed path on c0 after 1000 steps (B read from FP32 tables or rebuilt
+    from the coordinates): <= 1e-5 relative nodal displacement error and
+    <= 1e-5 relative error of E_kin + E_pot against the oracle."""
     p = gh.bar_pair()
+    p.gpu.set_launch(mixed_recompute=recompute)
     p.step(1000, akx.MIXED)
     err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
     assert err <= 1e-5, err
@@ -110,9 +116,12 @@ def test_cube_tet4_fp64_bitwise_and_mixed():
     assert err <= 1e-5, err
 
 
-def test_cube_tet4_mixed_1000_steps():
-    """The 1000-step bar of the mixed path on tetrahedron_4 (8^3 cube)."""
+@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 1), (1, 2)])
+def test_cube_tet4_mixed_1000_steps(recompute, layout):
+    """The 1000-step bar of the mixed path on tetrahedron_4 (8^3 cube), with
+    the gather, node-ordered and LDS-patch element kernels."""
     p = gh.cube_pair(8)
+    p.gpu.set_launch(mixed_recompute=recompute, slot_layout=layout)
     p.step(1000, akx.MIXED)
     err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
     assert err <= 1e-5, err
@@ -129,11 +138,13 @@ def test_hex8_neohookean_prescribed_velocity():
     assert gh.rel(p.gpu.get_velocity(), p.cpu.nodal("v")) <= 1e-12
     epg, epc = p.gpu.get_energy("potential"), p.cpu.energy("potential")
     assert abs(epg - epc) <= 1e-12 * abs(epc)
-    q = gh.hex_pair(6)
-    q.gpu.solve_step(300, akx.MIXED)
-    gh.oracle_hex_step(q, 300)
-    err = gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u"))
-    assert err <= 1e-5, err
+    for recompute, layout in ((0, 0), (1, 1), (1, 2)):
+        q = gh.hex_pair(6)
+        q.gpu.set_launch(mixed_recompute=recompute, slot_layout=layout)
+        q.gpu.solve_step(300, akx.MIXED)
+        gh.oracle_hex_step(q, 300)
+        err = gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u"))
+        assert err <= 1e-5, (recompute, err)
 
 
 def test_hex8_damage_law():
@@ -239,12 +250,35 @@ def test_internal_renumbering_is_invisible():
 
 def test_launch_shapes_do_not_change_results():
     """Block sizes and the XCD-aware mapping change the schedule, not the sums."""
-    ref = None
-    for eb, nb, x in ((256, 256, 1), (64, 128, 0), (128, 64, 1)):
-        p = gh.cube_pair(6)
-        p.gpu.set_launch(eb, nb, x)
-        p.gpu.solve_step(50, akx.MIXED)
-        u = p.gpu.get_displacement()
-        if ref is None:
-            ref = u
-        assert _same(u, ref)
+    # the LDS-patch kernel (layout 2) always rebuilds B from the coordinates
+    for recompute, shapes in (
+            (0, ((256, 256, 1, 0), (64, 128, 0, 1), (128, 64, 1, 1))),
+            (1, ((256, 256, 1, 0), (64, 128, 0, 1), (128, 64, 1, 1), (256, 128, 1, 2),
+                 (256, 64, 0, 2)))):
+        ref = None
+        for eb, nb, x, lay in shapes:
+            p = gh.cube_pair(6)
+            p.gpu.set_launch(eb, nb, x, recompute, lay)
+            p.gpu.solve_step(50, akx.MIXED)
+            u = p.gpu.get_displacement()
+            if ref is None:
+                ref = u
+            assert _same(u, ref)
+    p = gh.cube_pair(6)
+    p.gpu.set_launch(64, 64, 0, 0, 1)
+    p.step(50, akx.FP64)
+    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+
+
+def test_autotune_restores_state_and_keeps_results():
+    """The autotuner runs trial steps but leaves the model state untouched; a
+    tuned FP64 run stays bit-identical to the oracle."""
+    p = gh.cube_pair(6)
+    p.step(5)
+    u_before = p.gpu.get_displacement()
+    shape, ms = p.gpu.autotune(akx.MIXED, 3)
+    assert ms > 0 and shape["element_block"] in (64, 128, 256)
+    assert _same(p.gpu.get_displacement(), u_before)
+    p.gpu.autotune(akx.FP64, 3)
+    p.step(20)
+    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
