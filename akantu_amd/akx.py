@@ -55,7 +55,7 @@ SIGNATURES = {
     "akx_synchronize": (_I, [_P]),
     "akx_enable_timing": (_I, [_P, _I]),
     "akx_last_step_timing": (_I, [_P, _PD, _PD]),
-    "akx_set_launch": (_I, [_P, _I, _I, _I, _I, _I]),
+    "akx_set_launch": (_I, [_P, _I, _I, _I, _I, _I, _I]),
     "akx_autotune": (_I, [_P, _I, _I, _P, _PD]),
     "akx_query": (_I, [_P, ctypes.c_char_p, _PD]),
 }
@@ -178,24 +178,25 @@ class SolidMechanicsModel:
         return a.value, b.value
 
     def set_launch(self, element_block=0, node_block=0, xcd_remap=-1, mixed_recompute=-1,
-                   slot_layout=-1):
+                   slot_layout=-1, node_stage=-1):
         self.ctx.check(lib.akx_set_launch(self.ctx.h, element_block, node_block, xcd_remap,
-                                          mixed_recompute, slot_layout))
+                                          mixed_recompute, slot_layout, node_stage))
 
     def query(self, key):
         v = ctypes.c_double()
         self.ctx.check(lib.akx_query(self.ctx.h, key.encode(), ctypes.byref(v)))
         return v.value
 
+    LAUNCH_KEYS = ("element_block", "node_block", "xcd_remap", "mixed_recompute",
+                   "slot_layout", "node_stage")
+
     def autotune(self, precision=MIXED, nsteps=10):
         """Pick the fastest launch shape on this mesh; returns (shape, ms/step)."""
-        chosen = np.zeros(5, dtype=np.int32)
+        chosen = np.zeros(6, dtype=np.int32)
         best = ctypes.c_double()
         self.ctx.check(lib.akx_autotune(self.ctx.h, int(precision), int(nsteps), _ptr(chosen),
                                         ctypes.byref(best)))
-        shape = dict(element_block=int(chosen[0]), node_block=int(chosen[1]),
-                     xcd_remap=int(chosen[2]), mixed_recompute=int(chosen[3]),
-                     slot_layout=int(chosen[4]))
+        shape = {k: int(v) for k, v in zip(self.LAUNCH_KEYS, chosen)}
         return shape, best.value
 
     # -- fields --

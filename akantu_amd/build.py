@@ -1,4 +1,5 @@
-"""Build libakx.so (HIP, gfx950) in-tree with hipcc.
+"""Build libakx.so (HIP, gfx950) in-tree with hipcc, then the C++ API over it
+(libakantu_amd.so, g++) and its example drivers (facade/bin/).
 
 Flags:
   -ffp-contract=off       the FP64 path keeps the reference's operation order
@@ -22,6 +23,9 @@ ARCH = os.environ.get("AKX_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["akx_kernels.hip", "akx_api.cpp"]
 HEADERS = ["akx_device.hpp", "akx_internal.hpp"]
+FACADE = os.path.join(HERE, "facade")
+FACADE_LIB = os.path.join(HERE, "libakantu_amd.so")
+EXAMPLES = ["explicit_dynamic", "patch_test_explicit"]
 
 
 def hipcc():
@@ -54,7 +58,37 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_facade(verbose)
     return LIB
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_facade(verbose: bool = False) -> str:
+    """libakantu_amd.so (the akantu:: C++ API, host code only) linked against
+    libakx.so, and the example drivers in facade/bin/ (run-paths point back at
+    the package directory, so the tree can be moved as a whole)."""
+    cxx = os.environ.get("CXX", "g++")
+    inc = ["-I", os.path.join(FACADE, "include"), "-I", os.path.join(ROOT, "include")]
+    flags = ["-O2", "-std=c++17", "-Wall", "-Wextra"]
+    tmp = FACADE_LIB + ".tmp"
+    _run([cxx] + flags + ["-fPIC", "-shared"] + inc +
+         [os.path.join(FACADE, "src", "akantu.cc"), "-o", tmp, "-L", HERE, "-lakx",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, FACADE_LIB)
+    bdir = os.path.join(FACADE, "bin")
+    os.makedirs(bdir, exist_ok=True)
+    for ex in EXAMPLES:
+        out = os.path.join(bdir, ex)
+        _run([cxx] + flags + inc + [os.path.join(FACADE, "examples", ex + ".cc"), "-o",
+                                    out + ".tmp", "-L", HERE, "-lakantu_amd", "-lakx",
+                                    "-Wl,-rpath,$ORIGIN/../.."], verbose)
+        os.replace(out + ".tmp", out)
+    return FACADE_LIB
 
 
 if __name__ == "__main__":

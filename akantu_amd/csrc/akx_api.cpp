@@ -314,6 +314,13 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.mode = mode;
   A.xcd_remap = c->xcd_remap;
   A.node_ordered = c->node_ordered;
+  A.stage_lds = 0;
+  if (c->node_ordered && c->node_stage) {
+    // static LDS of k_node is at most 4 * 256 * 3 * 8 + 256 * 3 bytes (< 26 KiB)
+    const size_t bytes = size_t(c->span_max[c->node_block / 64 - 1]) * c->d *
+                         (precision == AKX_FP64 ? 8 : 4);
+    if (bytes > 0 && bytes <= 38 * 1024) A.stage_lds = int(bytes);
+  }
   return A;
 }
 
@@ -629,6 +636,12 @@ int akx_init(akx_ctx* c) {
   build_csr(c, order, off, slot);
   bool ascending = true;
   for (int64_t e = 0; e < E && ascending; ++e) ascending = (order[e] == uint32_t(e));
+  for (int k = 0; k < 4; ++k) {
+    const int64_t nb = 64 * (k + 1);
+    uint32_t mx = 0;
+    for (int64_t n0 = 0; n0 < N; n0 += nb) mx = std::max(mx, off[std::min(N, n0 + nb)] - off[n0]);
+    c->span_max[k] = mx;
+  }
   HIP_TRY(c, dalloc(&c->csr_off, off.size()));
   HIP_TRY(c, dalloc(&c->csr_slot, slot.size()));
   HIP_TRY(c, hipMemcpy(c->csr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
@@ -1111,21 +1124,30 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   hipEvent_t t0, t1;
   HIP_TRY(c, hipEventCreate(&t0));
   HIP_TRY(c, hipEventCreate(&t1));
-  const int keep[5] = {c->elem_block, c->node_block, c->xcd_remap, c->mixed_recompute,
-                       c->node_ordered};
-  int best[5] = {keep[0], keep[1], keep[2], keep[3], keep[4]};
+  const int keep[6] = {c->elem_block, c->node_block, c->xcd_remap, c->mixed_recompute,
+                       c->node_ordered, c->node_stage};
+  int best[6] = {keep[0], keep[1], keep[2], keep[3], keep[4], keep[5]};
   float best_t = std::numeric_limits<float>::max();
   const bool was_timing = c->timing;
   c->timing = false;
+  // candidates: slot layout x node staging x table/recompute x XCD mapping x block sizes
+  struct Shape { int eb, nb, x, rcmp, lay, stage; };
+  std::vector<Shape> cand;
   const int blocks[3] = {64, 128, 256};
   const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 3 : 2;
-  for (int lay = 0; lay < nlay && rc == AKX_OK; ++lay)
-  for (int rcmp = 0; rcmp < (precision == AKX_MIXED ? 2 : 1) && rc == AKX_OK; ++rcmp)
-    for (int x = 0; x < 2 && rc == AKX_OK; ++x)
-      for (int eb : blocks)
-        for (int nb : blocks) {
+  for (int lay = 0; lay < nlay; ++lay)
+    for (int stage = 0; stage < (lay ? 2 : 1); ++stage)
+      for (int rcmp = (lay == 2 ? 1 : 0); rcmp < (precision == AKX_MIXED ? 2 : 1); ++rcmp)
+        for (int x = 0; x < 2; ++x)
+          for (int eb : blocks) {
+            if (lay == 2 && eb != 256) continue;  // the patch kernel has its own block shape
+            for (int nb : blocks) cand.push_back(Shape{eb, nb, x, rcmp, lay, stage});
+          }
+  for (const Shape& sh : cand) {
+          const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;
           c->elem_block = eb; c->node_block = nb; c->xcd_remap = x; c->node_ordered = lay;
-          c->mixed_recompute = precision == AKX_MIXED ? rcmp : keep[3];
+          c->node_stage = sh.stage;
+          c->mixed_recompute = precision == AKX_MIXED ? sh.rcmp : keep[3];
           if ((rc = restore())) break;
           if ((rc = akx_solve_step(c, 1, precision))) break;  // warm-up
           if (hipEventRecord(t0, c->stream) != hipSuccess) { rc = AKX_ERR_HIP; break; }
@@ -1137,12 +1159,12 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
           if (ms < best_t) {
             best_t = ms;
             best[0] = eb; best[1] = nb; best[2] = x; best[3] = c->mixed_recompute;
-            best[4] = lay;
+            best[4] = lay; best[5] = sh.stage;
           }
-        }
+  }
   c->timing = was_timing;
   c->elem_block = best[0]; c->node_block = best[1]; c->xcd_remap = best[2];
-  c->mixed_recompute = best[3]; c->node_ordered = best[4];
+  c->mixed_recompute = best[3]; c->node_ordered = best[4]; c->node_stage = best[5];
   int rc2 = restore();
   (void)hipStreamSynchronize(c->stream);
   for (auto& s : saved) dfree(s.second);
@@ -1150,23 +1172,24 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   (void)hipEventDestroy(t1);
   if (rc) return rc;
   if (rc2) return rc2;
-  if (chosen) for (int k = 0; k < 5; ++k) chosen[k] = best[k];
+  if (chosen) for (int k = 0; k < 6; ++k) chosen[k] = best[k];
   if (best_ms) *best_ms = double(best_t) / nsteps;
   return AKX_OK;
 }
 
 int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
-                   int mixed_recompute, int slot_layout) {
+                   int mixed_recompute, int slot_layout, int node_stage) {
   if (!c) return AKX_ERR_ARGUMENT;
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
     return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
+  if (slot_layout > 2) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1 or 2");
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;
   if (mixed_recompute >= 0) c->mixed_recompute = mixed_recompute ? 1 : 0;
-  if (slot_layout > 2) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1 or 2");
   if (slot_layout >= 0) c->node_ordered = slot_layout;
+  if (node_stage >= 0) c->node_stage = node_stage ? 1 : 0;
   return AKX_OK;
 }
 

@@ -77,6 +77,7 @@ struct NodeArgs {
   int mode;
   int xcd_remap;
   int node_ordered;       // slots stored in CSR order (contiguous per node)
+  int stage_lds;          // > 0: bytes of LDS for the block's slot range (node-ordered only)
 };
 
 enum NodeMode {
@@ -169,6 +170,8 @@ struct akx_ctx {
 
   int elem_block = 256, node_block = 256, xcd_remap = 1, mixed_recompute = 1;
   int node_ordered = 0;             // slot layout (see store_slots)
+  int node_stage = 1;               // node kernel copies its slot range to LDS when it fits
+  uint32_t span_max[4] = {0, 0, 0, 0};  // largest slot range of a node block of 64, 128, 192, 256
   uint32_t* slot_pos = nullptr;     // [a][E] CSR position of (e, a)
   // mixed path: CSR by internal element id (any fixed order will do there)
   uint32_t *csr2_off = nullptr, *csr2_slot = nullptr, *slot2_pos = nullptr;

@@ -504,13 +504,26 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
     s_b[t] = A.blocked[k0 + t];
     if constexpr (HAS_FEXT) s_f[t] = A.fext[k0 + t];
   }
+  // Node-ordered slots with A.stage_lds > 0: the block's slots are one
+  // contiguous range [off[n0], off[n0 + cnt]); copy it into LDS with coalesced
+  // loads, then every thread reads its own run from there.
+  extern __shared__ __align__(16) unsigned char dyn[];
+  TS* st = reinterpret_cast<TS*>(dyn);
+  const bool staged = A.node_ordered && A.stage_lds > 0 && (mode & (NODE_CORRECT | NODE_GATHER));
+  const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
+  uint32_t base = 0;
+  if (staged) {
+    const uint32_t q0 = A.csr_off[n0], q1 = A.csr_off[n0 + cnt];
+    const int64_t len = int64_t(q1 - q0) * D;
+    for (int64_t t = threadIdx.x; t < len; t += nb) st[t] = slots[int64_t(q0) * D + t];
+    base = q0;
+  }
   __syncthreads();
   const int ln = threadIdx.x;
   const int64_t node = n0 + ln;
   const double dt = A.dt;
   if (ln < cnt) {
     if (mode & (NODE_CORRECT | NODE_GATHER)) {
-      const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
       double f[D], c[D];
 #pragma unroll
       for (int i = 0; i < D; ++i) { f[i] = 0.; c[i] = 0.; }
@@ -529,7 +542,13 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
       // (in-order) accumulation, so the gathers overlap instead of serialising.
       constexpr int CHUNK = 8;
       uint32_t p = p0;
-      if (A.node_ordered) {
+      if (staged) {
+        // the node's run, read from the block's LDS copy (starts at csr_off[n0])
+        for (uint32_t q = p0 - base; q < p1 - base; ++q)
+#pragma unroll
+          for (int i = 0; i < D; ++i) add(i, st[q * D + i]);
+        p = p1;
+      } else if (A.node_ordered) {
         // the node's contributions are contiguous: slots[p*D ...] for p in [p0, p1)
         for (; p + CHUNK <= p1; p += CHUNK) {
           TS x[CHUNK * D];
@@ -979,11 +998,11 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
                                 int block) {
   const unsigned g = grid_for(A.N, block);
   if (precision == 0) {
-    if (has_fext) k_node<D, double, true><<<g, block, 0, s>>>(A);
-    else k_node<D, double, false><<<g, block, 0, s>>>(A);
+    if (has_fext) k_node<D, double, true><<<g, block, A.stage_lds, s>>>(A);
+    else k_node<D, double, false><<<g, block, A.stage_lds, s>>>(A);
   } else {
-    if (has_fext) k_node<D, float, true><<<g, block, 0, s>>>(A);
-    else k_node<D, float, false><<<g, block, 0, s>>>(A);
+    if (has_fext) k_node<D, float, true><<<g, block, A.stage_lds, s>>>(A);
+    else k_node<D, float, false><<<g, block, A.stage_lds, s>>>(A);
   }
   return hipGetLastError();
 }

@@ -15,10 +15,16 @@ namespace akantu {
 
 template <typename T>
 class Array {
+  // One element per cell; a plain vector<bool> would not hand out bool&.
+  struct Cell {
+    T v;
+  };
+  static_assert(sizeof(Cell) == sizeof(T), "cell adds padding");
+
  public:
   Array() = default;
   explicit Array(UInt size, UInt nb_component = 1, const T& value = T(), const ID& id = "")
-      : size_(size), nb_component_(nb_component), values_(size_t(size) * nb_component, value),
+      : size_(size), nb_component_(nb_component), values_(size_t(size) * nb_component, Cell{value}),
         id_(id) {}
   Array(UInt size, UInt nb_component, const ID& id)
       : size_(size), nb_component_(nb_component), values_(size_t(size) * nb_component), id_(id) {}
@@ -26,13 +32,13 @@ class Array {
   UInt size() const { return size_; }
   UInt getNbComponent() const { return nb_component_; }
   const ID& getID() const { return id_; }
-  T* storage() { return values_.data(); }
-  const T* storage() const { return values_.data(); }
+  T* storage() { return reinterpret_cast<T*>(values_.data()); }
+  const T* storage() const { return reinterpret_cast<const T*>(values_.data()); }
 
-  T& operator()(UInt i, UInt j = 0) { return values_[size_t(i) * nb_component_ + j]; }
-  const T& operator()(UInt i, UInt j = 0) const { return values_[size_t(i) * nb_component_ + j]; }
-  T& operator[](size_t k) { return values_[k]; }
-  const T& operator[](size_t k) const { return values_[k]; }
+  T& operator()(UInt i, UInt j = 0) { return values_[size_t(i) * nb_component_ + j].v; }
+  const T& operator()(UInt i, UInt j = 0) const { return values_[size_t(i) * nb_component_ + j].v; }
+  T& operator[](size_t k) { return values_[k].v; }
+  const T& operator[](size_t k) const { return values_[k].v; }
 
   void resize(UInt size) {
     size_ = size;
@@ -43,7 +49,7 @@ class Array {
     nb_component_ = nb_component;
     values_.resize(size_t(size) * nb_component_);
   }
-  void set(const T& v) { std::fill(values_.begin(), values_.end(), v); }
+  void set(const T& v) { std::fill(values_.begin(), values_.end(), Cell{v}); }
   void clear() { set(T()); }
   void copy(const Array& other) {
     size_ = other.size_;
@@ -51,15 +57,15 @@ class Array {
     values_ = other.values_;
   }
 
-  typename std::vector<T>::iterator begin() { return values_.begin(); }
-  typename std::vector<T>::iterator end() { return values_.end(); }
-  typename std::vector<T>::const_iterator begin() const { return values_.begin(); }
-  typename std::vector<T>::const_iterator end() const { return values_.end(); }
+  T* begin() { return storage(); }
+  T* end() { return storage() + values_.size(); }
+  const T* begin() const { return storage(); }
+  const T* end() const { return storage() + values_.size(); }
 
  private:
   UInt size_ = 0;
   UInt nb_component_ = 1;
-  std::vector<T> values_;
+  std::vector<Cell> values_;
   ID id_;
 };
 

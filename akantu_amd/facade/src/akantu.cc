@@ -1,9 +1,11 @@
 // akantu.cc -- the akantu:: façade over the C ABI of libakx.so.
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <map>
 #include <random>
 #include <set>
@@ -121,7 +123,7 @@ void Mesh::setNodes(const Array<Real>& nodes) {
 }
 
 void Mesh::setConnectivity(ElementType type, const Array<UInt>& conn) {
-  if (getSpatialDimension(type) != dim_) AKANTU_EXCEPTION("element type of another dimension");
+  if (akantu::getSpatialDimension(type) != dim_) AKANTU_EXCEPTION("element type of another dimension");
   if (conn.getNbComponent() != getNbNodesPerElement(type)) AKANTU_EXCEPTION("bad connectivity width");
   type_ = type;
   conn_.copy(conn);
@@ -235,7 +237,7 @@ void Mesh::read(const std::string& filename) {
   for (size_t k = 0; k < ids.size(); ++k) id_map[ids[k]] = UInt(k);
   int found = -1;
   for (auto& el : els)
-    if (code.count(el.type) && getSpatialDimension(code[el.type]) == dim_) {
+    if (code.count(el.type) && akantu::getSpatialDimension(code[el.type]) == dim_) {
       if (found >= 0 && found != el.type) AKANTU_EXCEPTION("mixed element types are not supported");
       found = el.type;
     }

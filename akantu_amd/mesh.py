@@ -131,6 +131,32 @@ _FACETS = {
 }
 
 
+def write_msh(mesh: Mesh, path: str, groups: dict | None = None) -> None:
+    """Write a gmsh 2.2 ASCII file that read_msh (and Mesh::read of the C++
+    API) reads back: 1-based node ids, coordinates padded to 3D, one
+    element block of the mesh's type with physical tag 0, and, per entry
+    ``tag -> node ids`` of ``groups``, one gmsh point element (type 15) per node
+    with that physical tag, which the readers turn into node group
+    "phys_<tag>".
+    """
+    code = [k for k, v in _MSH_TYPES.items() if v == mesh.elem_type][0]
+    N, d = mesh.nodes.shape
+    xyz = np.zeros((N, 3))
+    xyz[:, :d] = mesh.nodes
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(N)]
+    lines += ["%d %.17g %.17g %.17g" % (k + 1, x, y, z) for k, (x, y, z) in enumerate(xyz)]
+    lines += ["$EndNodes", "$Elements"]
+    body = []
+    for e, row in enumerate(mesh.conn):
+        body.append("%d %d 2 0 0 %s" % (len(body) + 1, code, " ".join(str(int(v) + 1) for v in row)))
+    for tag, nodes in (groups or {}).items():
+        for v in nodes:
+            body.append("%d 15 2 %d %d %d" % (len(body) + 1, int(tag), int(tag), int(v) + 1))
+    lines += [str(len(body))] + body + ["$EndElements"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def boundary_facets(mesh: Mesh, with_owner: bool = False):
     """Facets (node ids) that belong to one element only; optionally also the
     owning element of each facet."""

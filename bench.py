@@ -157,7 +157,7 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 1 GiB device copy (calibration of the traffic counters)")
     ap.add_argument("--launch-mixed", default=None,
-                    help="element_block,node_block,xcd_remap,mixed_recompute,slot_layout "
+                    help="element_block,node_block,xcd_remap,mixed_recompute,slot_layout,node_stage "
                          "(skips the autotuner for the mixed path)")
     ap.add_argument("--launch-fp64", default=None, help="same, for the FP64 path")
     args = ap.parse_args()
@@ -186,7 +186,7 @@ def main():
 
     res = {}
     shapes = {}
-    keys = ("element_block", "node_block", "xcd_remap", "mixed_recompute", "slot_layout")
+    keys = akx.SolidMechanicsModel.LAUNCH_KEYS
     for name, prec, fixed in (("mixed", akx.MIXED, args.launch_mixed),
                               ("fp64", akx.FP64, args.launch_fp64)):
         if fixed:

@@ -184,11 +184,13 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
 /* Launch shape of the element and node kernels: threads per block (64..256,
  * 0 keeps the current value), XCD-aware block mapping (1/0, -1 keeps), for
  * the mixed path whether B and det(J) w are rebuilt from the coordinates (1)
- * or read from FP32 tables (0), and the layout of the element force slots:
- * element-ordered (0) or node-ordered (1) (-1 keeps).  None of these changes
- * a result bit. */
+ * or read from FP32 tables (0), the layout of the element force slots:
+ * element-ordered (0), node-ordered (1) or node-ordered written by LDS
+ * element patches (2, mixed path), and whether the node kernel copies its
+ * block's contiguous slot range to LDS first (node_stage 1/0) (-1 keeps).
+ * None of these changes a result bit. */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
-                   int mixed_recompute, int slot_layout);
+                   int mixed_recompute, int slot_layout, int node_stage);
 
 /* Scalar facts about the context, for diagnostics and byte models:
  *   "nb_nodes", "nb_elements", "csr_entries", "patch_blocks",
@@ -201,8 +203,9 @@ int akx_query(akx_ctx* ctx, const char* key, double* value);
  * mapping, mixed-path table/recompute variant, slot layout) over `nsteps`
  * steps of the given precision on the current mesh and keeps the fastest.
  * The model state (u, v, a, damage, ...) is restored afterwards.  chosen
- * (optional, 5 ints) receives element_block, node_block, xcd_remap,
- * mixed_recompute, slot_layout; best_ms (optional) the fastest time per step. */
+ * (optional, 6 ints) receives element_block, node_block, xcd_remap,
+ * mixed_recompute, slot_layout, node_stage; best_ms (optional) the fastest
+ * time per step. */
 int akx_autotune(akx_ctx* ctx, int precision, int nsteps, int* chosen, double* best_ms);
 
 #ifdef __cplusplus

@@ -251,23 +251,27 @@ def test_internal_renumbering_is_invisible():
 def test_launch_shapes_do_not_change_results():
     """Block sizes and the XCD-aware mapping change the schedule, not the sums."""
     # the LDS-patch kernel (layout 2) always rebuilds B from the coordinates
+    # node_stage (last entry) copies the node block's slot range to LDS first
     for recompute, shapes in (
-            (0, ((256, 256, 1, 0), (64, 128, 0, 1), (128, 64, 1, 1))),
-            (1, ((256, 256, 1, 0), (64, 128, 0, 1), (128, 64, 1, 1), (256, 128, 1, 2),
-                 (256, 64, 0, 2)))):
+            (0, ((256, 256, 1, 0, 0), (64, 128, 0, 1, 0), (128, 64, 1, 1, 1),
+                 (64, 256, 1, 1, 1))),
+            (1, ((256, 256, 1, 0, 0), (64, 128, 0, 1, 1), (128, 64, 1, 1, 0),
+                 (256, 128, 1, 2, 1), (256, 64, 0, 2, 0), (256, 256, 1, 2, 1)))):
         ref = None
-        for eb, nb, x, lay in shapes:
+        for eb, nb, x, lay, stage in shapes:
             p = gh.cube_pair(6)
-            p.gpu.set_launch(eb, nb, x, recompute, lay)
+            p.gpu.set_launch(eb, nb, x, recompute, lay, stage)
             p.gpu.solve_step(50, akx.MIXED)
             u = p.gpu.get_displacement()
             if ref is None:
                 ref = u
             assert _same(u, ref)
     p = gh.cube_pair(6)
-    p.gpu.set_launch(64, 64, 0, 0, 1)
-    p.step(50, akx.FP64)
-    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    for stage in (0, 1):
+        p = gh.cube_pair(6)
+        p.gpu.set_launch(64, 64, 0, 0, 1, stage)
+        p.step(50, akx.FP64)
+        assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
 
 
 def test_autotune_restores_state_and_keeps_results():
