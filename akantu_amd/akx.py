@@ -48,6 +48,9 @@ SIGNATURES = {
     "akx_node_numbering": (_I, [_P, _P]),
     "akx_set_time_step": (_I, [_P, _D]),
     "akx_solve_step": (_I, [_P, _I, _I]),
+    "akx_solve_step_graph": (_I, [_P, _I, _I]),
+    "akx_set_node_time_steps": (_I, [_P, _P]),
+    "akx_stable_time_step_per_material": (_I, [_P, _P]),
     "akx_assemble_internal_forces": (_I, [_P, _I]),
     "akx_stable_time_step": (_I, [_P, _PD]),
     "akx_energy": (_I, [_P, _I, _PD]),
@@ -152,6 +155,25 @@ class SolidMechanicsModel:
 
     def solve_step(self, nsteps=1, precision=FP64):
         self.ctx.check(lib.akx_solve_step(self.ctx.h, int(nsteps), int(precision)))
+
+    def solve_step_graph(self, nsteps=1, precision=FP64):
+        """solve_step as one captured HIP graph launch (same results)."""
+        self.ctx.check(lib.akx_solve_step_graph(self.ctx.h, int(nsteps), int(precision)))
+
+    def set_node_time_steps(self, dt_per_node):
+        """One time step per node (independent bodies in one model); None resets."""
+        if dt_per_node is None:
+            self.ctx.check(lib.akx_set_node_time_steps(self.ctx.h, None))
+            return
+        dt = np.ascontiguousarray(dt_per_node, dtype=np.float64)
+        if dt.shape != (self.mesh.nb_nodes,):
+            raise ValueError("one time step per node")
+        self.ctx.check(lib.akx_set_node_time_steps(self.ctx.h, _ptr(dt)))
+
+    def get_stable_time_step_per_material(self):
+        out = np.zeros(len(self.materials), dtype=np.float64)
+        self.ctx.check(lib.akx_stable_time_step_per_material(self.ctx.h, _ptr(out)))
+        return out
 
     def assemble_internal_forces(self, precision=FP64):
         self.ctx.check(lib.akx_assemble_internal_forces(self.ctx.h, int(precision)))

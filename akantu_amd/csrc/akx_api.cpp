@@ -78,7 +78,18 @@ void free_derived(akx_ctx* c) {
   c->initialized = false;
 }
 
+void free_graph(akx_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key = akx_ctx::GraphKey();
+}
+
 void free_mesh_buffers(akx_ctx* c) {
+  free_graph(c);
+  dfree(c->dt_node);
+  dfree(c->mat_dt);
   free_derived(c);
   dfree(c->X); dfree(c->conn_soa);
   dfree(c->u); dfree(c->v); dfree(c->a); dfree(c->fext); dfree(c->fint); dfree(c->mass);
@@ -310,6 +321,7 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.prev = c->prev;
   A.inc = c->inc;
   A.dt = c->dt;
+  A.dt_node = c->dt_node;
   for (int i = 0; i < 3; ++i) A.bc_inc[i] = c->bc_inc[i];
   A.mode = mode;
   A.xcd_remap = c->xcd_remap;
@@ -924,16 +936,22 @@ int akx_set_time_step(akx_ctx* c, double dt) {
   return AKX_OK;
 }
 
-int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
+namespace {
+
+int check_step_args(akx_ctx* c, int precision) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (precision != AKX_FP64 && precision != AKX_MIXED)
     return fail(c, AKX_ERR_ARGUMENT, "unknown precision");
-  if (!(c->dt > 0)) return fail(c, AKX_ERR_STATE, "time step not set");
-  if (nsteps <= 0) return AKX_OK;
-  (void)hipSetDevice(c->device);
+  if (!(c->dt > 0) && !c->dt_node) return fail(c, AKX_ERR_STATE, "time step not set");
+  return AKX_OK;
+}
+
+// The launches of nsteps steps on c->stream (recorded into a graph when the
+// stream is being captured; timing events only when `timing`).
+int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
   const int d = c->d;
-  const bool timing = c->timing;
+  int rc = AKX_OK;
   if (timing) {
     const size_t need = 2 * size_t(nsteps) + 1;
     while (c->pool.size() < need) {
@@ -974,7 +992,87 @@ int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
     c->last_elem_ms = el / nsteps;
     c->last_node_ms = nd / nsteps;
   }
+  return AKX_OK;
+}
+
+}  // namespace
+
+int akx_solve_step(akx_ctx* c, int nsteps, int precision) {
+  int rc = check_step_args(c, precision);
+  if (rc) return rc;
+  if (nsteps <= 0) return AKX_OK;
+  (void)hipSetDevice(c->device);
+  rc = enqueue_steps(c, nsteps, precision, c->timing);
+  if (rc) return rc;
   c->fint_valid = true;
+  return AKX_OK;
+}
+
+int akx_solve_step_graph(akx_ctx* c, int nsteps, int precision) {
+  int rc = check_step_args(c, precision);
+  if (rc) return rc;
+  if (nsteps <= 0) return AKX_OK;
+  (void)hipSetDevice(c->device);
+  akx_ctx::GraphKey k;
+  k.nsteps = nsteps; k.precision = precision;
+  k.eb = c->elem_block; k.nb = c->node_block; k.x = c->xcd_remap; k.rc = c->mixed_recompute;
+  k.lay = c->node_ordered; k.stage = c->node_stage;
+  k.fext = c->has_fext; k.bc = c->has_bc_inc; k.dt = c->dt;
+  for (int i = 0; i < 3; ++i) k.bc_inc[i] = c->bc_inc[i];
+  k.dt_node = c->dt_node;
+  if (!c->graph_exec || !(k == c->graph_key)) {
+    free_graph(c);
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_steps(c, nsteps, precision, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(c, AKX_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(e));
+    }
+    c->graph = g;
+    HIP_TRY(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->graph_key = k;
+  }
+  HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
+  c->fint_valid = true;
+  return AKX_OK;
+}
+
+int akx_set_node_time_steps(akx_ctx* c, const double* dt) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (!dt) {
+    dfree(c->dt_node);
+    return AKX_OK;
+  }
+  std::vector<double> h(size_t(c->N));
+  for (int64_t k = 0; k < c->N; ++k) {
+    const double v = dt[c->node_perm[k]];
+    if (!(v > 0) || !std::isfinite(v)) return fail(c, AKX_ERR_ARGUMENT, "time steps must be > 0");
+    h[k] = v;
+  }
+  if (!c->dt_node) HIP_TRY(c, dalloc(&c->dt_node, size_t(c->N)));
+  HIP_TRY(c, hipMemcpy(c->dt_node, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  return AKX_OK;
+}
+
+int akx_stable_time_step_per_material(akx_ctx* c, double* dt) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!dt) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  const int nm = int(c->mats.size());
+  if (!c->mat_dt) HIP_TRY(c, dalloc(&c->mat_dt, size_t(nm)));
+  HIP_TRY(c, launch_stable_dt_material(c->et, c->stream, c->E, c->X, c->u, c->conn_soa,
+                                       c->elem_mat_d, c->mats_d, nm, c->mat_dt));
+  std::vector<unsigned long long> bits(nm);
+  HIP_TRY(c, hipMemcpyAsync(bits.data(), c->mat_dt, size_t(nm) * 8, hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int m = 0; m < nm; ++m) std::memcpy(dt + m, &bits[m], 8);
   return AKX_OK;
 }
 

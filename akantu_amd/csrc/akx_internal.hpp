@@ -73,6 +73,7 @@ struct NodeArgs {
   double* prev;
   double* inc;
   double dt;
+  const double* dt_node;  // nullptr = dt for every node
   double bc_inc[3];
   int mode;
   int xcd_remap;
@@ -104,6 +105,10 @@ hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, c
                             const uint32_t* conn_soa, const uint16_t* elem_mat,
                             const MatParams* mats, double* partial, int nblocks_max,
                             double* out);
+hipError_t launch_stable_dt_material(int et, hipStream_t s, int64_t E, const double* X,
+                                     const double* u, const uint32_t* conn,
+                                     const uint16_t* elem_mat, const MatParams* mats, int nmat,
+                                     unsigned long long* out);
 hipError_t launch_energy(int kind_energy, int et, int mat_kind_uniform, hipStream_t s,
                          int64_t N, int64_t E, const double* u, const double* v,
                          const double* mass, const double* fext, const double* fint,
@@ -186,6 +191,26 @@ struct akx_ctx {
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal
   bool reorder = true;
   std::vector<uint32_t> node_perm, node_inv, elem_perm, elem_inv;
+
+  // one time step per node (internal numbering), nullptr = dt everywhere
+  double* dt_node = nullptr;
+  unsigned long long* mat_dt = nullptr;  // per-material stable time step (bit patterns)
+  // captured step sequence (akx_solve_step_graph), re-captured when any
+  // launch argument changes
+  struct GraphKey {
+    int nsteps = 0, precision = -1, eb = 0, nb = 0, x = 0, rc = 0, lay = 0, stage = 0;
+    bool fext = false, bc = false;
+    double dt = 0, bc_inc[3] = {0, 0, 0};
+    const double* dt_node = nullptr;
+    bool operator==(const GraphKey& o) const {
+      return nsteps == o.nsteps && precision == o.precision && eb == o.eb && nb == o.nb &&
+             x == o.x && rc == o.rc && lay == o.lay && stage == o.stage && fext == o.fext &&
+             bc == o.bc && dt == o.dt && bc_inc[0] == o.bc_inc[0] && bc_inc[1] == o.bc_inc[1] &&
+             bc_inc[2] == o.bc_inc[2] && dt_node == o.dt_node;
+    }
+  } graph_key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 
   std::vector<hipEvent_t> pool;  // per-step timing events
   bool timing = false;

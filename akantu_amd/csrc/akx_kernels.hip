@@ -521,8 +521,9 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   __syncthreads();
   const int ln = threadIdx.x;
   const int64_t node = n0 + ln;
-  const double dt = A.dt;
   if (ln < cnt) {
+    // one time step per node when independent bodies advance together
+    const double dt = A.dt_node ? A.dt_node[node] : A.dt;
     if (mode & (NODE_CORRECT | NODE_GATHER)) {
       double f[D], c[D];
 #pragma unroll
@@ -748,6 +749,59 @@ __global__ void __launch_bounds__(256) k_stable_dt(int64_t E, const double* __re
   }
   best = block_reduce(best, OpMin(), 1.7976931348623157e308);
   if (threadIdx.x == 0) partial[blockIdx.x] = best;
+}
+
+// Stable time step of each material's elements (one independent body per
+// material when bodies advance together): out[m] = min over the elements of
+// material m of h_e / c_p(m), as the IEEE bit pattern of the double.  For
+// positive doubles the bit patterns order like the values, so an unsigned
+// 64-bit atomic minimum gives an exact, order-independent result.
+template <int ET>
+__global__ void __launch_bounds__(256) k_stable_dt_material(
+    int64_t E, const double* __restrict__ X, const double* __restrict__ u,
+    const uint32_t* __restrict__ conn, const uint16_t* __restrict__ elem_mat,
+    const MatParams* __restrict__ mats, unsigned long long* out) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n;
+  for (int64_t e0 = int64_t(blockIdx.x) * blockDim.x; e0 < E; e0 += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = e0 + threadIdx.x;
+    const bool live = e < E;
+    int mat = -1;
+    double dte = 1.7976931348623157e308;
+    if (live) {
+      double x[d * n];
+#pragma unroll
+      for (int a = 0; a < n; ++a) {
+        const int64_t node = conn[a * E + e];
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          double p = X[node * d + i];
+          p += u[node * d + i];
+          x[i + a * d] = p;
+        }
+      }
+      mat = elem_mat ? elem_mat[e] : 0;
+      const MatParams m = mats[mat];
+      dte = element_h<ET>(x) / sqrt((m.lambda + 2 * m.mu) / m.rho);
+    }
+    // one atomic per wavefront when its live lanes share a material
+    const int m0 = __shfl(mat, 0, 64);
+    const bool uniform = __all(!live || mat == m0);
+    if (uniform) {
+      double v = dte;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+      if ((threadIdx.x & 63) == 0 && m0 >= 0)
+        atomicMin(out + m0, static_cast<unsigned long long>(__double_as_longlong(v)));
+    } else if (live) {
+      atomicMin(out + mat, static_cast<unsigned long long>(__double_as_longlong(dte)));
+    }
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 template <int IS_MIN>
@@ -1025,6 +1079,20 @@ hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, c
   AKX_ET_SWITCH(et, F)
 #undef F
   k_finish<1><<<1, 256, 0, s>>>(partial, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_stable_dt_material(int et, hipStream_t s, int64_t E, const double* X,
+                                     const double* u, const uint32_t* conn,
+                                     const uint16_t* elem_mat, const MatParams* mats, int nmat,
+                                     unsigned long long* out) {
+  clear_error();
+  k_fill_u64<<<grid_for(nmat, 256), 256, 0, s>>>(out, nmat, 0x7FEFFFFFFFFFFFFFull);
+  if (E == 0) return hipGetLastError();
+  const unsigned nb = unsigned(std::min<int64_t>(4096, grid_for(E, 256)));
+#define F(ET) k_stable_dt_material<ET><<<nb, 256, 0, s>>>(E, X, u, conn, elem_mat, mats, out)
+  AKX_ET_SWITCH(et, F)
+#undef F
   return hipGetLastError();
 }
 

@@ -360,6 +360,55 @@ def ensemble_tri3(count=512, min_elems=2000, max_elems=40000, seed=2024):
     return out
 
 
+@dataclasses.dataclass
+class Ensemble:
+    """Independent specimens meshed into one element-disjoint mesh."""
+    mesh: Mesh
+    node_offsets: np.ndarray  # (S+1,) first node of each specimen
+    elem_offsets: np.ndarray  # (S+1,) first element of each specimen
+    element_material: np.ndarray  # (E,) specimen index of every element
+    velocity: np.ndarray  # (N, d)
+    blocked: np.ndarray  # (N, d) bool
+    specimens: list
+
+    def node_specimen(self) -> np.ndarray:
+        return np.repeat(np.arange(len(self.specimens)), np.diff(self.node_offsets))
+
+    def specimen_mesh(self, k: int) -> Mesh:
+        """Specimen k as a mesh of its own, with the coordinates it has in the
+        ensemble (so that a model of it alone sees the same numbers)."""
+        n0, n1 = self.node_offsets[k], self.node_offsets[k + 1]
+        e0, e1 = self.elem_offsets[k], self.elem_offsets[k + 1]
+        return Mesh(self.mesh.elem_type, self.mesh.nodes[n0:n1],
+                    self.mesh.conn[e0:e1].astype(np.int64) - n0)
+
+
+def concatenate_specimens(specimens, gap: float = 0.1) -> Ensemble:
+    """Lay the specimens out on a square grid of tiles (no two overlap) and
+    concatenate nodes and elements in specimen order: specimen k owns nodes
+    node_offsets[k]:node_offsets[k+1] and elements elem_offsets[k]:elem_offsets[k+1]."""
+    S = len(specimens)
+    cols = max(1, int(math.ceil(math.sqrt(S))))
+    ext = np.array([sp.mesh.nodes.max(axis=0) - sp.mesh.nodes.min(axis=0) for sp in specimens])
+    tile = ext.max(axis=0) + gap
+    nodes, conn, vel, blk, em = [], [], [], [], []
+    noff, eoff = [0], [0]
+    for k, sp in enumerate(specimens):
+        shift = np.zeros(sp.mesh.dim)
+        shift[0] = (k % cols) * tile[0]
+        shift[1] = (k // cols) * tile[1]
+        nodes.append(sp.mesh.nodes + shift)
+        conn.append(sp.mesh.conn.astype(np.int64) + noff[-1])
+        vel.append(sp.velocity)
+        blk.append(sp.blocked)
+        em.append(np.full(sp.mesh.nb_elements, k, dtype=np.int32))
+        noff.append(noff[-1] + sp.mesh.nb_nodes)
+        eoff.append(eoff[-1] + sp.mesh.nb_elements)
+    m = Mesh(specimens[0].mesh.elem_type, np.concatenate(nodes), np.concatenate(conn))
+    return Ensemble(m, np.array(noff), np.array(eoff), np.concatenate(em),
+                    np.concatenate(vel), np.concatenate(blk), list(specimens))
+
+
 # ----------------------------------------------------------------------------
 # locality renumbering
 # ----------------------------------------------------------------------------

@@ -1,12 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit central-difference step on one MI355X.
 
-Workload (default): 3D linear elastic unit cube, 100^3 hexes Kuhn-split into
-6,000,000 tetrahedron_4 (1,030,301 nodes), interior nodes jittered by +-1 % of h,
-E = 210 GPa, nu = 0.3, rho = 7800 kg/m^3, bottom face blocked, top face v_z = -1 m/s,
-dt = 0.8 x stable.  A "step" is one solveStep(): predictor, internal forces
-(gradient -> stress -> B^T sigma -> integrate -> assembly), residual, lumped
-solve, corrector.
+Default workload (cube-tet4): 3D linear elastic unit cube, 100^3 hexes
+Kuhn-split into 6,000,000 tetrahedron_4 (1,030,301 nodes), interior nodes
+jittered by +-1 % of h, E = 210 GPa, nu = 0.3, rho = 7800 kg/m^3, bottom face
+blocked, top face v_z = -1 m/s, dt = 0.8 x stable.  A "step" is one
+solveStep(): predictor, internal forces (gradient -> stress -> B^T sigma ->
+integrate -> assembly), residual, lumped solve, corrector.
 
 value = element-QP updates per second of the mixed path (FP32 element kernels,
 FP64 nodal state), inputs resident in HBM, timed over --steps steps between
@@ -14,6 +14,16 @@ two stream synchronisations.  The FP64 path is timed the same way and reported
 beside it, with the relative nodal-displacement error of the mixed path after
 1000 steps.  The CPU baseline is the oracle (C++ FP64 restatement of the
 reference loops) timed on this host over a bounded number of steps.
+
+Other workloads (--workload), reported the same way on their own JSON line:
+  hex8-neohookean  128^3 hexahedron_8 (2x2x2 Gauss), finite-strain neo-Hookean
+                   E = 10 MPa, nu = 0.45, rho = 1000, bottom blocked, top face
+                   compressed at 0.5 m/s (per-step Dirichlet increment), 500 steps
+                   for the error run;
+  ensemble         512 independent plane-strain triangle_3 specimens (2,000 to
+                   40,000 elements each), each with its own material and stable
+                   time step, advanced together by one captured HIP graph per
+                   run; 2000 steps for the per-specimen error run.
 """
 from __future__ import annotations
 
@@ -37,7 +47,7 @@ def log(msg):
 
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
-                      patch_nodes=None, patch_blocks=0):
+                      patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -45,7 +55,8 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     are read by the element kernel (scatter) instead of the node kernel;
     patch_nodes: LDS-patch element kernel, which reads 16-bit local
     connectivity + 8-bit run ranks per (e, a) and, per patch node, its id,
-    run start and run offset (4 + 4 + 2 bytes)."""
+    run start and run offset (4 + 4 + 2 bytes); dt_per_node: one time step
+    per node (ensembles); state_bytes: per-point history read + written."""
     if patch_nodes is not None:
         elem = (E * n * (2 + 1)                            # local connectivity, run ranks
                 + patch_nodes * (4 + 4 + 2) + patch_blocks * (4 + 2)  # patch node lists
@@ -59,12 +70,14 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
                 + N * d * 8                                  # displacement gather
                 + (E * n * 4 if node_ordered else 0)         # slot positions
                 + E * n * d * pe)                            # element force slots (write)
+    elem += state_bytes
     node = ((N + 1) * 4 + (0 if node_ordered else E * n * 4)  # node -> slot CSR
             + E * n * d * pe         # force slots (read)
             + 6 * N * d * 8          # u, v, a read + write
             + N * 8                  # lumped mass (one value per node)
             + N * d                  # blocked flags
-            + (N * d * 8 if has_fext else 0))
+            + (N * d * 8 if has_fext else 0)
+            + (N * 8 if dt_per_node else 0))
     return elem, node
 
 
@@ -86,18 +99,27 @@ def reset(model, v0):
     model.set("acceleration", z)
 
 
-def timed(model, precision, steps, warmup, v0):
+def run_steps(model, precision, steps, graph):
+    if graph:
+        model.solve_step_graph(steps, precision)
+    else:
+        model.solve_step(steps, precision)
+
+
+def timed(model, precision, steps, warmup, v0, graph=False):
     reset(model, v0)
     if warmup:
-        model.solve_step(warmup, precision)
+        run_steps(model, precision, warmup, graph)
     model.synchronize()
     t0 = time.perf_counter()
-    model.solve_step(steps, precision)
+    run_steps(model, precision, steps, graph)
     model.synchronize()
     return time.perf_counter() - t0
 
 
 def kernel_times(model, precision, steps, v0):
+    """Average element / node kernel durations from HIP events recorded on the
+    model's stream around every launch."""
     reset(model, v0)
     model.enable_timing(True)
     model.solve_step(steps, precision)
@@ -132,9 +154,10 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     return 2 * nbytes / t / 1e9
 
 
-def cpu_baseline(m, blocked, v0, dt, warm=2, steps=10, threads=1):
+def cpu_baseline(m, materials, blocked, v0, dt, warm=2, steps=10, threads=1):
+    """The oracle on this host: `steps` timed steps after `warm` untimed ones."""
     from oracle import oracle as orc
-    o = orc.OracleModel(m, [dict(E=210e9, nu=0.3, rho=7800.0)], threads=threads)
+    o = orc.OracleModel(m, materials, threads=threads)
     o.blocked[:] = blocked
     o.nodal("v")[:] = v0
     o.set_time_step(dt)
@@ -145,12 +168,322 @@ def cpu_baseline(m, blocked, v0, dt, warm=2, steps=10, threads=1):
     return o, t / steps
 
 
+def tune(model, args, akx):
+    shapes = {}
+    keys = akx.SolidMechanicsModel.LAUNCH_KEYS
+    for name, prec, fixed in (("mixed", akx.MIXED, args.launch_mixed),
+                              ("fp64", akx.FP64, args.launch_fp64)):
+        if fixed:
+            shapes[name] = dict(zip(keys, (int(x) for x in fixed.split(","))))
+            log("%s: fixed launch %s" % (name, shapes[name]))
+        else:
+            shapes[name], tuned_ms = model.autotune(prec, 10)
+            log("%s: autotuned launch %s (%.4f ms/step)" % (name, shapes[name], tuned_ms))
+    return shapes
+
+
+def measure(model, shapes, args, akx, v0, QP, graph=False):
+    res = {}
+    for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
+        model.set_launch(**shapes[name])
+        t = timed(model, prec, args.steps, args.warmup, v0, graph)
+        el_ms, nd_ms = kernel_times(model, prec, min(args.steps, 200), v0)
+        res[name] = dict(seconds=t, ms_per_step=1e3 * t / args.steps,
+                         qp_per_s=QP * args.steps / t, element_ms=el_ms, node_ms=nd_ms,
+                         launch=shapes[name])
+        log("%s: %.4f ms/step  (element %.4f ms, node %.4f ms)" %
+            (name, res[name]["ms_per_step"], el_ms, nd_ms))
+    return res
+
+
+def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
+    """Roofline of the slower of the two mixed-path kernels."""
+    patch = shapes["mixed"]["slot_layout"] == 2 and model.query("patch_available") == 1
+    eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
+                               tables=not (shapes["mixed"]["mixed_recompute"] or patch),
+                               node_ordered=bool(shapes["mixed"]["slot_layout"]),
+                               patch_nodes=model.query("patch_nodes_total") if patch else None,
+                               patch_blocks=model.query("patch_blocks") if patch else 0, **kw)
+    el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
+    if el_ms >= nd_ms:
+        kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms
+    else:
+        kname, kbytes, kms = node_kernel, nb, nd_ms
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    step_gbs = (eb + nb) / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
+    return kname, kbytes, kms, achieved, eb, nb, step_gbs
+
+
+def base_record(args, res, dtype, config):
+    return {
+        "metric": METRIC,
+        "value": res["mixed"]["qp_per_s"],
+        "unit": "QP-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": config,
+        "mixed_over_fp64": res["mixed"]["qp_per_s"] / res["fp64"]["qp_per_s"],
+    }
+
+
+def traffic_for(kname, shapes, profile_json):
+    """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs, valid for the launch shape they were taken with."""
+    tpath = os.path.join(ROOT, "profiles", profile_json)
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as f:
+        tr = json.load(f)
+    k = tr.get("kernels", {}).get(kname)
+    if k and tr.get("launch_mixed") == shapes["mixed"]:
+        return k["fetch_bytes"] + k["write_bytes"]
+    return None
+
+
+def fp64_record(res, shapes, step_bytes=None):
+    r = res["fp64"]
+    out = {"value": r["qp_per_s"], "ms_per_step": r["ms_per_step"],
+           "element_ms": r["element_ms"], "node_ms": r["node_ms"], "launch": shapes["fp64"]}
+    if step_bytes:
+        out["step_gbs"] = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_cube(args, akx):
+    log("building workload: cube %d^3 hexes -> tetrahedron_4" % args.n)
+    m, blocked, v0 = make_workload(args.n)
+    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 4, 1, 3
+    log("E=%d N=%d" % (E, N))
+    mat = dict(E=210e9, nu=0.3, rho=7800.0)
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(**mat)
+    model.init_full()
+    model.set("blocked_dofs", blocked)
+    reset(model, v0)
+    dt = 0.8 * model.get_stable_time_step()
+    model.set_time_step(dt)
+    log("dt=%.6e" % dt)
+
+    shapes = tune(model, args, akx)
+    res = measure(model, shapes, args, akx, v0, E * Q)
+
+    # accuracy of the mixed path against the FP64 path after error-steps steps
+    log("error run: %d steps mixed and FP64" % args.error_steps)
+    reset(model, v0)
+    model.set_launch(**shapes["mixed"])
+    model.solve_step(args.error_steps, akx.MIXED)
+    u_mixed = model.get_displacement()
+    e_mixed = model.get_energy("kinetic") + model.get_energy("potential")
+    reset(model, v0)
+    model.set_launch(**shapes["fp64"])
+    model.solve_step(args.error_steps, akx.FP64)
+    u_fp64 = model.get_displacement()
+    e_fp64 = model.get_energy("kinetic") + model.get_energy("potential")
+    rel_u = float(np.linalg.norm(u_mixed - u_fp64) / np.linalg.norm(u_fp64))
+    rel_e = float(abs(e_mixed - e_fp64) / abs(e_fp64))
+    log("mixed vs FP64 after %d steps: rel u error %.3e, energy %.3e" %
+        (args.error_steps, rel_u, rel_e))
+
+    def elem_kernel(patch, rc):
+        return ("void akx::k_element_patch<2, 0>(akx::ElemArgs)" if patch else
+                "void akx::k_element_mixed<2, 0, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
+    kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
+        res, shapes, model, elem_kernel, "void akx::k_node<3, float, false>(akx::NodeArgs)",
+        E, N, n, Q, d)
+    traffic = traffic_for(kname, shapes, "cube_tet4_traffic.json") if args.n == 100 else None
+    eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
+                                   node_ordered=bool(shapes["fp64"]["slot_layout"]))
+    copy_gbs = copy_bandwidth()
+    log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
+
+    cpu = None
+    bitwise = None
+    if not args.no_cpu:
+        threads = 1
+        log("CPU baseline: oracle, %d thread, %d steps (+2 warm-up) on the full mesh"
+            % (threads, args.cpu_steps))
+        o, sec = cpu_baseline(m, [mat], blocked, v0, dt, 2, args.cpu_steps, threads)
+        cpu = {"value": E * Q / sec, "unit": "QP-updates/s", "cores": threads, "kind": "port",
+               "sample": "oracle (C++ FP64 restatement of the reference loops, g++ -O3 "
+                         "-ffp-contract=off), full %d-element mesh, %d timed steps after 2 "
+                         "warm-up steps, %.3f s/step" % (E, args.cpu_steps, sec)}
+        log("CPU: %.3f s/step" % sec)
+        # the FP64 GPU path against the oracle on the full mesh, same step count
+        reset(model, v0)
+        model.set_launch(**shapes["fp64"])
+        model.solve_step(2 + args.cpu_steps, akx.FP64)
+        bitwise = bool(np.array_equal(model.get_displacement(), o.nodal("u")) and
+                       np.array_equal(model.get_velocity(), o.nodal("v")))
+        log("FP64 GPU == oracle after %d steps on the full mesh: %s" % (2 + args.cpu_steps, bitwise))
+        del o
+
+    out = base_record(args, res, "f32 element kernels / f64 nodal state", {
+        "workload": "cube-tet4: 3D linear elastic cube, %d^3 hexes Kuhn-split into %d "
+                    "tetrahedron_4 (1 QP each), %d nodes, +-1%% interior jitter, "
+                    "E=210 GPa nu=0.3 rho=7800, bottom blocked, top v_z=-1 m/s, "
+                    "dt=0.8 x stable" % (args.n, E, N),
+        "elements": E, "nodes": N, "qp_per_element": Q, "precision": "mixed",
+        "parallelism": "1 GPU"})
+    out.update({
+        "fp64": fp64_record(res, shapes, eb64 + nb64),
+        "launch_mixed": shapes["mixed"],
+        "accuracy": {"steps": args.error_steps, "rel_u_error_mixed_vs_fp64": rel_u,
+                     "rel_energy_error_mixed_vs_fp64": rel_e,
+                     "fp64_gpu_bitwise_equal_oracle": bitwise},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": kbytes, "avg_ms": kms},
+        "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "element_ms": res["mixed"]["element_ms"],
+                          "node_ms": res["mixed"]["node_ms"],
+                          "element_gbs": eb / (res["mixed"]["element_ms"] * 1e-3) / 1e9,
+                          "node_gbs": nb / (res["mixed"]["node_ms"] * 1e-3) / 1e9},
+        "copy_ceiling_gbs": copy_gbs,
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def bench_hex8(args, akx):
+    from akantu_amd import mesh as akm
+    nh = args.n if args.n != 100 else 128
+    log("building workload: cube %d^3 hexahedron_8, neo-Hookean" % nh)
+    m = akm.cube_hex8(nh)
+    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 8, 8, 3
+    X = m.nodes
+    blocked = np.zeros_like(X, dtype=bool)
+    blocked[np.isclose(X[:, 2], 0.0)] = True
+    top = np.nonzero(np.isclose(X[:, 2], 1.0))[0]
+    v0 = np.zeros_like(X)
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(kind=akx.NEOHOOKEAN, E=1e7, nu=0.45, rho=1000.0)
+    model.init_full()
+    model.set("blocked_dofs", blocked)
+    dt = 0.8 * model.get_stable_time_step()
+    model.set_time_step(dt)
+    model.set_dirichlet_increment(top, 2, -0.5 * dt)
+    reset(model, v0)
+    log("E=%d N=%d dt=%.6e" % (E, N, dt))
+    shapes = tune(model, args, akx)
+    res = measure(model, shapes, args, akx, v0, E * Q)
+    steps = args.error_steps if args.error_steps != 1000 else 500
+    log("error run: %d steps mixed and FP64" % steps)
+    reset(model, v0)
+    model.set_launch(**shapes["mixed"])
+    model.solve_step(steps, akx.MIXED)
+    u_mixed = model.get_displacement()
+    reset(model, v0)
+    model.set_launch(**shapes["fp64"])
+    model.solve_step(steps, akx.FP64)
+    u_fp64 = model.get_displacement()
+    rel_u = float(np.linalg.norm(u_mixed - u_fp64) / np.linalg.norm(u_fp64))
+    log("mixed vs FP64 after %d steps: rel u error %.3e" % (steps, rel_u))
+
+    def elem_kernel(patch, rc):
+        return ("void akx::k_element_patch<3, 1>(akx::ElemArgs)" if patch else
+                "void akx::k_element_mixed<3, 1, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
+    kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
+        res, shapes, model, elem_kernel, "void akx::k_node<3, float, false>(akx::NodeArgs)",
+        E, N, n, Q, d)
+    out = base_record(args, res, "f32 element kernels / f64 nodal state", {
+        "workload": "hex8-neohookean: %d^3 hexahedron_8 (2x2x2 Gauss), %d nodes, finite-strain "
+                    "neo-Hookean E=10 MPa nu=0.45 rho=1000, bottom blocked, top compressed "
+                    "at 0.5 m/s, dt=0.8 x stable" % (nh, N),
+        "elements": E, "nodes": N, "qp_per_element": Q, "precision": "mixed",
+        "parallelism": "1 GPU"})
+    out.update({
+        "fp64": fp64_record(res, shapes),
+        "launch_mixed": shapes["mixed"],
+        "accuracy": {"steps": steps, "rel_u_error_mixed_vs_fp64": rel_u},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": kbytes, "avg_ms": kms},
+        "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    })
+    return out
+
+
+def bench_ensemble(args, akx):
+    from akantu_amd import mesh as akm
+    log("building workload: 512 triangle_3 specimens")
+    en = akm.concatenate_specimens(akm.ensemble_tri3(512))
+    m = en.mesh
+    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 3, 1, 2
+    model = akx.SolidMechanicsModel(m)
+    for s in en.specimens:
+        model.add_material(E=s.E, nu=s.nu, rho=s.rho)
+    model.assign_materials(en.element_material)
+    model.init_full()
+    model.set("blocked_dofs", en.blocked)
+    v0 = en.velocity
+    reset(model, v0)
+    dt_mat = model.get_stable_time_step_per_material()
+    model.set_node_time_steps(0.8 * dt_mat[en.node_specimen()])
+    log("E=%d N=%d, specimen dt in [%.3e, %.3e]" % (E, N, 0.8 * dt_mat.min(), 0.8 * dt_mat.max()))
+    shapes = tune(model, args, akx)
+    res = measure(model, shapes, args, akx, v0, E * Q, graph=True)
+    steps = args.error_steps if args.error_steps != 1000 else 2000
+    log("error run: %d steps mixed and FP64 (graph)" % steps)
+    u = {}
+    for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
+        reset(model, v0)
+        model.set_launch(**shapes[name])
+        model.solve_step_graph(steps, prec)
+        u[name] = model.get_displacement()
+    errs = []
+    for k in range(len(en.specimens)):
+        a, b = en.node_offsets[k], en.node_offsets[k + 1]
+        errs.append(float(np.linalg.norm(u["mixed"][a:b] - u["fp64"][a:b]) /
+                          np.linalg.norm(u["fp64"][a:b])))
+    errs = np.array(errs)
+    log("per-specimen rel u error: max %.3e median %.3e" % (errs.max(), np.median(errs)))
+
+    def elem_kernel(patch, rc):
+        return ("void akx::k_element_patch<0, 0>(akx::ElemArgs)" if patch else
+                "void akx::k_element_mixed<0, 0, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
+    kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
+        res, shapes, model, elem_kernel, "void akx::k_node<2, float, false>(akx::NodeArgs)",
+        E, N, n, Q, d, dt_per_node=True)
+    out = base_record(args, res, "f32 element kernels / f64 nodal state", {
+        "workload": "ensemble: 512 plane-strain triangle_3 specimens, %d elements and %d nodes "
+                    "in total, per-specimen E, nu, rho and stable dt, sinusoidal initial "
+                    "velocity, one captured HIP graph per run" % (E, N),
+        "elements": E, "nodes": N, "specimens": 512, "qp_per_element": Q,
+        "precision": "mixed", "parallelism": "1 GPU"})
+    out.update({
+        "fp64": fp64_record(res, shapes),
+        "launch_mixed": shapes["mixed"],
+        "accuracy": {"steps": steps, "per_specimen_rel_u_error_max": float(errs.max()),
+                     "per_specimen_rel_u_error_median": float(np.median(errs))},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": kbytes, "avg_ms": kms},
+        "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    })
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=100, help="hexes per cube edge (6 n^3 tets)")
+    ap.add_argument("--workload", default="cube-tet4",
+                    choices=["cube-tet4", "hex8-neohookean", "ensemble"])
+    ap.add_argument("--n", type=int, default=100, help="hexes per cube edge")
     ap.add_argument("--error-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -170,149 +503,9 @@ def main():
         sys.exit(2)
 
     from akantu_amd import akx
-
-    log("building workload: cube %d^3 hexes -> tetrahedron_4" % args.n)
-    m, blocked, v0 = make_workload(args.n)
-    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 4, 1, 3
-    log("E=%d N=%d" % (E, N))
-    model = akx.SolidMechanicsModel(m)
-    model.add_material(E=210e9, nu=0.3, rho=7800.0)
-    model.init_full()
-    model.set("blocked_dofs", blocked)
-    reset(model, v0)
-    dt = 0.8 * model.get_stable_time_step()
-    model.set_time_step(dt)
-    log("dt=%.6e" % dt)
-
-    res = {}
-    shapes = {}
-    keys = akx.SolidMechanicsModel.LAUNCH_KEYS
-    for name, prec, fixed in (("mixed", akx.MIXED, args.launch_mixed),
-                              ("fp64", akx.FP64, args.launch_fp64)):
-        if fixed:
-            shapes[name] = dict(zip(keys, (int(x) for x in fixed.split(","))))
-            log("%s: fixed launch %s" % (name, shapes[name]))
-        else:
-            shapes[name], tuned_ms = model.autotune(prec, 10)
-            log("%s: autotuned launch %s (%.4f ms/step)" % (name, shapes[name], tuned_ms))
-    for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
-        model.set_launch(**shapes[name])
-        t = timed(model, prec, args.steps, args.warmup, v0)
-        el_ms, nd_ms = kernel_times(model, prec, args.steps, v0)
-        res[name] = dict(seconds=t, ms_per_step=1e3 * t / args.steps,
-                         qp_per_s=E * Q * args.steps / t, element_ms=el_ms, node_ms=nd_ms,
-                         launch=shapes[name])
-        log("%s: %.4f ms/step  (element %.4f ms, node %.4f ms)" %
-            (name, res[name]["ms_per_step"], el_ms, nd_ms))
-
-    # accuracy of the mixed path against the FP64 path after error-steps steps
-    log("error run: %d steps mixed and FP64" % args.error_steps)
-    reset(model, v0)
-    model.set_launch(**shapes["mixed"])
-    model.solve_step(args.error_steps, akx.MIXED)
-    u_mixed = model.get_displacement()
-    e_mixed = model.get_energy("kinetic") + model.get_energy("potential")
-    reset(model, v0)
-    model.set_launch(**shapes["fp64"])
-    model.solve_step(args.error_steps, akx.FP64)
-    u_fp64 = model.get_displacement()
-    e_fp64 = model.get_energy("kinetic") + model.get_energy("potential")
-    rel_u = float(np.linalg.norm(u_mixed - u_fp64) / np.linalg.norm(u_fp64))
-    rel_e = float(abs(e_mixed - e_fp64) / abs(e_fp64))
-    log("mixed vs FP64 after %d steps: rel u error %.3e, energy %.3e" %
-        (args.error_steps, rel_u, rel_e))
-
-    # roofline of the dominant kernel (mixed path)
-    patch = shapes["mixed"]["slot_layout"] == 2
-    eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
-                               tables=not shapes["mixed"]["mixed_recompute"],
-                               node_ordered=bool(shapes["mixed"]["slot_layout"]),
-                               patch_nodes=model.query("patch_nodes_total") if patch else None,
-                               patch_blocks=model.query("patch_blocks") if patch else 0)
-    el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
-    rc_flag = "true" if shapes["mixed"]["mixed_recompute"] else "false"
-    if el_ms >= nd_ms:
-        kname = ("void akx::k_element_patch<2, 0>(akx::ElemArgs)" if patch else
-                 "void akx::k_element_mixed<2, 0, %s>(akx::ElemArgs)" % rc_flag)
-        kbytes, kms = eb, el_ms
-    else:
-        kname, kbytes, kms = "void akx::k_node<3, float, false>(akx::NodeArgs)", nb, nd_ms
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    # HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # runs, valid for the launch shape they were taken with
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_c1.json")
-    if os.path.exists(tpath) and args.n == 100:
-        with open(tpath) as f:
-            tr = json.load(f)
-        k = tr.get("kernels", {}).get(kname)
-        if k and tr.get("launch_mixed") == shapes["mixed"]:
-            traffic = k["fetch_bytes"] + k["write_bytes"]
-    step_bytes = eb + nb
-    step_gbs = step_bytes / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
-    eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
-                                   node_ordered=bool(shapes["fp64"]["slot_layout"]))
-    copy_gbs = copy_bandwidth()
-    log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
-
-    cpu = None
-    bitwise = None
-    if not args.no_cpu:
-        threads = 1
-        log("CPU baseline: oracle, %d thread, %d steps (+2 warm-up) on the full mesh"
-            % (threads, args.cpu_steps))
-        o, sec = cpu_baseline(m, blocked, v0, dt, 2, args.cpu_steps, threads)
-        cpu = {"value": E * Q / sec, "unit": "QP-updates/s", "cores": threads, "kind": "port",
-               "sample": "oracle (C++ FP64 restatement of the reference loops, g++ -O3 "
-                         "-ffp-contract=off), full 6M-element mesh, %d timed steps after 2 "
-                         "warm-up steps, %.3f s/step" % (args.cpu_steps, sec)}
-        log("CPU: %.3f s/step" % sec)
-        # the FP64 GPU path against the oracle on the full mesh, same step count
-        reset(model, v0)
-        model.solve_step(2 + args.cpu_steps, akx.FP64)
-        bitwise = bool(np.array_equal(model.get_displacement(), o.nodal("u")) and
-                       np.array_equal(model.get_velocity(), o.nodal("v")))
-        log("FP64 GPU == oracle after %d steps on the full mesh: %s" % (2 + args.cpu_steps, bitwise))
-        del o
-
-    out = {
-        "metric": METRIC,
-        "value": res["mixed"]["qp_per_s"],
-        "unit": "QP-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": res["mixed"]["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 element kernels / f64 nodal state",
-        "data": "synthetic",
-        "config": {"workload": "3D linear elastic cube, %d^3 hexes Kuhn-split into %d "
-                               "tetrahedron_4 (1 QP each), %d nodes, +-1%% interior jitter, "
-                               "E=210 GPa nu=0.3 rho=7800, bottom blocked, top v_z=-1 m/s, "
-                               "dt=0.8 x stable" % (args.n, E, N),
-                   "elements": E, "nodes": N, "qp_per_element": Q, "precision": "mixed",
-                   "parallelism": "1 GPU"},
-        "fp64": {"value": res["fp64"]["qp_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
-                 "element_ms": res["fp64"]["element_ms"], "node_ms": res["fp64"]["node_ms"],
-                 "step_gbs": (eb64 + nb64) / (res["fp64"]["ms_per_step"] * 1e-3) / 1e9},
-        "mixed_over_fp64": res["mixed"]["qp_per_s"] / res["fp64"]["qp_per_s"],
-        "accuracy": {"steps": args.error_steps, "rel_u_error_mixed_vs_fp64": rel_u,
-                     "rel_energy_error_mixed_vs_fp64": rel_e,
-                     "fp64_gpu_bitwise_equal_oracle": bitwise},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": kbytes, "avg_ms": kms},
-        "roofline_step": {"algorithmic_bytes": step_bytes, "achieved": step_gbs,
-                          "frac": step_gbs / HBM_PEAK_GBS,
-                          "element_ms": el_ms, "node_ms": nd_ms,
-                          "element_gbs": eb / (el_ms * 1e-3) / 1e9,
-                          "node_gbs": nb / (nd_ms * 1e-3) / 1e9},
-        "copy_ceiling_gbs": copy_gbs,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    fn = {"cube-tet4": bench_cube, "hex8-neohookean": bench_hex8,
+          "ensemble": bench_ensemble}[args.workload]
+    print(json.dumps(fn(args, akx)), flush=True)
 
 
 if __name__ == "__main__":

@@ -155,6 +155,29 @@ int akx_set_time_step(akx_ctx* ctx, double dt);
  * akx_set_dirichlet_increment are applied before each predictor. */
 int akx_solve_step(akx_ctx* ctx, int nsteps, int precision);
 
+/* akx_solve_step as one HIP graph launch: the 2 nsteps + 1 kernel launches of
+ * the steps are captured once and replayed; the capture is redone when the
+ * step count, the precision, the time step(s), the Dirichlet increment, the
+ * presence of external forces or the launch shape changes.  Same results,
+ * bit for bit, as akx_solve_step; no per-kernel timing.  Replaces the
+ * driver loop `for (s...) model.solveStep();` (examples/explicit/
+ * explicit_dynamic.cc:88-99) for loops without host work between steps. */
+int akx_solve_step_graph(akx_ctx* ctx, int nsteps, int precision);
+
+/* One time step per node (nb_nodes values, caller numbering; NULL restores
+ * the single akx_set_time_step value): independent bodies meshed into one
+ * context (no element joins two bodies) then advance together, each with the
+ * time step its own SolidMechanicsModel::setTimeStep
+ * (solid_mechanics_model.cc:438) would have received.  The energies that
+ * scale with the time step (external work) use the single value. */
+int akx_set_node_time_steps(akx_ctx* ctx, const double* dt_per_node);
+
+/* SolidMechanicsModel::getStableTimeStep (solid_mechanics_model.cc:482-533)
+ * restricted to the elements of each material: dt[m] for every material m
+ * (one value per akx_material_add), the stable step of body m when each body
+ * has its own material. */
+int akx_stable_time_step_per_material(akx_ctx* ctx, double* dt);
+
 /* SolidMechanicsModel::assembleInternalForces (solid_mechanics_model.cc:375-414)
  * on the current displacement; the result is the AKX_INTERNAL_FORCE field. */
 int akx_assemble_internal_forces(akx_ctx* ctx, int precision);
