@@ -61,6 +61,15 @@ SIGNATURES = {
     "akx_set_launch": (_I, [_P, _I, _I, _I, _I, _I, _I]),
     "akx_autotune": (_I, [_P, _I, _I, _P, _PD]),
     "akx_query": (_I, [_P, ctypes.c_char_p, _PD]),
+    "akx_surrogate_setup": (_I, [_P, _I, _D]),
+    "akx_surrogate_generate": (_I, [_P, _I64, _I, _D, _D, _D, ctypes.c_uint64]),
+    "akx_surrogate_train": (_I, [_P, _I, _I, _D, ctypes.c_uint64, _PD]),
+    "akx_surrogate_params": (_I, [_P, _P, _I64, _I]),
+    "akx_surrogate_gradients": (_I, [_P, _I64, _P, _P, _P, _PD]),
+    "akx_surrogate_infer": (_I, [_P, _I64, _P, _P]),
+    "akx_surrogate_dataset": (_I, [_P, _I64, _I64, _P, _P]),
+    "akx_surrogate_enable": (_I, [_P, _I]),
+    "akx_surrogate_fallbacks": (_I, [_P, ctypes.POINTER(ctypes.c_uint64), _I]),
 }
 
 
@@ -220,6 +229,62 @@ class SolidMechanicsModel:
                                         ctypes.byref(best)))
         shape = {k: int(v) for k, v in zip(self.LAUNCH_KEYS, chosen)}
         return shape, best.value
+
+    # -- constitutive surrogate (neohookean_damage on tetrahedron_4) --
+    def surrogate_setup(self, mat_id, eps_max):
+        self.ctx.check(lib.akx_surrogate_setup(self.ctx.h, int(mat_id), float(eps_max)))
+
+    def surrogate_generate(self, paths, steps, inc_std=2e-4, yd_scale=1e6, yd_shape=5.0,
+                           seed=1):
+        self.ctx.check(lib.akx_surrogate_generate(self.ctx.h, int(paths), int(steps),
+                                                  float(inc_std), float(yd_scale),
+                                                  float(yd_shape), int(seed)))
+
+    def surrogate_train(self, iterations, batch=8192, lr=2e-3, seed=7):
+        loss = ctypes.c_double()
+        self.ctx.check(lib.akx_surrogate_train(self.ctx.h, int(iterations), int(batch),
+                                               float(lr), int(seed), ctypes.byref(loss)))
+        return loss.value
+
+    def surrogate_params(self, values=None):
+        """Download (values None) or upload the FP32 parameter vector."""
+        n = int(self.query("surrogate_parameters"))
+        if values is None:
+            out = np.zeros(n, dtype=np.float32)
+            self.ctx.check(lib.akx_surrogate_params(self.ctx.h, _ptr(out), n, 0))
+            return out
+        arr = np.ascontiguousarray(values, dtype=np.float32)
+        self.ctx.check(lib.akx_surrogate_params(self.ctx.h, _ptr(arr), arr.size, 1))
+
+    def surrogate_gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(int(self.query("surrogate_parameters")), dtype=np.float32)
+        loss = ctypes.c_double()
+        self.ctx.check(lib.akx_surrogate_gradients(self.ctx.h, x.shape[0], _ptr(x), _ptr(t),
+                                                   _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def surrogate_infer(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        self.ctx.check(lib.akx_surrogate_infer(self.ctx.h, x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def surrogate_dataset(self, first, count):
+        x = np.zeros((count, 7), dtype=np.float32)
+        t = np.zeros((count, 7), dtype=np.float32)
+        self.ctx.check(lib.akx_surrogate_dataset(self.ctx.h, int(first), int(count), _ptr(x),
+                                                 _ptr(t)))
+        return x, t
+
+    def surrogate_enable(self, on=True):
+        self.ctx.check(lib.akx_surrogate_enable(self.ctx.h, int(bool(on))))
+
+    def surrogate_fallbacks(self, reset=False):
+        v = ctypes.c_uint64()
+        self.ctx.check(lib.akx_surrogate_fallbacks(self.ctx.h, ctypes.byref(v), int(bool(reset))))
+        return v.value
 
     # -- fields --
     def _shape(self, name):

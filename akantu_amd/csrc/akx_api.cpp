@@ -304,6 +304,19 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.lconn = c->lconn;
   A.lrank = c->lrank;
   A.tabf = c->tabf;
+  A.sur_img = nullptr;
+  A.sur_b = nullptr;
+  A.sur_eps = A.sur_sS = A.sur_sW = 0.f;
+  A.sur_fallbacks = nullptr;
+  if (precision == AKX_MIXED && c->sur.enabled && g.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
+    A.sur_img = c->sur.img;
+    A.sur_b = c->sur.imgb;
+    A.sur_eps = float(c->sur.eps);
+    A.sur_sS = float(c->sur.sS);
+    A.sur_sW = float(c->sur.sW);
+    A.sur_fallbacks = c->sur.fallbacks;
+    A.patch = 0;
+  }
   return A;
 }
 
@@ -1020,6 +1033,7 @@ int akx_solve_step_graph(akx_ctx* c, int nsteps, int precision) {
   k.fext = c->has_fext; k.bc = c->has_bc_inc; k.dt = c->dt;
   for (int i = 0; i < 3; ++i) k.bc_inc[i] = c->bc_inc[i];
   k.dt_node = c->dt_node;
+  k.surrogate = c->sur.enabled;
   if (!c->graph_exec || !(k == c->graph_key)) {
     free_graph(c);
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1189,7 +1203,142 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "xcd_remap") *value = c->xcd_remap;
   else if (k == "mixed_recompute") *value = c->mixed_recompute;
   else if (k == "slot_layout") *value = c->node_ordered;
+  else if (k == "node_stage") *value = c->node_stage;
+  else if (k == "surrogate_eps") *value = c->sur.eps;
+  else if (k == "surrogate_stress_scale") *value = c->sur.sS;
+  else if (k == "surrogate_energy_scale") *value = c->sur.sW;
+  else if (k == "surrogate_samples") *value = double(c->sur.nsamples);
+  else if (k == "surrogate_parameters") *value = double(akx::mlp::NPARAM);
+  else if (k == "surrogate_enabled") *value = c->sur.enabled ? 1. : 0.;
   else return fail(c, AKX_ERR_ARGUMENT, "unknown query key " + k);
+  return AKX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// constitutive surrogate
+int akx_surrogate_setup(akx_ctx* c, int mat_id, double eps_max) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (mat_id < 0 || mat_id >= int(c->mats.size()))
+    return fail(c, AKX_ERR_ARGUMENT, "no such material");
+  const auto& m = c->mats[mat_id];
+  if (m.kind != AKX_MATERIAL_NEOHOOKEAN_DAMAGE)
+    return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate stands in for neohookean_damage only");
+  if (c->d != 3) return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate is three-dimensional");
+  if (!(eps_max > 0) || eps_max > 0.5) return fail(c, AKX_ERR_ARGUMENT, "eps_max in (0, 0.5]");
+  (void)hipSetDevice(c->device);
+  auto& S = c->sur;
+  S.mat = mat_id;
+  S.lambda = m.lambda;
+  S.mu = m.mu;
+  S.Sd = m.Sd;
+  S.eps = eps_max;
+  S.sS = (m.lambda + 2 * m.mu) * eps_max * eps_max;
+  S.sW = (m.lambda + 2 * m.mu) * eps_max * eps_max * eps_max;
+  HIP_TRY(c, S.ensure_params());
+  HIP_TRY(c, S.refresh(c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  S.setup = true;
+  return AKX_OK;
+}
+
+int akx_surrogate_generate(akx_ctx* c, int64_t paths, int steps, double inc_std,
+                           double yd_scale, double yd_shape, uint64_t seed) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->sur.setup) return fail(c, AKX_ERR_STATE, "akx_surrogate_setup has not been called");
+  if (paths <= 0 || steps <= 0 || !(inc_std > 0) || !(yd_scale > 0) || !(yd_shape > 0))
+    return fail(c, AKX_ERR_ARGUMENT, "bad training-set parameters");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, c->sur.generate(c->stream, paths, steps, inc_std, yd_scale, yd_shape, seed));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_surrogate_train(akx_ctx* c, int iterations, int batch, double lr, uint64_t seed,
+                        double* final_loss) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->sur.setup || c->sur.nsamples == 0)
+    return fail(c, AKX_ERR_STATE, "no training set (akx_surrogate_generate)");
+  if (iterations <= 0 || batch <= 0 || batch % 256 != 0 || !(lr > 0))
+    return fail(c, AKX_ERR_ARGUMENT, "iterations > 0, batch a multiple of 256, lr > 0");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, c->sur.train(c->stream, iterations, batch, lr, seed, final_loss));
+  return AKX_OK;
+}
+
+int akx_surrogate_params(akx_ctx* c, float* params, int64_t count, int upload) {
+  if (!c || !params) return AKX_ERR_ARGUMENT;
+  if (count != akx::mlp::NPARAM)
+    return fail(c, AKX_ERR_ARGUMENT, "the network has " + std::to_string(akx::mlp::NPARAM) +
+                                         " parameters");
+  (void)hipSetDevice(c->device);
+  if (upload) HIP_TRY(c, c->sur.set_params(c->stream, params));
+  else HIP_TRY(c, c->sur.get_params(c->stream, params));
+  return AKX_OK;
+}
+
+int akx_surrogate_gradients(akx_ctx* c, int64_t n, const float* x, const float* t, float* grad,
+                            double* loss) {
+  if (!c || !x || !t || !grad) return AKX_ERR_ARGUMENT;
+  if (n <= 0 || n % 256 != 0 || n > (int64_t(1) << 24))
+    return fail(c, AKX_ERR_ARGUMENT, "batch must be a positive multiple of 256");
+  (void)hipSetDevice(c->device);
+  double l = 0;
+  HIP_TRY(c, c->sur.gradients(c->stream, n, x, t, grad, &l));
+  if (loss) *loss = l;
+  return AKX_OK;
+}
+
+int akx_surrogate_infer(akx_ctx* c, int64_t n, const float* x, float* y) {
+  if (!c || (n > 0 && (!x || !y)) || n < 0) return AKX_ERR_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, c->sur.infer(c->stream, n, x, y));
+  return AKX_OK;
+}
+
+int akx_surrogate_dataset(akx_ctx* c, int64_t first, int64_t count, float* x, float* t) {
+  if (!c || !x || !t) return AKX_ERR_ARGUMENT;
+  if (first < 0 || count < 0 || first + count > c->sur.nsamples)
+    return fail(c, AKX_ERR_ARGUMENT, "sample range outside the training set");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, hipMemcpy(x, c->sur.X + first * 7, size_t(count) * 28, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(t, c->sur.T + first * 7, size_t(count) * 28, hipMemcpyDeviceToHost));
+  return AKX_OK;
+}
+
+int akx_surrogate_enable(akx_ctx* c, int on) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!on) {
+    c->sur.enabled = false;
+    return AKX_OK;
+  }
+  if (!c->sur.setup) return fail(c, AKX_ERR_STATE, "akx_surrogate_setup has not been called");
+  if (c->et != AKX_TETRAHEDRON_4)
+    return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate element kernel is for tetrahedron_4");
+  for (const auto& m : c->mats)
+    if (m.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE &&
+        (m.lambda != c->sur.lambda || m.mu != c->sur.mu || m.Sd != c->sur.Sd))
+      return fail(c, AKX_ERR_UNSUPPORTED,
+                  "every neohookean_damage material must be the one the surrogate was set up for");
+  c->sur.enabled = true;
+  return AKX_OK;
+}
+
+int akx_surrogate_fallbacks(akx_ctx* c, uint64_t* count, int reset) {
+  if (!c || !count) return AKX_ERR_ARGUMENT;
+  if (!c->sur.fallbacks) {
+    *count = 0;
+    return AKX_OK;
+  }
+  (void)hipSetDevice(c->device);
+  unsigned long long v = 0;
+  HIP_TRY(c, hipMemcpyAsync(&v, c->sur.fallbacks, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *count = v;
+  if (reset) HIP_TRY(c, hipMemset(c->sur.fallbacks, 0, 8));
   return AKX_OK;
 }
 
@@ -1198,7 +1347,7 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   if (rc) return rc;
   if (precision != AKX_FP64 && precision != AKX_MIXED)
     return fail(c, AKX_ERR_ARGUMENT, "unknown precision");
-  if (!(c->dt > 0)) return fail(c, AKX_ERR_STATE, "time step not set");
+  if (!(c->dt > 0) && !c->dt_node) return fail(c, AKX_ERR_STATE, "time step not set");
   if (nsteps <= 0) nsteps = 10;
   (void)hipSetDevice(c->device);
   // save the state every step touches

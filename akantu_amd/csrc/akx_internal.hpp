@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "akx_device.hpp"
+#include "akx_mlp.hpp"
 
 namespace akx {
 
@@ -56,6 +57,11 @@ struct ElemArgs {
   const uint16_t* lconn;     // [a][E] patch-local node index
   const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
   ElemTablesF tabf;
+  // constitutive surrogate (k_element_surrogate); sur_img == nullptr: plain law
+  const void* sur_img;
+  const float* sur_b;
+  float sur_eps, sur_sS, sur_sW;
+  unsigned long long* sur_fallbacks;
 };
 
 struct NodeArgs {
@@ -119,6 +125,58 @@ hipError_t launch_energy(int kind_energy, int et, int mat_kind_uniform, hipStrea
 hipError_t launch_point_fields(int et, int kind, hipStream_t s, const ElemArgs& A, double* gradu,
                                double* stress);
 hipError_t launch_add(hipStream_t s, int64_t n, const double* x, const double* y, double* out);
+
+// The constitutive surrogate of one context: training set, FP32 master
+// weights with Adam moments, BF16 working copies, batch buffers
+// (akx_surrogate.hip).
+struct Surrogate {
+  // plain law it stands in for and the scaling of inputs / outputs
+  bool setup = false, enabled = false;
+  int mat = -1;
+  double lambda = 0, mu = 0, Sd = 0, eps = 0, sS = 0, sW = 0;
+  // parameters
+  float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+  int64_t step = 0;
+  __bf16* Wbf[5] = {};
+  __bf16* WT[5] = {};
+  mlp::bf16x8* img = nullptr;
+  float* imgb = nullptr;
+  unsigned long long* fallbacks = nullptr;
+  // training set
+  float *X = nullptr, *T = nullptr;
+  int64_t nsamples = 0;
+  // batch buffers
+  int bcap = 0;
+  __bf16 *X0 = nullptr, *X0T = nullptr, *dY = nullptr, *dYT = nullptr;
+  float *Tb = nullptr, *Yb = nullptr, *loss_part = nullptr;
+  __bf16* H[5] = {};
+  __bf16* HT[5] = {};
+  float* part[5] = {};
+  float* Zt[4] = {};
+  __bf16* dZ[4] = {};
+  __bf16* dZT[4] = {};
+  float *io_x = nullptr, *io_y = nullptr;
+  int64_t io_cap = 0;
+
+  ~Surrogate();
+  void release();
+  void release_batch();
+  hipError_t ensure_params();
+  hipError_t ensure_batch(int B);
+  hipError_t ensure_io(int64_t n);
+  hipError_t refresh(hipStream_t s);
+  hipError_t pass(hipStream_t s, int B, bool grad_only, float lr);
+  hipError_t batch_loss(hipStream_t s, int B, double* loss);
+  hipError_t generate(hipStream_t s, int64_t paths, int steps, double inc_std, double yd_scale,
+                      double yd_shape, uint64_t seed);
+  hipError_t train(hipStream_t s, int iterations, int B, double lr, uint64_t seed,
+                   double* final_loss);
+  hipError_t gradients(hipStream_t s, int64_t n, const float* x, const float* t, float* grad,
+                       double* loss);
+  hipError_t infer(hipStream_t s, int64_t n, const float* x, float* y);
+  hipError_t set_params(hipStream_t s, const float* host);
+  hipError_t get_params(hipStream_t s, float* host);
+};
 
 }  // namespace akx
 
@@ -202,15 +260,18 @@ struct akx_ctx {
     bool fext = false, bc = false;
     double dt = 0, bc_inc[3] = {0, 0, 0};
     const double* dt_node = nullptr;
+    bool surrogate = false;
     bool operator==(const GraphKey& o) const {
       return nsteps == o.nsteps && precision == o.precision && eb == o.eb && nb == o.nb &&
              x == o.x && rc == o.rc && lay == o.lay && stage == o.stage && fext == o.fext &&
              bc == o.bc && dt == o.dt && bc_inc[0] == o.bc_inc[0] && bc_inc[1] == o.bc_inc[1] &&
-             bc_inc[2] == o.bc_inc[2] && dt_node == o.dt_node;
+             bc_inc[2] == o.bc_inc[2] && dt_node == o.dt_node && surrogate == o.surrogate;
     }
   } graph_key;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+
+  akx::Surrogate sur;
 
   std::vector<hipEvent_t> pool;  // per-step timing events
   bool timing = false;

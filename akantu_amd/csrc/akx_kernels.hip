@@ -15,6 +15,7 @@
 #include <algorithm>
 
 #include "akx_internal.hpp"
+#include "akx_mlp.hpp"
 
 namespace akx {
 
@@ -380,6 +381,164 @@ __global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
   float fe[dn];
   mixed_element_forces<ET, KIND, RECOMPUTE>(A, e, m, du, dX, fe);
   store_slots<d, n>(A, e, fe);
+}
+
+// ===========================================================================
+// element kernel, mixed precision, neo-Hookean + damage through the
+// constitutive surrogate (tetrahedron_4).  One wave = 32 elements; lanes j
+// and j + 32 both form element j's Green strain, the network (akx_mlp.hpp)
+// returns S_NH and W_NH for the 32 points together, lane j applies the damage
+// update and the B^T P integration.  A point outside the training envelope
+// (|E_k| > eps_max or damage outside [0, 1]) takes the plain FP32 law instead
+// and is counted.  Blocks are persistent: the network's weights are staged in
+// LDS once per block.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_element_surrogate(ElemArgs A) {
+#pragma clang fp contract(fast)
+  constexpr int d = 3, n = 4, dn = 12;
+  __shared__ mlp::Image L;
+  mlp::stage_image(L, static_cast<const mlp::bf16x8*>(A.sur_img), A.sur_b);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, j = lane & 31, h = lane >> 5;
+  const int64_t E = A.E;
+  const int64_t tiles = (A.count + 31) / 32;
+  const float eps = A.sur_eps, ieps = 1.f / A.sur_eps;
+  unsigned long long fallbacks = 0;
+  for (int64_t tile = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
+       tile += int64_t(gridDim.x) * (blockDim.x >> 6)) {
+    const int64_t t = tile * 32 + j;
+    const bool valid = t < A.count;
+    const int64_t e = valid ? (A.filter ? int64_t(A.filter[t]) : t) : 0;
+    const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+    int64_t nodes[n];
+#pragma unroll
+    for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
+    float du[dn], dX[dn];
+    double u0[d], x0[d];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      u0[i] = A.u[nodes[0] * d + i];
+      x0[i] = A.X[nodes[0] * d + i];
+      du[i] = 0.f;
+      dX[i] = 0.f;
+    }
+#pragma unroll
+    for (int a = 1; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
+        dX[i + a * d] = float(A.X[nodes[a] * d + i] - x0[i]);
+      }
+    // B = J^-1 dN/dxi and det(J) w from the coordinates (tetrahedron_4)
+    float Jm[9];
+#pragma unroll
+    for (int jj = 0; jj < d; ++jj)
+#pragma unroll
+      for (int i = 0; i < d; ++i) Jm[i + jj * d] = dX[jj + (i + 1) * d];
+    const float det = det3(Jm);
+    const float idet = 1.f / det;
+    float Ji[9];
+    Ji[0] = (Jm[4] * Jm[8] - Jm[7] * Jm[5]) * idet;
+    Ji[1] = (Jm[2] * Jm[7] - Jm[8] * Jm[1]) * idet;
+    Ji[2] = (Jm[1] * Jm[5] - Jm[4] * Jm[2]) * idet;
+    Ji[3] = (Jm[5] * Jm[6] - Jm[8] * Jm[3]) * idet;
+    Ji[4] = (Jm[0] * Jm[8] - Jm[6] * Jm[2]) * idet;
+    Ji[5] = (Jm[2] * Jm[3] - Jm[5] * Jm[0]) * idet;
+    Ji[6] = (Jm[3] * Jm[7] - Jm[6] * Jm[4]) * idet;
+    Ji[7] = (Jm[1] * Jm[6] - Jm[7] * Jm[0]) * idet;
+    Ji[8] = (Jm[0] * Jm[4] - Jm[3] * Jm[1]) * idet;
+    float Bq[dn];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      float s = 0.f;
+#pragma unroll
+      for (int l = 0; l < d; ++l) s -= Ji[i + l * d];
+      Bq[i] = s;
+#pragma unroll
+      for (int a = 1; a < n; ++a) Bq[i + a * d] = Ji[i + (a - 1) * d];
+    }
+    const float jq = det * A.tabf.w[0];
+    float G[9];
+#pragma unroll
+    for (int jj = 0; jj < d; ++jj)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float s = 0.f;
+#pragma unroll
+        for (int a = 1; a < n; ++a) s += du[i + a * d] * Bq[jj + a * d];
+        G[i + jj * d] = s;
+      }
+    // Green-Lagrange strain E = (G + G^T + G^T G) / 2, Voigt xx yy zz yz xz xy
+    float Ev[6];
+    const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int a = vi[k], b = vj[k];
+      float gtg = 0.f;
+#pragma unroll
+      for (int l = 0; l < d; ++l) gtg += G[l + a * d] * G[l + b * d];
+      Ev[k] = 0.5f * (G[a + b * d] + G[b + a * d] + gtg);
+    }
+    const double dam_old = valid ? A.damage[e] : 0.;
+    bool inside = dam_old >= 0. && dam_old <= 1.;
+    float x[7], y[7];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      inside = inside && fabsf(Ev[k]) <= eps;
+      x[k] = valid ? Ev[k] * ieps : 0.f;
+    }
+    x[6] = float(dam_old);
+    mlp::eval_wave(L, x, y);
+    if (!valid || h) continue;
+    float S[9], W;
+    if (inside) {
+      const float tr = Ev[0] + Ev[1] + Ev[2];
+      const float ee = Ev[0] * Ev[0] + Ev[1] * Ev[1] + Ev[2] * Ev[2] +
+                       2.f * (Ev[3] * Ev[3] + Ev[4] * Ev[4] + Ev[5] * Ev[5]);
+      float Sv[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        Sv[k] = 2.f * m.mu_f * Ev[k] + (k < 3 ? m.lambda_f * tr : 0.f) + A.sur_sS * y[k];
+      W = 0.5f * m.lambda_f * tr * tr + m.mu_f * ee + A.sur_sW * y[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        S[vi[k] + vj[k] * d] = Sv[k];
+        S[vj[k] + vi[k] * d] = Sv[k];
+      }
+    } else {
+      neohookean_stress_f32<d>(m.lambda_f, m.mu_f, G, S, &W);
+      ++fallbacks;
+    }
+    // Marigo update (material_marigo_inline_impl.hh:67-76) with Y = W
+    double dam = dam_old;
+    const double Fd = double(W) - A.Yd[e] - m.Sd * dam;
+    if (Fd > 0) dam = (double(W) - A.Yd[e]) / m.Sd;
+    dam = fmin(dam, 1.);
+    A.damage[e] = dam;
+    const float scale = jq * float(1. - dam);
+    float P[9];
+#pragma unroll
+    for (int i = 0; i < d; ++i)
+#pragma unroll
+      for (int jj = 0; jj < d; ++jj) {
+        float s = S[i + jj * d];
+#pragma unroll
+        for (int l = 0; l < d; ++l) s += G[i + l * d] * S[l + jj * d];
+        P[i + jj * d] = s * scale;
+      }
+    float fe[dn];
+#pragma unroll
+    for (int a = 0; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float s = 0.f;
+#pragma unroll
+        for (int jj = 0; jj < d; ++jj) s += P[i + jj * d] * Bq[jj + a * d];
+        fe[i + a * d] = s;
+      }
+    store_slots<d, n>(A, e, fe);
+  }
+  if (fallbacks) atomicAdd(A.sur_fallbacks, fallbacks);
 }
 
 // ===========================================================================
@@ -1007,6 +1166,15 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
     case NEOHOOKEAN: k_element_fp64<ET, NEOHOOKEAN><<<g, block, 0, s>>>(A); break;
     case NH_DAMAGE: k_element_fp64<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
     default: return hipErrorInvalidValue;
+    }
+  } else if (A.sur_img) {
+    if constexpr (ET == TET4) {
+      if (kind != NH_DAMAGE) return hipErrorInvalidValue;
+      const int64_t tiles = (A.count + 31) / 32;
+      const unsigned gs = unsigned(std::min<int64_t>(256, (tiles + 3) / 4));
+      k_element_surrogate<<<gs, 256, 0, s>>>(A);
+    } else {
+      return hipErrorInvalidValue;
     }
   } else if (A.patch) {
     // one block per patch of 256 elements (the patches are built for that size)

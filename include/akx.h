@@ -231,6 +231,51 @@ int akx_query(akx_ctx* ctx, const char* key, double* value);
  * time per step. */
 int akx_autotune(akx_ctx* ctx, int precision, int nsteps, int* chosen, double* best_ms);
 
+/* ---- constitutive surrogate -------------------------------------------------
+ * A 7 -> 128 x 4 -> 7 SiLU network (BF16 weights and activations, FP32
+ * accumulation, gfx950 MFMA) that stands in for the neo-Hookean part of a
+ * neohookean_damage material (material_neohookean_inline_impl.hh:46-65,
+ * 145-158); the Marigo damage update (material_marigo_inline_impl.hh:67-76)
+ * stays closed-form because it needs each point's own threshold Yd.
+ *   inputs  (7): Green-Lagrange strain E (xx, yy, zz, yz, xz, xy) / eps_max,
+ *                damage
+ *   outputs (7): (S_NH - S_lin) / sS (same order), (W_NH - W_lin) / sW,
+ *                S_lin = lambda tr(E) I + 2 mu E, W_lin = lambda/2 tr(E)^2 + mu E:E,
+ *                sS = (lambda + 2 mu) eps_max^2, sW = (lambda + 2 mu) eps_max^3.
+ * No counterpart in the reference (it has no learned material). */
+
+/* Bind the surrogate to material mat_id (kind neohookean_damage, 3D) with the
+ * strain envelope |E_k| <= eps_max; initialises the network. */
+int akx_surrogate_setup(akx_ctx* ctx, int mat_id, double eps_max);
+/* Training set on the GPU: `paths` random monotone / cyclic strain paths of
+ * `steps` steps (Gaussian increments of standard deviation inc_std, reflected
+ * at eps_max), damage along each path from the plain law with a Weibull
+ * threshold Yd = yd_scale * Weibull(shape yd_shape); paths * steps samples. */
+int akx_surrogate_generate(akx_ctx* ctx, int64_t paths, int steps, double inc_std,
+                           double yd_scale, double yd_shape, uint64_t seed);
+/* Minibatch Adam on the GPU's own forward and backward kernels (batch a
+ * multiple of 256; learning rate warm-up then cosine decay to lr / 100);
+ * final_loss (optional): mean squared error of the last batch. */
+int akx_surrogate_train(akx_ctx* ctx, int iterations, int batch, double lr, uint64_t seed,
+                        double* final_loss);
+/* FP32 parameters (W0, b0, ..., W4, b4; row-major W[out][in]); count must be
+ * the network's parameter count (akx_query "surrogate_parameters"). */
+int akx_surrogate_params(akx_ctx* ctx, float* params, int64_t count, int upload);
+/* Gradient of the mean squared error over the batch (x, t: n x 7 scaled
+ * inputs / targets, n a multiple of 256) with respect to every parameter, from
+ * the training kernels, without an update; loss (optional). */
+int akx_surrogate_gradients(akx_ctx* ctx, int64_t n, const float* x, const float* t,
+                            float* grad, double* loss);
+/* Network outputs for n scaled inputs (n x 7 -> n x 7), inference kernel. */
+int akx_surrogate_infer(akx_ctx* ctx, int64_t n, const float* x, float* y);
+/* Samples [first, first + count) of the training set (scaled, n x 7 each). */
+int akx_surrogate_dataset(akx_ctx* ctx, int64_t first, int64_t count, float* x, float* t);
+/* Mixed-precision steps evaluate neohookean_damage points through the
+ * network (1) or the plain law (0).  Points outside the envelope fall back to
+ * the plain law and are counted (akx_surrogate_fallbacks). */
+int akx_surrogate_enable(akx_ctx* ctx, int on);
+int akx_surrogate_fallbacks(akx_ctx* ctx, uint64_t* count, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,153 @@
+"""The constitutive surrogate on the GPU: MFMA inference and training kernels
+against the NumPy reference (tests/mlp_reference.py), the training set
+against the neo-Hookean law, what training gains, and the surrogate element
+step against the plain neohookean_damage step."""
+import numpy as np
+import pytest
+
+from akantu_amd import akx
+from akantu_amd import mesh as akm
+from tests import mlp_reference as ref
+
+pytestmark = pytest.mark.gpu
+
+PLATE = dict(E=1e9, nu=0.3, rho=1200.0)
+SD = 5e6
+
+
+def _lame(E, nu):
+    return nu * E / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def _plate_model(nx=12, ny=12, nz=3, seed=4, eps=0.05):
+    m = akm.notched_plate_tet4(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    yd = 1e6 * rng.weibull(5.0, size=m.nb_elements)
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(kind=akx.NEOHOOKEAN_DAMAGE, Sd=SD, Yd=yd, **PLATE)
+    model.init_full()
+    model.surrogate_setup(0, eps)
+    return model
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_inference_kernel_matches_reference():
+    """Chained 32x32x16 MFMA layers with the permuted weight image give the
+    network of the reference (checks the operand / result lane layout)."""
+    model = _plate_model()
+    p = ref.init_params(11).astype(np.float32)
+    model.surrogate_params(p)
+    assert np.array_equal(model.surrogate_params(), p)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, size=(1000, 7)).astype(np.float32)  # not a multiple of 32
+    y = model.surrogate_infer(x)
+    yr, _ = ref.forward(p.astype(np.float64), x, rounded=True)
+    assert _rel(y, yr) < 1e-2
+    assert np.abs(y - yr).max() < 3e-2 * np.abs(yr).max()
+
+
+def test_training_gradients_match_reference():
+    """Forward + backward MFMA kernels (no update) against the CPU reference
+    with the same BF16 rounding points, and the loss they report."""
+    model = _plate_model()
+    p = ref.init_params(21).astype(np.float32)
+    model.surrogate_params(p)
+    rng = np.random.default_rng(22)
+    x = rng.uniform(-1, 1, size=(512, 7)).astype(np.float32)
+    t = rng.normal(scale=0.3, size=(512, 7)).astype(np.float32)
+    g, loss = model.surrogate_gradients(x, t)
+    lr, gr = ref.loss_and_grad(p.astype(np.float64), x, t, rounded=True)
+    assert abs(loss - lr) <= 1e-3 * lr
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in ref.SHAPES])
+    for (name, _), a, b in zip(ref.SHAPES, offs[:-1], offs[1:]):
+        err = np.linalg.norm(g[a:b] - gr[a:b]) / np.linalg.norm(gr[a:b])
+        assert err < 2e-2, (name, err)
+    # and against the exact FP64 gradient, to the BF16 precision
+    _, ge = ref.loss_and_grad(p.astype(np.float64), x, t, rounded=False)
+    assert np.linalg.norm(g - ge) / np.linalg.norm(ge) < 5e-2
+
+
+def test_training_set_follows_the_plain_law():
+    model = _plate_model()
+    model.surrogate_generate(400, 50, seed=3)
+    assert model.query("surrogate_samples") == 20000
+    x, t = model.surrogate_dataset(0, 20000)
+    eps = model.query("surrogate_eps")
+    sS, sW = model.query("surrogate_stress_scale"), model.query("surrogate_energy_scale")
+    lam, mu = _lame(PLATE["E"], PLATE["nu"])
+    E = x[:, :6].astype(np.float64) * eps
+    assert np.abs(x[:, :6]).max() <= 1.0 + 1e-6
+    assert x[:, 6].min() >= 0 and x[:, 6].max() <= 1
+    S, W = ref.neohookean_green(E, lam, mu)
+    Sl, Wl = ref.linear_green(E, lam, mu)
+    assert np.abs(Sl + t[:, :6] * sS - S).max() <= 1e-5 * np.abs(S).max()
+    assert np.abs(Wl + t[:, 6] * sW - W).max() <= 1e-5 * np.abs(W).max()
+    # paths reach the envelope and damage grows on some of them
+    assert np.abs(x[:, :6]).max() > 0.5
+    assert (x[:, 6] > 0).mean() > 0.01
+
+
+def _stress_errors(model, x, t):
+    """Relative error of the surrogate's S_NH and of the linear part alone."""
+    eps = model.query("surrogate_eps")
+    sS = model.query("surrogate_stress_scale")
+    lam, mu = _lame(PLATE["E"], PLATE["nu"])
+    E = x[:, :6].astype(np.float64) * eps
+    S, _ = ref.neohookean_green(E, lam, mu)
+    Sl, _ = ref.linear_green(E, lam, mu)
+    y = model.surrogate_infer(x)
+    Ss = Sl + y[:, :6].astype(np.float64) * sS
+    return _rel(Ss, S), _rel(Sl, S)
+
+
+def test_training_gains_over_the_linear_law():
+    """A short training run: the network's stress is far closer to the
+    neo-Hookean stress than what is left without it (its linear part), on
+    samples it was not trained on."""
+    model = _plate_model()
+    model.surrogate_generate(4000, 50, seed=5)
+    loss0 = model.surrogate_train(1, batch=4096)
+    loss = model.surrogate_train(1500, batch=4096, lr=2e-3)
+    assert loss < 0.05 * loss0
+    held = _plate_model()
+    held.surrogate_generate(200, 50, seed=99)
+    x, t = held.surrogate_dataset(0, 10000)
+    err, err_lin = _stress_errors(model, x, t)
+    print("held-out relative stress error: surrogate %.3e, linear part alone %.3e" % (err, err_lin))
+    assert err < 0.1 * err_lin
+    assert err < 5e-3
+
+
+def test_surrogate_step_against_plain_law():
+    """The surrogate element kernel inside the explicit step: close to the
+    plain mixed step, and with an envelope nothing fits (eps_max tiny), every
+    point takes the plain law and the result is the plain step's."""
+    def run(enable, eps=0.05, train=True):
+        model = _plate_model(eps=eps)
+        if train:
+            model.surrogate_generate(4000, 50, seed=5)
+            model.surrogate_train(1500, batch=4096, lr=2e-3)
+        X = model.mesh.nodes
+        blocked = np.zeros_like(X, dtype=bool)
+        bottom = np.isclose(X[:, 1], 0.0)
+        top = np.isclose(X[:, 1], X[:, 1].max())
+        blocked[bottom] = True
+        model.set("blocked_dofs", blocked)
+        dt = 0.8 * model.get_stable_time_step()
+        model.set_time_step(dt)
+        model.set_dirichlet_increment(np.nonzero(top)[0], 1, 20.0 * dt)
+        model.surrogate_enable(enable)
+        model.solve_step(300, akx.MIXED)
+        return model.get_displacement(), model.get("damage"), model.surrogate_fallbacks()
+
+    u_plain, d_plain, _ = run(False)
+    u_sur, d_sur, fb = run(True)
+    print("surrogate vs plain after 300 steps: rel u %.3e, fallbacks %d" % (_rel(u_sur, u_plain), fb))
+    assert _rel(u_sur, u_plain) < 1e-3
+    u_fb, d_fb, fb_all = run(True, eps=1e-9, train=False)
+    assert fb_all > 0
+    assert _rel(u_fb, u_plain) < 1e-6
+    assert np.abs(d_fb - d_plain).max() < 1e-6

@@ -1,0 +1,74 @@
+"""Pins of the CPU reference of the surrogate network (tests/mlp_reference.py),
+which the GPU training and inference kernels are checked against."""
+import numpy as np
+
+from tests import mlp_reference as ref
+
+
+def test_exact_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    p = ref.init_params(1)
+    x = rng.uniform(-1, 1, size=(16, 7))
+    t = rng.normal(size=(16, 7))
+    _, g = ref.loss_and_grad(p, x, t, rounded=False)
+    # central differences on a sample of parameters of every layer
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in ref.SHAPES])
+    for (name, _), a, b in zip(ref.SHAPES, offs[:-1], offs[1:]):
+        for k in rng.choice(np.arange(a, b), size=min(6, b - a), replace=False):
+            h = 1e-6
+            pp, pm = p.copy(), p.copy()
+            pp[k] += h
+            pm[k] -= h
+            lp, _ = ref.loss_and_grad(pp, x, t, rounded=False)
+            lm, _ = ref.loss_and_grad(pm, x, t, rounded=False)
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (name, k, fd, g[k])
+
+
+def test_rounded_gradient_is_close_to_exact():
+    """BF16 rounding at the GPU's rounding points changes the gradient by
+    about the BF16 precision, not more."""
+    rng = np.random.default_rng(5)
+    p = ref.init_params(2)
+    x = rng.uniform(-1, 1, size=(256, 7)).astype(np.float32)
+    t = rng.normal(size=(256, 7))
+    _, ge = ref.loss_and_grad(p, x, t, rounded=False)
+    _, gr = ref.loss_and_grad(p, x, t, rounded=True)
+    assert np.linalg.norm(gr - ge) / np.linalg.norm(ge) < 3e-2
+
+
+def test_bf16_rounding():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5, 0.0], dtype=np.float32)
+    r = ref.bf16(v)
+    # ties to even: 1 + 2^-8 -> 1, 1 + 3 * 2^-8 -> 1 + 2^-6
+    assert list(r) == [1.0, 1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -6, -2.5, 0.0]
+
+
+def test_neohookean_green_matches_oracle_point_stress():
+    """The Green-strain form of the neo-Hookean law (used for the training
+    set) against the oracle's neo-Hookean point stress on F = U = sqrt(C)."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(9)
+    E_young, nu = 1e9, 0.3
+    lam = nu * E_young / ((1 + nu) * (1 - 2 * nu))
+    mu = E_young / (2 * (1 + nu))
+    Ev = rng.uniform(-0.05, 0.05, size=(20, 6))
+    S, W = ref.neohookean_green(Ev, lam, mu)
+    idx = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for k in range(20):
+        C = np.eye(3)
+        for m, (i, j) in enumerate(idx):
+            C[i, j] += 2 * Ev[k, m]
+            if i != j:
+                C[j, i] += 2 * Ev[k, m]
+        w, V = np.linalg.eigh(C)
+        U = V @ np.diag(np.sqrt(w)) @ V.T
+        So, _ = orc.point_stress(1, 3, E_young, nu, U - np.eye(3))
+        Wo = orc.point_energy(1, 3, E_young, nu, U - np.eye(3))
+        Sv = np.array([So[i, j] for i, j in idx])
+        assert np.abs(Sv - S[k]).max() <= 1e-9 * np.abs(So).max()
+        assert abs(Wo - W[k]) <= 1e-9 * abs(Wo)
+    # the linear part is the small-strain limit
+    Sl, Wl = ref.linear_green(Ev * 1e-4, lam, mu)
+    Sn, Wn = ref.neohookean_green(Ev * 1e-4, lam, mu)
+    assert np.abs(Sl - Sn).max() <= 1e-3 * np.abs(Sn).max()
