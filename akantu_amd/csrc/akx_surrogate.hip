@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_sur_gen(int64_t paths, int steps, doubl
   }
   nrm = sqrt(nrm);
   for (int k = 0; k < 6; ++k) dir[k] /= nrm;
-  const double drift = inc_std * (0.25 + 2.0 * r.uniform());
+  const double drift = inc_std * (0.5 + 2.5 * r.uniform());
   const int period = 10 + int(40 * r.uniform());
   const double Yd = yd_scale * pow(-log(r.uniform()), 1. / yd_shape);
   double E[6] = {0, 0, 0, 0, 0, 0};

@@ -23,7 +23,11 @@ Other workloads (--workload), reported the same way on their own JSON line:
   ensemble         512 independent plane-strain triangle_3 specimens (2,000 to
                    40,000 elements each), each with its own material and stable
                    time step, advanced together by one captured HIP graph per
-                   run; 2000 steps for the per-specimen error run.
+                   run; 2000 steps for the per-specimen error run;
+  plate-surrogate  notched plate, ~4M tetrahedron_4, neo-Hookean + damage: the
+                   plain law against the MLP surrogate (trained on the GPU
+                   first), with the surrogate's stress error on the run's own
+                   strain states.
 """
 from __future__ import annotations
 
@@ -168,12 +172,28 @@ def cpu_baseline(m, materials, blocked, v0, dt, warm=2, steps=10, threads=1):
     return o, t / steps
 
 
-def tune(model, args, akx):
+def tune(model, args, akx, profiled=None):
+    """Launch shapes: given on the command line, else the ones the committed
+    profiles were taken with (`profiled`, a traffic JSON under profiles/,
+    written from an earlier autotuned run), else the autotuner's choice."""
     shapes = {}
     keys = akx.SolidMechanicsModel.LAUNCH_KEYS
+    stored = {}
+    if profiled and not args.autotune:
+        path = os.path.join(ROOT, "profiles", profiled)
+        if os.path.exists(path):
+            with open(path) as f:
+                tr = json.load(f)
+            for name in ("mixed", "fp64"):
+                sh = tr.get("launch_" + name)
+                if sh and set(sh) == set(keys):
+                    stored[name] = sh
     for name, prec, fixed in (("mixed", akx.MIXED, args.launch_mixed),
                               ("fp64", akx.FP64, args.launch_fp64)):
-        if fixed:
+        if not fixed and name in stored:
+            shapes[name] = dict(stored[name])
+            log("%s: launch of the committed profiles %s" % (name, shapes[name]))
+        elif fixed:
             shapes[name] = dict(zip(keys, (int(x) for x in fixed.split(","))))
             log("%s: fixed launch %s" % (name, shapes[name]))
         else:
@@ -272,7 +292,7 @@ def bench_cube(args, akx):
     model.set_time_step(dt)
     log("dt=%.6e" % dt)
 
-    shapes = tune(model, args, akx)
+    shapes = tune(model, args, akx, "cube_tet4_traffic.json" if args.n == 100 else None)
     res = measure(model, shapes, args, akx, v0, E * Q)
 
     # accuracy of the mixed path against the FP64 path after error-steps steps
@@ -476,13 +496,150 @@ def bench_ensemble(args, akx):
     return out
 
 
+def bench_plate_surrogate(args, akx):
+    """Notched plate of tetrahedron_4 under tension, neo-Hookean + Marigo
+    damage (Weibull thresholds): the plain law against the MLP surrogate in
+    the mixed step, with the network trained on the GPU first."""
+    from akantu_amd import mesh as akm
+    nx = args.n if args.n != 100 else 150
+    nz = max(2, nx // 5)
+    log("building workload: notched plate %d x %d x %d cells -> tetrahedron_4" % (nx, nx, nz))
+    m = akm.notched_plate_tet4(nx, nx, nz)
+    E, N, n, Q, d = m.nb_elements, m.nb_nodes, 4, 1, 3
+    rng = np.random.default_rng(2024)
+    yd = 1e6 * rng.weibull(5.0, size=E)
+    Eyoung, nu, rho, Sd, eps = 1e9, 0.3, 1200.0, 5e6, 0.05
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(kind=akx.NEOHOOKEAN_DAMAGE, E=Eyoung, nu=nu, rho=rho, Sd=Sd, Yd=yd)
+    model.init_full()
+    X = m.nodes
+    blocked = np.zeros_like(X, dtype=bool)
+    blocked[np.isclose(X[:, 1], 0.0)] = True
+    top = np.nonzero(np.isclose(X[:, 1], X[:, 1].max()))[0]
+    model.set("blocked_dofs", blocked)
+    dt = 0.8 * model.get_stable_time_step()
+    model.set_time_step(dt)
+    model.set_dirichlet_increment(top, 1, 20.0 * dt)
+    v0 = np.zeros_like(X)
+    reset(model, v0)
+    log("E=%d N=%d dt=%.3e" % (E, N, dt))
+
+    # training set and training on the GPU
+    model.surrogate_setup(0, eps)
+    t0 = time.perf_counter()
+    model.surrogate_generate(40000, 250, inc_std=2e-4, yd_scale=1e6, yd_shape=5.0, seed=1)
+    model.synchronize()
+    t_gen = time.perf_counter() - t0
+    iters = args.train_iters
+    t0 = time.perf_counter()
+    loss = model.surrogate_train(iters, batch=8192, lr=2e-3, seed=7)
+    t_train = time.perf_counter() - t0
+    log("training set 10M samples in %.2f s; %d Adam iterations (batch 8192) in %.2f s, "
+        "final loss %.3e" % (t_gen, iters, t_train, loss))
+
+    # plain vs surrogate step, same launch shape (element-ordered slots)
+    model.set_launch(256, 256, 1, 1, 1, 1)
+    res = {}
+    for name, on in (("plain", False), ("surrogate", True)):
+        model.surrogate_enable(on)
+        model.set("damage", np.zeros(E))
+        t = timed(model, akx.MIXED, args.steps, args.warmup, v0)
+        el_ms, nd_ms = kernel_times(model, akx.MIXED, min(args.steps, 100), v0)
+        res[name] = dict(ms_per_step=1e3 * t / args.steps, qp_per_s=E * Q * args.steps / t,
+                         element_ms=el_ms, node_ms=nd_ms)
+        log("%s: %.4f ms/step (element %.4f ms, node %.4f ms)" % (name, res[name]["ms_per_step"],
+                                                                el_ms, nd_ms))
+    # accuracy on the simulation's own strain states after `steps` plain steps
+    steps = args.error_steps if args.error_steps != 1000 else 400
+    model.surrogate_enable(False)
+    model.set("damage", np.zeros(E))
+    reset(model, v0)
+    model.solve_step(steps, akx.MIXED)
+    u_plain = model.get_displacement()
+    G = model.get("grad_u").reshape(-1, 3, 3).transpose(0, 2, 1)  # (i, j) = du_i/dX_j
+    dmg = model.get("damage")
+    Eg = 0.5 * (G + G.transpose(0, 2, 1) + np.einsum("pki,pkj->pij", G, G))
+    idx = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    Ev = np.stack([Eg[:, i, j] for i, j in idx], axis=1)
+    lam, mu = nu * Eyoung / ((1 + nu) * (1 - 2 * nu)), Eyoung / (2 * (1 + nu))
+    inside = np.all(np.abs(Ev) <= eps, axis=1)
+    x = np.concatenate([Ev / eps, dmg[:, None]], axis=1).astype(np.float32)
+    y = model.surrogate_infer(x).astype(np.float64)
+    trE = Ev[:, 0] + Ev[:, 1] + Ev[:, 2]
+    S_lin = 2 * mu * Ev
+    S_lin[:, :3] += (lam * trE)[:, None]
+    S_sur = S_lin + y[:, :6] * model.query("surrogate_stress_scale")
+    # plain law in FP64 from C = I + 2E
+    C = np.zeros((E, 3, 3))
+    for k, (i, j) in enumerate(idx):
+        C[:, i, j] += 2 * Ev[:, k]
+        if i != j:
+            C[:, j, i] += 2 * Ev[:, k]
+    C += np.eye(3)
+    Ci = np.linalg.inv(C)
+    lnJ = 0.5 * np.log(np.linalg.det(C))
+    S_nh = mu * (np.eye(3) - Ci) + (lam * lnJ)[:, None, None] * Ci
+    S_ref = np.stack([S_nh[:, i, j] for i, j in idx], axis=1)
+    wgt = (1.0 - dmg)[:, None]
+    diff = (S_sur - S_ref) * wgt
+    rel_stress = float(np.linalg.norm(diff[inside]) / np.linalg.norm((S_ref * wgt)[inside]))
+    per_qp = np.linalg.norm(diff, axis=1) / np.maximum(np.linalg.norm(S_ref * wgt, axis=1), 1e-30)
+    p99 = float(np.percentile(per_qp[inside & (np.linalg.norm(S_ref, axis=1) > 1e-3 *
+                                               np.linalg.norm(S_ref, axis=1).max())], 99))
+    # the surrogate run itself, from the same start
+    model.surrogate_enable(True)
+    model.set("damage", np.zeros(E))
+    reset(model, v0)
+    model.surrogate_fallbacks(reset=True)
+    model.solve_step(steps, akx.MIXED)
+    u_sur = model.get_displacement()
+    fb = model.surrogate_fallbacks()
+    rel_u = float(np.linalg.norm(u_sur - u_plain) / np.linalg.norm(u_plain))
+    log("after %d steps: stress rel error %.3e (p99 per point %.3e), %.2f%% points inside the "
+        "envelope, displacement rel error %.3e, fallbacks %d" %
+        (steps, rel_stress, p99, 100 * inside.mean(), rel_u, fb))
+    flop_qp = 2 * (7 * 128 + 3 * 128 * 128 + 128 * 7)
+    tflops = flop_qp * E * Q / (res["surrogate"]["element_ms"] * 1e-3) / 1e12
+    out = {
+        "metric": "element-QP updates/s per explicit step, surrogate vs plain neo-Hookean+damage",
+        "value": res["surrogate"]["qp_per_s"], "unit": "QP-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["surrogate"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA network (f32 accumulation) / f32 element math / f64 nodal state",
+        "data": "synthetic",
+        "config": {"workload": "plate-surrogate: notched plate 0.1 x 0.1 x 0.02 m, %d "
+                               "tetrahedron_4, %d nodes, neo-Hookean + Marigo damage E=1 GPa "
+                               "nu=0.3 rho=1200 Sd=5 MPa, Yd ~ 1 MPa Weibull(m=5), tension at "
+                               "20 m/s; 7-128x4-7 SiLU surrogate trained on 10M GPU-generated "
+                               "samples" % (E, N),
+                   "elements": E, "nodes": N, "qp_per_element": Q, "parallelism": "1 GPU"},
+        "plain": res["plain"], "surrogate": res["surrogate"],
+        "surrogate_over_plain": res["surrogate"]["qp_per_s"] / res["plain"]["qp_per_s"],
+        "training": {"samples": int(model.query("surrogate_samples")), "generate_s": t_gen,
+                     "iterations": iters, "batch": 8192, "train_s": t_train, "final_loss": loss},
+        "accuracy": {"steps": steps, "rel_stress_error": rel_stress,
+                     "p99_point_rel_stress_error": p99, "inside_envelope": float(inside.mean()),
+                     "rel_u_error_vs_plain": rel_u, "fallback_points": int(fb)},
+        "roofline": {"bound": "mfma", "kernel": "akx::k_element_surrogate(akx::ElemArgs)",
+                     "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "traffic": None,
+                     "algorithmic_flops_per_qp": flop_qp,
+                     "avg_ms": res["surrogate"]["element_ms"]},
+        "cpu_baseline": None,
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="cube-tet4",
-                    choices=["cube-tet4", "hex8-neohookean", "ensemble"])
+                    choices=["cube-tet4", "hex8-neohookean", "ensemble", "plate-surrogate"])
+    ap.add_argument("--train-iters", type=int, default=6000,
+                    help="Adam iterations of the surrogate (plate-surrogate)")
     ap.add_argument("--n", type=int, default=100, help="hexes per cube edge")
     ap.add_argument("--error-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -493,6 +650,8 @@ def main():
                     help="element_block,node_block,xcd_remap,mixed_recompute,slot_layout,node_stage "
                          "(skips the autotuner for the mixed path)")
     ap.add_argument("--launch-fp64", default=None, help="same, for the FP64 path")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the launch shapes even where profiled ones are committed")
     args = ap.parse_args()
     if args.copy_only:
         print(json.dumps({"copy_bytes": 1 << 30, "copy_gbs": copy_bandwidth(1 << 30, 3)}))
@@ -504,7 +663,7 @@ def main():
 
     from akantu_amd import akx
     fn = {"cube-tet4": bench_cube, "hex8-neohookean": bench_hex8,
-          "ensemble": bench_ensemble}[args.workload]
+          "ensemble": bench_ensemble, "plate-surrogate": bench_plate_surrogate}[args.workload]
     print(json.dumps(fn(args, akx)), flush=True)
 
 
