@@ -70,6 +70,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->csr2_off); dfree(c->csr2_slot); dfree(c->slot2_pos);
   dfree(c->pnode_off); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
   dfree(c->lconn); dfree(c->lrank);
+  dfree(c->pcsr_off); dfree(c->pentry);
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
@@ -275,6 +276,19 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
     std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
 }
 
+// Mixed path with per-patch partial sums (slot layout 3): the element kernel
+// reduces each node's contributions inside its patch, the node kernel sums
+// FP64 partials.  Needs the LDS patches and a single launch group.
+bool uses_partials(const akx_ctx* c, int precision) {
+  return precision == AKX_MIXED && c->node_ordered == 3 && c->patch_ok && c->groups.size() == 1 &&
+         !c->groups[0].filter && !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE);
+}
+// arithmetic of the node kernel's slots: FP64 for the FP64 path and for
+// per-patch partials, FP32 otherwise
+int node_precision(const akx_ctx* c, int precision) {
+  return uses_partials(c, precision) ? AKX_FP64 : precision;
+}
+
 ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   ElemArgs A;
   A.count = g.count;
@@ -293,7 +307,8 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.recompute = c->mixed_recompute;
   A.X = c->X;
   A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
-  A.patch = (precision == AKX_MIXED && c->node_ordered == 2 && c->patch_ok && !g.filter) ? 1 : 0;
+  A.patch = (precision == AKX_MIXED && c->node_ordered >= 2 && c->patch_ok && !g.filter) ? 1 : 0;
+  A.partials = (A.patch && c->node_ordered == 3) ? 1 : 0;
   A.patch_max_nodes = c->patch_max_nodes;
   A.patch_lds = c->patch_lds;
   A.patch_blocks = c->patch_blocks;
@@ -316,6 +331,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
     A.sur_sW = float(c->sur.sW);
     A.sur_fallbacks = c->sur.fallbacks;
     A.patch = 0;
+    A.partials = 0;
   }
   return A;
 }
@@ -340,6 +356,13 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.xcd_remap = c->xcd_remap;
   A.node_ordered = c->node_ordered;
   A.stage_lds = 0;
+  if (uses_partials(c, precision)) {
+    // one FP64 partial sum per (patch, node): a short gather per node
+    A.csr_off = c->pcsr_off;
+    A.csr_slot = c->pentry;
+    A.node_ordered = 0;
+    return A;
+  }
   if (c->node_ordered && c->node_stage) {
     // static LDS of k_node is at most 4 * 256 * 3 * 8 + 256 * 3 bytes (< 26 KiB)
     const size_t bytes = size_t(c->span_max[c->node_block / 64 - 1]) * c->d *
@@ -439,6 +462,18 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
   HIP_TRY(c, hipMemcpy(c->run_base, rbase.data(), rbase.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->lconn, lconn.data(), lconn.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->lrank, lrank.data(), lrank.size(), hipMemcpyHostToDevice));
+  // node -> patch-node entries (ascending patch), for the per-patch partial sums
+  {
+    std::vector<uint32_t> off(size_t(c->N) + 1, 0), ent(pids.size());
+    for (uint32_t v : pids) off[v + 1]++;
+    for (int64_t k = 0; k < c->N; ++k) off[k + 1] += off[k];
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (size_t j = 0; j < pids.size(); ++j) ent[fill[pids[j]]++] = uint32_t(j);
+    HIP_TRY(c, dalloc(&c->pcsr_off, off.size()));
+    HIP_TRY(c, dalloc(&c->pentry, ent.size()));
+    HIP_TRY(c, hipMemcpy(c->pcsr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->pentry, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+  }
   c->patch_max_nodes = maxm;
   c->patch_nodes_total = int64_t(pids.size());
   c->patch_lds = int(lds);
@@ -976,7 +1011,7 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
   // predictor of step 1
   {
     NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0), precision);
-    HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
+    HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
   }
   for (int s = 1; s <= nsteps; ++s) {
     const bool last = (s == nsteps);
@@ -988,7 +1023,7 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
     if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
     else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
     NodeArgs A = node_args(c, mode, precision);
-    HIP_TRY(c, launch_node(d, precision, c->has_fext, c->stream, A, c->node_block));
+    HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
   }
   if (timing) {
@@ -1099,7 +1134,7 @@ int akx_assemble_internal_forces(akx_ctx* c, int precision) {
   rc = run_elements(c, precision);
   if (rc) return rc;
   NodeArgs A = node_args(c, NODE_GATHER | NODE_STORE_FINT, precision);
-  HIP_TRY(c, launch_node(c->d, precision, c->has_fext, c->stream, A, c->node_block));
+  HIP_TRY(c, launch_node(c->d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
   c->fint_valid = true;
   return AKX_OK;
 }
@@ -1381,14 +1416,19 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   struct Shape { int eb, nb, x, rcmp, lay, stage; };
   std::vector<Shape> cand;
   const int blocks[3] = {64, 128, 256};
-  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 3 : 2;
+  // layouts 2 and 3 (LDS patches) exist on the mixed path only; staging the
+  // node kernel's slot range applies to node-ordered slots (layouts 1, 2)
+  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 4 : 2;
   for (int lay = 0; lay < nlay; ++lay)
-    for (int stage = 0; stage < (lay ? 2 : 1); ++stage)
-      for (int rcmp = (lay == 2 ? 1 : 0); rcmp < (precision == AKX_MIXED ? 2 : 1); ++rcmp)
+    for (int stage = 0; stage < ((lay == 1 || lay == 2) ? 2 : 1); ++stage)
+      for (int rcmp = (lay >= 2 ? 1 : 0); rcmp < (precision == AKX_MIXED ? 2 : 1); ++rcmp)
         for (int x = 0; x < 2; ++x)
           for (int eb : blocks) {
-            if (lay == 2 && eb != 256) continue;  // the patch kernel has its own block shape
-            for (int nb : blocks) cand.push_back(Shape{eb, nb, x, rcmp, lay, stage});
+            if (lay >= 2 && eb != 256) continue;  // the patch kernel has its own block shape
+            for (int nb : blocks) {
+              if (stage && nb != 256) continue;  // LDS-bound occupancy below 256 threads
+              cand.push_back(Shape{eb, nb, x, rcmp, lay, stage});
+            }
           }
   for (const Shape& sh : cand) {
           const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;
@@ -1430,7 +1470,7 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
     return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
-  if (slot_layout > 2) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1 or 2");
+  if (slot_layout > 3) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1, 2 or 3");
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;

@@ -47,6 +47,7 @@ struct ElemArgs {
   const uint32_t* slot_pos;  // [a][E] CSR position of (e, a); nullptr = element-ordered slots
   // LDS-patch variant of the mixed kernel (k_element_patch)
   int patch;                 // 1: use it
+  int partials;              // 1: write one FP64 sum per (patch, node) instead of the runs
   int patch_max_nodes;       // most distinct nodes of one patch
   int patch_lds;             // dynamic LDS bytes per block
   int64_t patch_blocks;      // number of patches
@@ -245,6 +246,8 @@ struct akx_ctx {
   uint32_t *pnode_off = nullptr, *pnode_ids = nullptr, *run_start = nullptr;
   uint16_t *run_base = nullptr, *lconn = nullptr;
   uint8_t* lrank = nullptr;
+  // node -> entries of the per-patch partial sums (slot layout 3)
+  uint32_t *pcsr_off = nullptr, *pentry = nullptr;
   akx::ElemTablesF tabf;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal
   bool reorder = true;

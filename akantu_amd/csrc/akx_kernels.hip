@@ -550,7 +550,10 @@ __global__ void __launch_bounds__(256) k_element_surrogate(ElemArgs A) {
 //     2. computes every element from LDS through 16-bit patch-local node ids
 //        (same arithmetic as k_element_mixed<..., true>);
 //     3. stages the n*d forces of each element in LDS at the element's place
-//        in the patch's per-node runs and writes the runs out contiguously.
+//        in the patch's per-node runs and writes the runs out contiguously,
+//        or (A.partials) sums each node's run in FP64 and writes one partial
+//        per patch node, which the node kernel gathers through a short
+//        node -> patch-entry CSR.
 //   Valid with the mixed-path CSR, which lists each node's slots by element
 //   id: the patch's contributions to one node are then one contiguous run.
 // ===========================================================================
@@ -572,6 +575,17 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   float* s_f = reinterpret_cast<float*>(
       smem + ((size_t(A.patch_max_nodes) * (2 * d * 8 + 4) + size_t(A.patch_max_nodes + 1) * 2 + 15) &
               ~size_t(15)));
+  // the element's own indices do not depend on the staged nodes: issue their
+  // loads before the staging so that both latency chains overlap
+  const int t = threadIdx.x;
+  const int64_t e = e0 + min(t, ne - 1);
+  int ln[n], rk[n];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    ln[a] = A.lconn[a * A.E + e];
+    rk[a] = A.lrank[a * A.E + e];
+  }
+  const int mat = A.elem_mat ? A.elem_mat[e] : 0;
   for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
     const int64_t node = A.pnode_ids[k0 + k];
 #pragma unroll
@@ -579,18 +593,13 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
       s_u[k * d + i] = A.u[node * d + i];
       s_x[k * d + i] = A.X[node * d + i];
     }
-    s_rs[k] = A.run_start[k0 + k];
+    if (!A.partials) s_rs[k] = A.run_start[k0 + k];
     s_rb[k] = A.run_base[k0 + b + k];
   }
   if (threadIdx.x == 0) s_rb[m_nodes] = A.run_base[k0 + b + m_nodes];
   __syncthreads();
-  const int t = threadIdx.x;
   if (t < ne) {
-    const int64_t e = e0 + t;
-    const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
-    int ln[n];
-#pragma unroll
-    for (int a = 0; a < n; ++a) ln[a] = A.lconn[a * A.E + e];
+    const MatParams m = A.mats[mat];
     float du[dn], dX[dn];
 #pragma unroll
     for (int i = 0; i < d; ++i) { du[i] = 0.f; dX[i] = 0.f; }
@@ -605,12 +614,27 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
     mixed_element_forces<ET, KIND, true>(A, e, m, du, dX, fe);
 #pragma unroll
     for (int a = 0; a < n; ++a) {
-      const int q = s_rb[ln[a]] + A.lrank[a * A.E + e];
+      const int q = s_rb[ln[a]] + rk[a];
 #pragma unroll
       for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
     }
   }
   __syncthreads();
+  if (A.partials) {
+    // one FP64 sum per patch node, its run summed in element order
+    double* __restrict__ part = static_cast<double*>(A.slots);
+    for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
+      double f[d];
+#pragma unroll
+      for (int i = 0; i < d; ++i) f[i] = 0.;
+      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q)
+#pragma unroll
+        for (int i = 0; i < d; ++i) f[i] += double(s_f[q * d + i]);
+#pragma unroll
+      for (int i = 0; i < d; ++i) part[(int64_t(k0) + k) * d + i] = f[i];
+    }
+    return;
+  }
   // contiguous runs: output q belongs to the node k with rb[k] <= q < rb[k+1]
   float* __restrict__ out = static_cast<float*>(A.slots);
   const int total = ne * n;
@@ -737,10 +761,23 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
 #pragma unroll
           for (int i = 0; i < D; ++i) add(i, x[j][i]);
       }
-      for (; p < p1; ++p) {
-        const int64_t s = A.csr_slot[p];
+      // tail in groups of 4 with all loads of a group issued first (the short
+      // per-patch partial lists end up here entirely)
+      for (; p < p1; p += 4) {
+        const int cnt4 = int(min<uint32_t>(4u, p1 - p));
+        uint32_t s[4];
 #pragma unroll
-        for (int i = 0; i < D; ++i) add(i, slots[s * D + i]);
+        for (int j = 0; j < 4; ++j) s[j] = A.csr_slot[j < cnt4 ? p + j : p];
+        TS x[4][D];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < cnt4)
+#pragma unroll
+            for (int i = 0; i < D; ++i) add(i, x[j][i]);
       }
       const double M = A.mass[node];
       const double dcoef = 0.5 * dt;            // beta dt

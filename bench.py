@@ -51,7 +51,8 @@ def log(msg):
 
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
-                      patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0):
+                      patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0,
+                      partials=False):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -60,7 +61,17 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     patch_nodes: LDS-patch element kernel, which reads 16-bit local
     connectivity + 8-bit run ranks per (e, a) and, per patch node, its id,
     run start and run offset (4 + 4 + 2 bytes); dt_per_node: one time step
-    per node (ensembles); state_bytes: per-point history read + written."""
+    per node (ensembles); state_bytes: per-point history read + written;
+    partials: the patch kernel writes one FP64 sum per patch node (no run
+    start needed) and the node kernel gathers those through a node -> entry
+    CSR instead of reading E*n slots."""
+    if partials:
+        elem = (E * n * (2 + 1) + patch_nodes * (4 + 2) + patch_blocks * (4 + 2)
+                + 2 * N * d * 8 + patch_nodes * d * 8 + state_bytes)
+        node = ((N + 1) * 4 + patch_nodes * 4 + patch_nodes * d * 8
+                + 6 * N * d * 8 + N * 8 + N * d
+                + (N * d * 8 if has_fext else 0) + (N * 8 if dt_per_node else 0))
+        return elem, node
     if patch_nodes is not None:
         elem = (E * n * (2 + 1)                            # local connectivity, run ranks
                 + patch_nodes * (4 + 4 + 2) + patch_blocks * (4 + 2)  # patch node lists
@@ -218,17 +229,22 @@ def measure(model, shapes, args, akx, v0, QP, graph=False):
 
 def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
     """Roofline of the slower of the two mixed-path kernels."""
-    patch = shapes["mixed"]["slot_layout"] == 2 and model.query("patch_available") == 1
+    lay = shapes["mixed"]["slot_layout"]
+    patch = lay >= 2 and model.query("patch_available") == 1
+    partials = patch and lay == 3
     eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
                                tables=not (shapes["mixed"]["mixed_recompute"] or patch),
-                               node_ordered=bool(shapes["mixed"]["slot_layout"]),
+                               node_ordered=bool(lay),
                                patch_nodes=model.query("patch_nodes_total") if patch else None,
-                               patch_blocks=model.query("patch_blocks") if patch else 0, **kw)
+                               patch_blocks=model.query("patch_blocks") if patch else 0,
+                               partials=partials, **kw)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms
     else:
-        kname, kbytes, kms = node_kernel, nb, nd_ms
+        # per-patch partials are FP64: the node kernel is its double-slot instance
+        kname = node_kernel.replace("float", "double") if partials else node_kernel
+        kbytes, kms = nb, nd_ms
     achieved = kbytes / (kms * 1e-3) / 1e9
     step_gbs = (eb + nb) / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
     return kname, kbytes, kms, achieved, eb, nb, step_gbs

@@ -116,10 +116,10 @@ def test_cube_tet4_fp64_bitwise_and_mixed():
     assert err <= 1e-5, err
 
 
-@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 1), (1, 2)])
+@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 1), (1, 2), (1, 3)])
 def test_cube_tet4_mixed_1000_steps(recompute, layout):
     """The 1000-step bar of the mixed path on tetrahedron_4 (8^3 cube), with
-    the gather, node-ordered and LDS-patch element kernels."""
+    the gather, node-ordered, LDS-patch and per-patch partial-sum kernels."""
     p = gh.cube_pair(8)
     p.gpu.set_launch(mixed_recompute=recompute, slot_layout=layout)
     p.step(1000, akx.MIXED)
@@ -256,7 +256,10 @@ def test_launch_shapes_do_not_change_results():
             (0, ((256, 256, 1, 0, 0), (64, 128, 0, 1, 0), (128, 64, 1, 1, 1),
                  (64, 256, 1, 1, 1))),
             (1, ((256, 256, 1, 0, 0), (64, 128, 0, 1, 1), (128, 64, 1, 1, 0),
-                 (256, 128, 1, 2, 1), (256, 64, 0, 2, 0), (256, 256, 1, 2, 1)))):
+                 (256, 128, 1, 2, 1), (256, 64, 0, 2, 0), (256, 256, 1, 2, 1))),
+            # per-patch FP64 partial sums group the additions differently from
+            # the compensated slot sums: identical among themselves
+            (1, ((256, 256, 1, 3, 0), (256, 64, 0, 3, 0), (256, 128, 1, 3, 1)))):
         ref = None
         for eb, nb, x, lay, stage in shapes:
             p = gh.cube_pair(6)
