@@ -447,8 +447,10 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
       }
     poff[b + 1] = poff[b] + uint32_t(m);
   }
+  // node state (u | X, FP64), run starts, run bases, then the forces with
+  // 4 floats per 3D contribution (2 in 2D) -- the layout of k_element_patch
   const size_t lds = ((size_t(maxm) * (2 * d * 8 + 4) + size_t(maxm + 1) * 2 + 15) & ~size_t(15)) +
-                     size_t(PATCH) * n * d * 4;
+                     size_t(PATCH) * n * (d == 3 ? 4 : 2) * 4;
   if (lds > 64 * 1024) return AKX_OK;
   HIP_TRY(c, dalloc(&c->pnode_off, poff.size()));
   HIP_TRY(c, dalloc(&c->pnode_ids, pids.size()));

@@ -567,10 +567,12 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   if (e0 >= A.count) return;  // uniform over the block
   const int ne = int(min<int64_t>(blockDim.x, A.count - e0));
   const uint32_t k0 = A.pnode_off[b], m_nodes = A.pnode_off[b + 1] - k0;
-  // LDS: u (FP64), X (FP64), run start (u32), run base (u16, m+1), forces (f32)
-  double* s_u = reinterpret_cast<double*>(smem);
-  double* s_x = s_u + size_t(A.patch_max_nodes) * d;
-  uint32_t* s_rs = reinterpret_cast<uint32_t*>(s_x + size_t(A.patch_max_nodes) * d);
+  // LDS: per node (u | X) in FP64 (2d doubles, read as 16-byte vectors), run
+  // start (u32), run base (u16, m+1), forces (FP32, FW per contribution: a
+  // 3D force padded to 4 floats is one 16-byte LDS access)
+  constexpr int FW = d == 3 ? 4 : 2;
+  double* s_ux = reinterpret_cast<double*>(smem);
+  uint32_t* s_rs = reinterpret_cast<uint32_t*>(s_ux + size_t(A.patch_max_nodes) * 2 * d);
   uint16_t* s_rb = reinterpret_cast<uint16_t*>(s_rs + A.patch_max_nodes);
   float* s_f = reinterpret_cast<float*>(
       smem + ((size_t(A.patch_max_nodes) * (2 * d * 8 + 4) + size_t(A.patch_max_nodes + 1) * 2 + 15) &
@@ -588,11 +590,15 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   const int mat = A.elem_mat ? A.elem_mat[e] : 0;
   for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
     const int64_t node = A.pnode_ids[k0 + k];
+    double v[2 * d];
 #pragma unroll
     for (int i = 0; i < d; ++i) {
-      s_u[k * d + i] = A.u[node * d + i];
-      s_x[k * d + i] = A.X[node * d + i];
+      v[i] = A.u[node * d + i];
+      v[d + i] = A.X[node * d + i];
     }
+    double2* dst = reinterpret_cast<double2*>(s_ux + size_t(k) * 2 * d);
+#pragma unroll
+    for (int j = 0; j < d; ++j) dst[j] = make_double2(v[2 * j], v[2 * j + 1]);
     if (!A.partials) s_rs[k] = A.run_start[k0 + k];
     s_rb[k] = A.run_base[k0 + b + k];
   }
@@ -600,23 +606,41 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   __syncthreads();
   if (t < ne) {
     const MatParams m = A.mats[mat];
+    auto node_ux = [&](int k, double* v) {
+      const double2* src = reinterpret_cast<const double2*>(s_ux + size_t(k) * 2 * d);
+#pragma unroll
+      for (int j = 0; j < d; ++j) {
+        const double2 w = src[j];
+        v[2 * j] = w.x;
+        v[2 * j + 1] = w.y;
+      }
+    };
+    double v0[2 * d];
+    node_ux(ln[0], v0);
     float du[dn], dX[dn];
 #pragma unroll
     for (int i = 0; i < d; ++i) { du[i] = 0.f; dX[i] = 0.f; }
 #pragma unroll
-    for (int a = 1; a < n; ++a)
+    for (int a = 1; a < n; ++a) {
+      double v[2 * d];
+      node_ux(ln[a], v);
 #pragma unroll
       for (int i = 0; i < d; ++i) {
-        du[i + a * d] = float(s_u[ln[a] * d + i] - s_u[ln[0] * d + i]);
-        dX[i + a * d] = float(s_x[ln[a] * d + i] - s_x[ln[0] * d + i]);
+        du[i + a * d] = float(v[i] - v0[i]);
+        dX[i + a * d] = float(v[d + i] - v0[d + i]);
       }
+    }
     float fe[dn];
     mixed_element_forces<ET, KIND, true>(A, e, m, du, dX, fe);
 #pragma unroll
     for (int a = 0; a < n; ++a) {
       const int q = s_rb[ln[a]] + rk[a];
-#pragma unroll
-      for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
+      if constexpr (FW == 4) {
+        reinterpret_cast<float4*>(s_f)[q] =
+            make_float4(fe[a * d], fe[a * d + 1], fe[a * d + 2], 0.f);
+      } else {
+        reinterpret_cast<float2*>(s_f)[q] = make_float2(fe[a * d], fe[a * d + 1]);
+      }
     }
   }
   __syncthreads();
@@ -627,9 +651,18 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
       double f[d];
 #pragma unroll
       for (int i = 0; i < d; ++i) f[i] = 0.;
-      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q)
+      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q) {
+        float g[FW];
+        if constexpr (FW == 4) {
+          const float4 w = reinterpret_cast<const float4*>(s_f)[q];
+          g[0] = w.x; g[1] = w.y; g[2] = w.z; g[3] = w.w;
+        } else {
+          const float2 w = reinterpret_cast<const float2*>(s_f)[q];
+          g[0] = w.x; g[1] = w.y;
+        }
 #pragma unroll
-        for (int i = 0; i < d; ++i) f[i] += double(s_f[q * d + i]);
+        for (int i = 0; i < d; ++i) f[i] += double(g[i]);
+      }
 #pragma unroll
       for (int i = 0; i < d; ++i) part[(int64_t(k0) + k) * d + i] = f[i];
     }
@@ -646,7 +679,7 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
     }
     const int64_t p = int64_t(s_rs[lo]) + (q - s_rb[lo]);
 #pragma unroll
-    for (int i = 0; i < d; ++i) out[p * d + i] = s_f[q * d + i];
+    for (int i = 0; i < d; ++i) out[p * d + i] = s_f[q * FW + i];
   }
 }
 
