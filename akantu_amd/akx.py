@@ -286,6 +286,21 @@ class SolidMechanicsModel:
         self.ctx.check(lib.akx_surrogate_fallbacks(self.ctx.h, ctypes.byref(v), int(bool(reset))))
         return v.value
 
+    # -- output (SolidMechanicsModel::setBaseName / addDumpField / dump) --
+    def set_base_name(self, name, directory="paraview"):
+        from .dumper import Dumper
+        self._dumper = Dumper(self, name, directory)
+
+    def add_dump_field(self, name):
+        if getattr(self, "_dumper", None) is None:
+            self.set_base_name("model")
+        self._dumper.add_dump_field(name)
+
+    def dump(self, time=None):
+        if getattr(self, "_dumper", None) is None:
+            self.set_base_name("model")
+        return self._dumper.dump(time)
+
     # -- fields --
     def _shape(self, name):
         m, d = self.mesh, self.dim

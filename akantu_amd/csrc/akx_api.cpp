@@ -68,7 +68,7 @@ void free_derived(akx_ctx* c) {
   free_csr(c);
   dfree(c->slot_pos);
   dfree(c->csr2_off); dfree(c->csr2_slot); dfree(c->slot2_pos);
-  dfree(c->pnode_off); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
+  dfree(c->pnode_cnt); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
   dfree(c->lconn); dfree(c->lrank);
   dfree(c->pcsr_off); dfree(c->pentry);
   c->patch_ok = false;
@@ -281,7 +281,10 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
 // FP64 partials.  Needs the LDS patches and a single launch group.
 bool uses_partials(const akx_ctx* c, int precision) {
   return precision == AKX_MIXED && c->node_ordered == 3 && c->patch_ok && c->groups.size() == 1 &&
-         !c->groups[0].filter && !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE);
+         !c->groups[0].filter &&
+         !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
+         // the partials (padded stride) live in the slot buffer
+         c->patch_blocks * int64_t(c->patch_max_nodes) <= c->E * int64_t(c->n);
 }
 // arithmetic of the node kernel's slots: FP64 for the FP64 path and for
 // per-patch partials, FP32 otherwise
@@ -308,11 +311,11 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.X = c->X;
   A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
   A.patch = (precision == AKX_MIXED && c->node_ordered >= 2 && c->patch_ok && !g.filter) ? 1 : 0;
-  A.partials = (A.patch && c->node_ordered == 3) ? 1 : 0;
+  A.partials = (A.patch && uses_partials(c, precision)) ? 1 : 0;
   A.patch_max_nodes = c->patch_max_nodes;
   A.patch_lds = c->patch_lds;
   A.patch_blocks = c->patch_blocks;
-  A.pnode_off = c->pnode_off;
+  A.pnode_cnt = c->pnode_cnt;
   A.pnode_ids = c->pnode_ids;
   A.run_start = c->run_start;
   A.run_base = c->run_base;
@@ -356,7 +359,9 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.xcd_remap = c->xcd_remap;
   A.node_ordered = c->node_ordered;
   A.stage_lds = 0;
+  A.partials = 0;
   if (uses_partials(c, precision)) {
+    A.partials = 1;
     // one FP64 partial sum per (patch, node): a short gather per node
     A.csr_off = c->pcsr_off;
     A.csr_slot = c->pentry;
@@ -449,28 +454,49 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
   }
   // node state (u | X, FP64), run starts, run bases, then the forces with
   // 4 floats per 3D contribution (2 in 2D) -- the layout of k_element_patch
-  const size_t lds = ((size_t(maxm) * (2 * d * 8 + 4) + size_t(maxm + 1) * 2 + 15) & ~size_t(15)) +
-                     size_t(PATCH) * n * (d == 3 ? 4 : 2) * 4;
+  // LDS of k_element_patch: the node lists (patch_force_offset) and the
+  // forces, d floats per contribution
+  const size_t lds = patch_force_offset(maxm, d) + size_t(PATCH) * n * d * 4;
   if (lds > 64 * 1024) return AKX_OK;
-  HIP_TRY(c, dalloc(&c->pnode_off, poff.size()));
-  HIP_TRY(c, dalloc(&c->pnode_ids, pids.size()));
-  HIP_TRY(c, dalloc(&c->run_start, rstart.size()));
-  HIP_TRY(c, dalloc(&c->run_base, rbase.size()));
+  // per-patch lists padded to the stride S = maxm; padding repeats the
+  // patch's first node (a valid address) and its run data
+  const int S = maxm;
+  if (uint64_t(nb) * uint64_t(S + 1) >= (uint64_t(1) << 32)) return AKX_OK;
+  std::vector<uint32_t> cnt(nb), pids_p(size_t(nb) * S), rstart_p(size_t(nb) * S);
+  std::vector<uint16_t> rbase_p(size_t(nb) * (S + 1));
+  for (int64_t b = 0; b < nb; ++b) {
+    const uint32_t m = poff[b + 1] - poff[b];
+    cnt[b] = m;
+    for (int k = 0; k < S; ++k) {
+      const uint32_t src = poff[b] + (uint32_t(k) < m ? uint32_t(k) : 0u);
+      pids_p[size_t(b) * S + k] = pids[src];
+      rstart_p[size_t(b) * S + k] = rstart[src];
+    }
+    for (int k = 0; k <= S; ++k)
+      rbase_p[size_t(b) * (S + 1) + k] = rbase[poff[b] + b + std::min<uint32_t>(uint32_t(k), m)];
+  }
+  HIP_TRY(c, dalloc(&c->pnode_cnt, cnt.size()));
+  HIP_TRY(c, dalloc(&c->pnode_ids, pids_p.size()));
+  HIP_TRY(c, dalloc(&c->run_start, rstart_p.size()));
+  HIP_TRY(c, dalloc(&c->run_base, rbase_p.size()));
   HIP_TRY(c, dalloc(&c->lconn, lconn.size()));
   HIP_TRY(c, dalloc(&c->lrank, lrank.size()));
-  HIP_TRY(c, hipMemcpy(c->pnode_off, poff.data(), poff.size() * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->pnode_ids, pids.data(), pids.size() * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->run_start, rstart.data(), rstart.size() * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->run_base, rbase.data(), rbase.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->pnode_cnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->pnode_ids, pids_p.data(), pids_p.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->run_start, rstart_p.data(), rstart_p.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->run_base, rbase_p.data(), rbase_p.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->lconn, lconn.data(), lconn.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->lrank, lrank.data(), lrank.size(), hipMemcpyHostToDevice));
-  // node -> patch-node entries (ascending patch), for the per-patch partial sums
+  // node -> patch-node entries b * S + k (ascending patch), for the per-patch
+  // partial sums
   {
     std::vector<uint32_t> off(size_t(c->N) + 1, 0), ent(pids.size());
     for (uint32_t v : pids) off[v + 1]++;
     for (int64_t k = 0; k < c->N; ++k) off[k + 1] += off[k];
     std::vector<uint32_t> fill(off.begin(), off.end() - 1);
-    for (size_t j = 0; j < pids.size(); ++j) ent[fill[pids[j]]++] = uint32_t(j);
+    for (int64_t b = 0; b < nb; ++b)
+      for (uint32_t k = 0; k < cnt[b]; ++k)
+        ent[fill[pids[poff[b] + k]]++] = uint32_t(b * S + k);
     HIP_TRY(c, dalloc(&c->pcsr_off, off.size()));
     HIP_TRY(c, dalloc(&c->pentry, ent.size()));
     HIP_TRY(c, hipMemcpy(c->pcsr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));

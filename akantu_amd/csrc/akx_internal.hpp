@@ -48,13 +48,13 @@ struct ElemArgs {
   // LDS-patch variant of the mixed kernel (k_element_patch)
   int patch;                 // 1: use it
   int partials;              // 1: write one FP64 sum per (patch, node) instead of the runs
-  int patch_max_nodes;       // most distinct nodes of one patch
+  int patch_max_nodes;       // most distinct nodes of one patch = stride S of the lists
   int patch_lds;             // dynamic LDS bytes per block
   int64_t patch_blocks;      // number of patches
-  const uint32_t* pnode_off; // [patch] -> first entry in pnode_ids / run_start
-  const uint32_t* pnode_ids; // patch nodes (internal ids, ascending)
-  const uint32_t* run_start; // first mixed-CSR position of the patch's run for the node
-  const uint16_t* run_base;  // [pnode_off[b] + b + k]: prefix of run lengths (m+1 per patch)
+  const uint32_t* pnode_cnt; // [patch] distinct nodes m of the patch
+  const uint32_t* pnode_ids; // [b * S + k] patch nodes (internal ids, ascending; padded)
+  const uint32_t* run_start; // [b * S + k] first mixed-CSR position of the node's run
+  const uint16_t* run_base;  // [b * (S + 1) + k] prefix of run lengths (padded)
   const uint16_t* lconn;     // [a][E] patch-local node index
   const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
   ElemTablesF tabf;
@@ -86,6 +86,7 @@ struct NodeArgs {
   int xcd_remap;
   int node_ordered;       // slots stored in CSR order (contiguous per node)
   int stage_lds;          // > 0: bytes of LDS for the block's slot range (node-ordered only)
+  int partials;           // slots are FP64 per-patch partial sums (csr_off/csr_slot: node -> entries)
 };
 
 enum NodeMode {
@@ -96,6 +97,12 @@ enum NodeMode {
   NODE_STORE_INC = 16,  // increment = u - previous after the corrector
   NODE_GATHER = 32      // gather forces only (with NODE_STORE_FINT)
 };
+
+// Byte offset of the forces in the LDS of k_element_patch with list stride S:
+// S nodes x 2d floats, S run starts (u32), S + 1 run bases (u16), 16-aligned.
+inline __host__ __device__ size_t patch_force_offset(int S, int d) {
+  return (size_t(S) * (2 * d * 4 + 4) + size_t(S + 1) * 2 + 15) & ~size_t(15);
+}
 
 // Launchers (akx_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_precompute(int et, hipStream_t s, int64_t E, int64_t N, const double* X,
@@ -243,7 +250,7 @@ struct akx_ctx {
   bool patch_ok = false;
   int patch_max_nodes = 0, patch_lds = 0;
   int64_t patch_blocks = 0, patch_nodes_total = 0;
-  uint32_t *pnode_off = nullptr, *pnode_ids = nullptr, *run_start = nullptr;
+  uint32_t *pnode_cnt = nullptr, *pnode_ids = nullptr, *run_start = nullptr;
   uint16_t *run_base = nullptr, *lconn = nullptr;
   uint8_t* lrank = nullptr;
   // node -> entries of the per-patch partial sums (slot layout 3)

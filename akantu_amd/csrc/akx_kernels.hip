@@ -566,19 +566,18 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   const int64_t e0 = b * int64_t(blockDim.x);
   if (e0 >= A.count) return;  // uniform over the block
   const int ne = int(min<int64_t>(blockDim.x, A.count - e0));
-  const uint32_t k0 = A.pnode_off[b], m_nodes = A.pnode_off[b + 1] - k0;
-  // LDS: per node (u | X) in FP64 (2d doubles, read as 16-byte vectors), run
-  // start (u32), run base (u16, m+1), forces (FP32, FW per contribution: a
-  // 3D force padded to 4 floats is one 16-byte LDS access)
-  constexpr int FW = d == 3 ? 4 : 2;
-  double* s_ux = reinterpret_cast<double*>(smem);
-  uint32_t* s_rs = reinterpret_cast<uint32_t*>(s_ux + size_t(A.patch_max_nodes) * 2 * d);
-  uint16_t* s_rb = reinterpret_cast<uint16_t*>(s_rs + A.patch_max_nodes);
-  float* s_f = reinterpret_cast<float*>(
-      smem + ((size_t(A.patch_max_nodes) * (2 * d * 8 + 4) + size_t(A.patch_max_nodes + 1) * 2 + 15) &
-              ~size_t(15)));
+  // per-patch lists are padded to a fixed stride S (the largest patch), so
+  // that no load waits for the patch's offset; padding repeats a valid node
+  const int S = A.patch_max_nodes;
+  const int64_t k0 = b * int64_t(S);
+  // LDS: per node (u - u_ref | X - X_ref) in FP32, run start (u32), run base
+  // (u16, S + 1), then the forces (FP32, d per contribution)
+  float* s_ux = reinterpret_cast<float*>(smem);
+  uint32_t* s_rs = reinterpret_cast<uint32_t*>(s_ux + size_t(S) * 2 * d);
+  uint16_t* s_rb = reinterpret_cast<uint16_t*>(s_rs + S);
+  float* s_f = reinterpret_cast<float*>(smem + patch_force_offset(S, d));
   // the element's own indices do not depend on the staged nodes: issue their
-  // loads before the staging so that both latency chains overlap
+  // loads first so that both latency chains overlap
   const int t = threadIdx.x;
   const int64_t e = e0 + min(t, ne - 1);
   int ln[n], rk[n];
@@ -588,46 +587,56 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
     rk[a] = A.lrank[a * A.E + e];
   }
   const int mat = A.elem_mat ? A.elem_mat[e] : 0;
-  for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
+  const uint32_t m_nodes = A.pnode_cnt[b];
+  // node values relative to the patch's first node, formed in FP64 and then
+  // rounded: the element differences below stay accurate in FP32
+  const int64_t ref = A.pnode_ids[k0];
+  double uref[d], xref[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    uref[i] = A.u[ref * d + i];
+    xref[i] = A.X[ref * d + i];
+  }
+  for (int k = threadIdx.x; k < S; k += blockDim.x) {
     const int64_t node = A.pnode_ids[k0 + k];
-    double v[2 * d];
+    float v[2 * d];
 #pragma unroll
     for (int i = 0; i < d; ++i) {
-      v[i] = A.u[node * d + i];
-      v[d + i] = A.X[node * d + i];
+      v[i] = float(A.u[node * d + i] - uref[i]);
+      v[d + i] = float(A.X[node * d + i] - xref[i]);
     }
-    double2* dst = reinterpret_cast<double2*>(s_ux + size_t(k) * 2 * d);
+    float2* dst = reinterpret_cast<float2*>(s_ux + size_t(k) * 2 * d);
 #pragma unroll
-    for (int j = 0; j < d; ++j) dst[j] = make_double2(v[2 * j], v[2 * j + 1]);
+    for (int j = 0; j < d; ++j) dst[j] = make_float2(v[2 * j], v[2 * j + 1]);
     if (!A.partials) s_rs[k] = A.run_start[k0 + k];
-    s_rb[k] = A.run_base[k0 + b + k];
+    s_rb[k] = A.run_base[b * int64_t(S + 1) + k];
   }
-  if (threadIdx.x == 0) s_rb[m_nodes] = A.run_base[k0 + b + m_nodes];
+  if (threadIdx.x == 0) s_rb[S] = A.run_base[b * int64_t(S + 1) + S];
   __syncthreads();
   if (t < ne) {
     const MatParams m = A.mats[mat];
-    auto node_ux = [&](int k, double* v) {
-      const double2* src = reinterpret_cast<const double2*>(s_ux + size_t(k) * 2 * d);
+    auto node_ux = [&](int k, float* v) {
+      const float2* src = reinterpret_cast<const float2*>(s_ux + size_t(k) * 2 * d);
 #pragma unroll
       for (int j = 0; j < d; ++j) {
-        const double2 w = src[j];
+        const float2 w = src[j];
         v[2 * j] = w.x;
         v[2 * j + 1] = w.y;
       }
     };
-    double v0[2 * d];
+    float v0[2 * d];
     node_ux(ln[0], v0);
     float du[dn], dX[dn];
 #pragma unroll
     for (int i = 0; i < d; ++i) { du[i] = 0.f; dX[i] = 0.f; }
 #pragma unroll
     for (int a = 1; a < n; ++a) {
-      double v[2 * d];
+      float v[2 * d];
       node_ux(ln[a], v);
 #pragma unroll
       for (int i = 0; i < d; ++i) {
-        du[i + a * d] = float(v[i] - v0[i]);
-        dX[i + a * d] = float(v[d + i] - v0[d + i]);
+        du[i + a * d] = v[i] - v0[i];
+        dX[i + a * d] = v[d + i] - v0[d + i];
       }
     }
     float fe[dn];
@@ -635,12 +644,8 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
 #pragma unroll
     for (int a = 0; a < n; ++a) {
       const int q = s_rb[ln[a]] + rk[a];
-      if constexpr (FW == 4) {
-        reinterpret_cast<float4*>(s_f)[q] =
-            make_float4(fe[a * d], fe[a * d + 1], fe[a * d + 2], 0.f);
-      } else {
-        reinterpret_cast<float2*>(s_f)[q] = make_float2(fe[a * d], fe[a * d + 1]);
-      }
+#pragma unroll
+      for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
     }
   }
   __syncthreads();
@@ -651,20 +656,11 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
       double f[d];
 #pragma unroll
       for (int i = 0; i < d; ++i) f[i] = 0.;
-      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q) {
-        float g[FW];
-        if constexpr (FW == 4) {
-          const float4 w = reinterpret_cast<const float4*>(s_f)[q];
-          g[0] = w.x; g[1] = w.y; g[2] = w.z; g[3] = w.w;
-        } else {
-          const float2 w = reinterpret_cast<const float2*>(s_f)[q];
-          g[0] = w.x; g[1] = w.y;
-        }
+      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q)
 #pragma unroll
-        for (int i = 0; i < d; ++i) f[i] += double(g[i]);
-      }
+        for (int i = 0; i < d; ++i) f[i] += double(s_f[q * d + i]);
 #pragma unroll
-      for (int i = 0; i < d; ++i) part[(int64_t(k0) + k) * d + i] = f[i];
+      for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = f[i];
     }
     return;
   }
@@ -679,7 +675,7 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
     }
     const int64_t p = int64_t(s_rs[lo]) + (q - s_rb[lo]);
 #pragma unroll
-    for (int i = 0; i < d; ++i) out[p * d + i] = s_f[q * FW + i];
+    for (int i = 0; i < d; ++i) out[p * d + i] = s_f[q * d + i];
   }
 }
 
@@ -697,8 +693,11 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
 // FP64 slots: plain sum in slot order (bit-identical to the sequential
 // scatter).  FP32 slots: FP64 Neumaier-compensated sum.
 // ===========================================================================
-template <int D, typename TS, bool HAS_FEXT>
+// SRC: what the node gathers -- 0 FP64 element slots, 1 FP32 element slots,
+// 2 FP64 per-patch partial sums.
+template <int D, int SRC, bool HAS_FEXT>
 __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
+  using TS = typename std::conditional<SRC == 1, float, double>::type;
   // The block's nodal state is staged through LDS: global loads and stores of
   // u, v, a, f_ext and the blocked flags are contiguous runs of doubles (fully
   // coalesced, each array written once), while each thread works on one node.
@@ -1290,11 +1289,16 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
                                 int block) {
   const unsigned g = grid_for(A.N, block);
   if (precision == 0) {
-    if (has_fext) k_node<D, double, true><<<g, block, A.stage_lds, s>>>(A);
-    else k_node<D, double, false><<<g, block, A.stage_lds, s>>>(A);
+    if (A.partials) {
+      if (has_fext) k_node<D, 2, true><<<g, block, A.stage_lds, s>>>(A);
+      else k_node<D, 2, false><<<g, block, A.stage_lds, s>>>(A);
+    } else {
+      if (has_fext) k_node<D, 0, true><<<g, block, A.stage_lds, s>>>(A);
+      else k_node<D, 0, false><<<g, block, A.stage_lds, s>>>(A);
+    }
   } else {
-    if (has_fext) k_node<D, float, true><<<g, block, A.stage_lds, s>>>(A);
-    else k_node<D, float, false><<<g, block, A.stage_lds, s>>>(A);
+    if (has_fext) k_node<D, 1, true><<<g, block, A.stage_lds, s>>>(A);
+    else k_node<D, 1, false><<<g, block, A.stage_lds, s>>>(A);
   }
   return hipGetLastError();
 }

@@ -70,9 +70,15 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// sigmoid through the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, about
+// 1 ulp each): the result is rounded to BF16 right after, so an IEEE division
+// (a ten-instruction sequence per value) would buy nothing
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.f / (1.f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.f + z * (1.f - s));
 }
 

@@ -1,0 +1,120 @@
+"""ParaView output of a model: one VTK XML unstructured-grid file (.vtu) per
+dump plus a .pvd collection that lists them with their time.
+
+Plays the part of the reference's dumper for this path
+(SolidMechanicsModel::setBaseName / addDumpField / dump,
+src/model/model.cc and src/io/dumper/), which writes nodal fields
+(displacement, velocity, ...) and per-element quadrature fields into
+`paraview/<base>_<step>.vtu`.  Nodal fields are point data; per-point fields
+of elements with several quadrature points are averaged per element as cell
+data.  Files are ASCII so that they can be read without any VTK library.
+"""
+from __future__ import annotations
+
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import mesh as akm
+
+# VTK cell types of the element types of the path
+VTK_CELL = {akm.TRIANGLE_3: 5, akm.QUADRANGLE_4: 9, akm.TETRAHEDRON_4: 10, akm.HEXAHEDRON_8: 12}
+
+NODAL = ("displacement", "velocity", "acceleration", "external_force", "internal_force",
+         "mass", "blocked_dofs")
+POINTWISE = ("stress", "grad_u", "damage")
+
+
+def _array_xml(parent, name, values, ncomp):
+    values = np.asarray(values)
+    kind = "UInt8" if values.dtype == np.uint8 or values.dtype == bool else "Float64"
+    da = ET.SubElement(parent, "DataArray", type=kind, Name=name,
+                       NumberOfComponents=str(ncomp), format="ascii")
+    flat = values.reshape(-1)
+    if kind == "UInt8":
+        da.text = " ".join(str(int(v)) for v in flat)
+    else:
+        da.text = " ".join(repr(float(v)) for v in flat)
+
+
+def write_vtu(path, mesh, point_data=None, cell_data=None):
+    """Write one .vtu: coordinates padded to 3D, point and cell arrays."""
+    N, d = mesh.nodes.shape
+    E, n = mesh.conn.shape
+    root = ET.Element("VTKFile", type="UnstructuredGrid", version="0.1", byte_order="LittleEndian")
+    grid = ET.SubElement(root, "UnstructuredGrid")
+    piece = ET.SubElement(grid, "Piece", NumberOfPoints=str(N), NumberOfCells=str(E))
+    pts = ET.SubElement(piece, "Points")
+    xyz = np.zeros((N, 3))
+    xyz[:, :d] = mesh.nodes
+    _array_xml(pts, "positions", xyz, 3)
+    cells = ET.SubElement(piece, "Cells")
+    conn = ET.SubElement(cells, "DataArray", type="Int64", Name="connectivity", format="ascii")
+    conn.text = " ".join(str(int(v)) for v in mesh.conn.reshape(-1))
+    off = ET.SubElement(cells, "DataArray", type="Int64", Name="offsets", format="ascii")
+    off.text = " ".join(str(n * (k + 1)) for k in range(E))
+    typ = ET.SubElement(cells, "DataArray", type="UInt8", Name="types", format="ascii")
+    typ.text = " ".join([str(VTK_CELL[mesh.elem_type])] * E)
+    if point_data:
+        pd = ET.SubElement(piece, "PointData")
+        for name, arr in point_data.items():
+            arr = np.asarray(arr)
+            ncomp = 1 if arr.ndim == 1 else arr.shape[1]
+            if ncomp == 2:  # ParaView draws vectors with 3 components
+                arr = np.concatenate([arr, np.zeros((arr.shape[0], 1), arr.dtype)], axis=1)
+                ncomp = 3
+            _array_xml(pd, name, arr, ncomp)
+    if cell_data:
+        cd = ET.SubElement(piece, "CellData")
+        for name, arr in cell_data.items():
+            arr = np.asarray(arr)
+            _array_xml(cd, name, arr, 1 if arr.ndim == 1 else arr.shape[1])
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    ET.ElementTree(root).write(path, xml_declaration=True, encoding="utf-8")
+
+
+def write_pvd(path, entries):
+    """entries: (time, vtu file name relative to the .pvd)."""
+    root = ET.Element("VTKFile", type="Collection", version="0.1")
+    col = ET.SubElement(root, "Collection")
+    for t, f in entries:
+        ET.SubElement(col, "DataSet", timestep=repr(float(t)), group="", part="0", file=f)
+    ET.ElementTree(root).write(path, xml_declaration=True, encoding="utf-8")
+
+
+class Dumper:
+    """model.set_base_name / add_dump_field / dump, for akx.SolidMechanicsModel."""
+
+    def __init__(self, model, base_name, directory="paraview"):
+        self.model = model
+        self.base = base_name
+        self.dir = directory
+        self.fields = []
+        self.count = 0
+        self.entries = []
+
+    def add_dump_field(self, name):
+        if name not in NODAL + POINTWISE:
+            raise ValueError("unknown dump field %r" % name)
+        if name not in self.fields:
+            self.fields.append(name)
+
+    def dump(self, time=None):
+        m = self.model.mesh
+        pd, cd = {}, {}
+        for f in self.fields:
+            a = self.model.get(f)
+            if f in NODAL:
+                pd[f] = a.astype(np.uint8) if a.dtype == bool else a
+            else:
+                Q = m.nb_qp
+                per = a.reshape(m.nb_elements, Q, -1).mean(axis=1)
+                cd[f] = per[:, 0] if per.shape[1] == 1 else per
+        name = "%s_%04d.vtu" % (self.base, self.count)
+        write_vtu(os.path.join(self.dir, name), m, pd, cd)
+        t = self.count * self.model.time_step if time is None else time
+        self.entries.append((t, name))
+        write_pvd(os.path.join(self.dir, self.base + ".pvd"), self.entries)
+        self.count += 1
+        return os.path.join(self.dir, name)

@@ -7,9 +7,12 @@
  * Same loop as examples/explicit/explicit_dynamic.cc:38-106 of the reference
  * (initialize -> Mesh::read -> initFull(_explicit_lumped_mass) ->
  * getStableTimeStep -> pulse through getDisplacement() -> solveStep +
- * getEnergy per step); the dumper calls are left out.
+ * getEnergy per step), with the ParaView dumps of displacement, velocity,
+ * acceleration and stress every k steps when k is given (the reference dumps
+ * every step).
  *
  *   explicit_dynamic <mesh.msh> <material.dat> <dim> <steps> <fp64|mixed> <out prefix>
+ *                    [dump every k steps]
  *
  * Writes <prefix>energy.csv (id,rtime,epot,ekin,tot), <prefix>dt.txt, and the
  * initial and final displacements as <prefix>u0.txt and <prefix>u.txt (one
@@ -46,6 +49,7 @@ int main(int argc, char* argv[]) {
   const UInt dim = UInt(std::atoi(argv[3]));
   const UInt max_steps = UInt(std::atoi(argv[4]));
   const std::string precision = argv[5], prefix = argv[6];
+  const UInt dump_every = argc > 7 ? UInt(std::atoi(argv[7])) : 0;
   try {
     initialize(material_file, argc, argv);
     const Real pulse_width = 2.;
@@ -74,6 +78,15 @@ int main(int argc, char* argv[]) {
 
     write_nodal(prefix + "u0.txt", displacement, dim);
 
+    if (dump_every) {
+      model.setBaseName("explicit_dynamic", prefix + "paraview");
+      model.addDumpField("displacement");
+      model.addDumpField("velocity");
+      model.addDumpField("acceleration");
+      model.addDumpField("stress");
+      model.dump();
+    }
+
     std::ofstream energy(prefix + "energy.csv");
     energy.precision(17);
     energy << "id,rtime,epot,ekin,tot" << std::endl;
@@ -84,6 +97,7 @@ int main(int argc, char* argv[]) {
       energy << s << "," << s * time_step << "," << epot << "," << ekin << "," << epot + ekin
              << std::endl;
       if (s % 100 == 0) std::cout << "passing step " << s << "/" << max_steps << std::endl;
+      if (dump_every && s % dump_every == 0) model.dump();
     }
 
     std::ofstream dt_out(prefix + "dt.txt");

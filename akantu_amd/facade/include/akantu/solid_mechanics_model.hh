@@ -93,6 +93,15 @@ class SolidMechanicsModel {
   Mesh& getMesh() { return mesh_; }
   UInt getSpatialDimension() const { return dim_; }
 
+  /// ParaView output (Model::setBaseName / addDumpField / dump, src/model/model.cc):
+  /// paraview/<base>_<NNNN>.vtu per dump and paraview/<base>.pvd listing them.
+  /// Nodal fields: displacement, velocity, acceleration, external_force,
+  /// internal_force, mass, blocked_dofs; per-element (averaged over the
+  /// quadrature points): stress, grad_u, damage.
+  void setBaseName(const std::string& name, const std::string& directory = "paraview");
+  void addDumpField(const std::string& field);
+  void dump();
+
   /// extensions: arithmetic of the element kernels, launch autotuning, kernel times
   void setPrecision(Precision p) { precision_ = p; }
   Precision getPrecision() const { return precision_; }
@@ -123,6 +132,10 @@ class SolidMechanicsModel {
 
   Array<Real> u_, v_, a_, fext_, fint_, mass_, pos_, inc_, prev_;
   Array<bool> blocked_;
+  std::string dump_base_ = "model", dump_dir_ = "paraview";
+  std::vector<std::string> dump_fields_;
+  std::vector<std::pair<Real, std::string>> dumps_;
+  UInt steps_done_ = 0;
   bool host_stale_[kCount] = {};
   bool host_dirty_[kCount] = {};
 };

@@ -4,6 +4,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <limits>
 #include <map>
@@ -575,6 +576,7 @@ void SolidMechanicsModel::solveStep(UInt nsteps) {
   if (!initialized_) AKANTU_EXCEPTION("initFull first");
   pushDirty();
   check(akx_solve_step(ctx_, int(nsteps), int(precision_)), "akx_solve_step");
+  steps_done_ += nsteps;
   for (int id : {kU, kV, kA, kFint, kPos, kInc, kPrev}) host_stale_[id] = true;
 }
 
@@ -657,6 +659,112 @@ void SolidMechanicsModel::autotune(UInt nsteps) {
 
 void SolidMechanicsModel::getLastStepTiming(Real& element_ms, Real& node_ms) {
   check(akx_last_step_timing(ctx_, &element_ms, &node_ms), "akx_last_step_timing");
+}
+
+/* ------------------------------------------------------------------------ */
+/* ParaView output                                                           */
+/* ------------------------------------------------------------------------ */
+void SolidMechanicsModel::setBaseName(const std::string& name, const std::string& directory) {
+  dump_base_ = name;
+  dump_dir_ = directory;
+  dumps_.clear();
+}
+
+void SolidMechanicsModel::addDumpField(const std::string& field) {
+  static const char* known[] = {"displacement", "velocity", "acceleration", "external_force",
+                                "internal_force", "mass", "blocked_dofs", "stress", "grad_u",
+                                "damage"};
+  bool ok = false;
+  for (const char* k : known) ok = ok || field == k;
+  if (!ok) AKANTU_EXCEPTION("unknown dump field " << field);
+  if (std::find(dump_fields_.begin(), dump_fields_.end(), field) == dump_fields_.end())
+    dump_fields_.push_back(field);
+}
+
+namespace {
+void vtu_array(std::ostream& o, const std::string& name, const Real* v, size_t count,
+               UInt ncomp, UInt pad_to) {
+  o << "<DataArray type=\"Float64\" Name=\"" << name << "\" NumberOfComponents=\"" << pad_to
+    << "\" format=\"ascii\">\n";
+  for (size_t k = 0; k < count; ++k) {
+    for (UInt c = 0; c < pad_to; ++c) o << (c < ncomp ? v[k * ncomp + c] : 0.) << ' ';
+    o << '\n';
+  }
+  o << "</DataArray>\n";
+}
+}  // namespace
+
+void SolidMechanicsModel::dump() {
+  if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  const ElementType t = mesh_.getElementType();
+  const UInt N = mesh_.getNbNodes(), E = mesh_.getNbElement(), n = getNbNodesPerElement(t);
+  const UInt Q = getNbIntegrationPoints(t), d = dim_;
+  static const std::map<ElementType, int> vtk = {
+      {_triangle_3, 5}, {_quadrangle_4, 9}, {_tetrahedron_4, 10}, {_hexahedron_8, 12}};
+  char num[16];
+  std::snprintf(num, sizeof(num), "%04u", unsigned(dumps_.size()));
+  const std::string file = dump_base_ + "_" + num + ".vtu";
+  std::error_code ec;
+  std::filesystem::create_directories(dump_dir_, ec);
+  std::ofstream o(dump_dir_ + "/" + file);
+  if (!o) AKANTU_EXCEPTION("cannot write " << dump_dir_ << "/" << file);
+  o.precision(17);
+  o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+       "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\""
+    << N << "\" NumberOfCells=\"" << E << "\">\n<Points>\n";
+  vtu_array(o, "positions", mesh_.getNodes().storage(), N, d, 3);
+  o << "</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+  const auto& conn = mesh_.getConnectivity(t);
+  for (UInt e = 0; e < E; ++e) {
+    for (UInt a = 0; a < n; ++a) o << conn(e, a) << ' ';
+    o << '\n';
+  }
+  o << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+  for (UInt e = 0; e < E; ++e) o << size_t(e + 1) * n << '\n';
+  o << "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
+  for (UInt e = 0; e < E; ++e) o << vtk.at(t) << '\n';
+  o << "</DataArray>\n</Cells>\n<PointData>\n";
+  for (const auto& f : dump_fields_) {
+    const Array<Real>* a = nullptr;
+    if (f == "displacement") a = &getDisplacement();
+    else if (f == "velocity") a = &getVelocity();
+    else if (f == "acceleration") a = &getAcceleration();
+    else if (f == "external_force") a = &getExternalForce();
+    else if (f == "internal_force") a = &getInternalForce();
+    else if (f == "mass") a = &getMass();
+    if (a) vtu_array(o, f, a->storage(), N, d, d == 2 ? 3 : d);
+    if (f == "blocked_dofs") {
+      const auto& b = getBlockedDOFs();
+      std::vector<Real> v(size_t(N) * d);
+      for (size_t k = 0; k < v.size(); ++k) v[k] = b[k] ? 1. : 0.;
+      vtu_array(o, f, v.data(), N, d, d == 2 ? 3 : d);
+    }
+  }
+  o << "</PointData>\n<CellData>\n";
+  pushDirty();
+  for (const auto& f : dump_fields_) {
+    int field = -1;
+    if (f == "stress") field = AKX_STRESS;
+    else if (f == "grad_u") field = AKX_GRAD_U;
+    else if (f == "damage") field = AKX_DAMAGE;
+    if (field < 0) continue;
+    const UInt comp = field == AKX_DAMAGE ? 1 : d * d;
+    std::vector<Real> all(size_t(E) * Q * comp), avg(size_t(E) * comp, 0.);
+    check(akx_field_download(ctx_, field, all.data(), all.size() * 8), "akx_field_download");
+    for (UInt e = 0; e < E; ++e)
+      for (UInt q = 0; q < Q; ++q)
+        for (UInt c = 0; c < comp; ++c) avg[size_t(e) * comp + c] += all[(size_t(e) * Q + q) * comp + c] / Q;
+    vtu_array(o, f, avg.data(), E, comp, comp);
+  }
+  o << "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+  dumps_.push_back({Real(steps_done_) * time_step_, file});
+  std::ofstream pvd(dump_dir_ + "/" + dump_base_ + ".pvd");
+  pvd.precision(17);
+  pvd << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n<Collection>\n";
+  for (const auto& dd : dumps_)
+    pvd << "<DataSet timestep=\"" << dd.first << "\" group=\"\" part=\"0\" file=\"" << dd.second
+        << "\"/>\n";
+  pvd << "</Collection>\n</VTKFile>\n";
 }
 
 void SolidMechanicsModel::downloadPointField(int field, Array<Real>& out, const Material& m) {

@@ -242,8 +242,8 @@ def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms
     else:
-        # per-patch partials are FP64: the node kernel is its double-slot instance
-        kname = node_kernel.replace("float", "double") if partials else node_kernel
+        # per-patch partials: the node kernel's partial-sum instance
+        kname = node_kernel.replace(", 1, ", ", 2, ") if partials else node_kernel
         kbytes, kms = nb, nd_ms
     achieved = kbytes / (kms * 1e-3) / 1e9
     step_gbs = (eb + nb) / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
@@ -332,7 +332,7 @@ def bench_cube(args, akx):
         return ("void akx::k_element_patch<2, 0>(akx::ElemArgs)" if patch else
                 "void akx::k_element_mixed<2, 0, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
     kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
-        res, shapes, model, elem_kernel, "void akx::k_node<3, float, false>(akx::NodeArgs)",
+        res, shapes, model, elem_kernel, "void akx::k_node<3, 1, false>(akx::NodeArgs)",
         E, N, n, Q, d)
     traffic = traffic_for(kname, shapes, "cube_tet4_traffic.json") if args.n == 100 else None
     eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
@@ -428,7 +428,7 @@ def bench_hex8(args, akx):
         return ("void akx::k_element_patch<3, 1>(akx::ElemArgs)" if patch else
                 "void akx::k_element_mixed<3, 1, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
     kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
-        res, shapes, model, elem_kernel, "void akx::k_node<3, float, false>(akx::NodeArgs)",
+        res, shapes, model, elem_kernel, "void akx::k_node<3, 1, false>(akx::NodeArgs)",
         E, N, n, Q, d)
     out = base_record(args, res, "f32 element kernels / f64 nodal state", {
         "workload": "hex8-neohookean: %d^3 hexahedron_8 (2x2x2 Gauss), %d nodes, finite-strain "
@@ -500,7 +500,7 @@ def bench_ensemble(args, akx):
         return ("void akx::k_element_patch<0, 0>(akx::ElemArgs)" if patch else
                 "void akx::k_element_mixed<0, 0, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
     kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
-        res, shapes, model, elem_kernel, "void akx::k_node<2, float, false>(akx::NodeArgs)",
+        res, shapes, model, elem_kernel, "void akx::k_node<2, 1, false>(akx::NodeArgs)",
         E, N, n, Q, d, dt_per_node=True)
     out = base_record(args, res, "f32 element kernels / f64 nodal state", {
         "workload": "ensemble: 512 plane-strain triangle_3 specimens, %d elements and %d nodes "

@@ -100,8 +100,12 @@ def test_explicit_dynamic_driver_matches_oracle(precision, tmp_path):
     (tmp_path / "material.dat").write_text(BAR_STEEL)
     steps = 300
     r = _run([os.path.join(BIN, "explicit_dynamic"), str(msh), str(tmp_path / "material.dat"),
-              "3", str(steps), precision, str(tmp_path) + "/"], tmp_path)
+              "3", str(steps), precision, str(tmp_path) + "/", "100"], tmp_path)
     assert r.returncode == 0
+    # ParaView dumps: the initial state and every 100 steps
+    dumps = sorted(p.name for p in (tmp_path / "paraview").glob("explicit_dynamic_*.vtu"))
+    assert len(dumps) == 1 + steps // 100
+    assert (tmp_path / "paraview" / "explicit_dynamic.pvd").exists()
     u0 = np.loadtxt(tmp_path / "u0.txt")
     u = np.loadtxt(tmp_path / "u.txt")
     dt = float((tmp_path / "dt.txt").read_text())
