@@ -326,6 +326,8 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.sur_b = nullptr;
   A.sur_eps = A.sur_sS = A.sur_sW = 0.f;
   A.sur_fallbacks = nullptr;
+  A.sur_x = A.sur_y = nullptr;
+  A.net_ev0 = A.net_ev1 = nullptr;
   if (precision == AKX_MIXED && c->sur.enabled && g.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
     A.sur_img = c->sur.img;
     A.sur_b = c->sur.imgb;
@@ -333,6 +335,8 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
     A.sur_sS = float(c->sur.sS);
     A.sur_sW = float(c->sur.sW);
     A.sur_fallbacks = c->sur.fallbacks;
+    A.sur_x = c->sur.pt_x;
+    A.sur_y = c->sur.pt_y;
     A.patch = 0;
     A.partials = 0;
   }
@@ -383,9 +387,17 @@ int check_ready(akx_ctx* c) {
   return AKX_OK;
 }
 
-int run_elements(akx_ctx* c, int precision) {
+// time_net: record events around the surrogate network launch (if any)
+int run_elements(akx_ctx* c, int precision, bool time_net = false) {
   for (auto& g : c->groups) {
     ElemArgs A = elem_args(c, g, precision);
+    if (time_net && A.sur_img) {
+      for (auto& ev : c->net_ev)
+        if (!ev) HIP_TRY(c, hipEventCreate(&ev));
+      A.net_ev0 = c->net_ev[0];
+      A.net_ev1 = c->net_ev[1];
+      c->net_timed = true;
+    }
     HIP_TRY(c, launch_element(c->et, g.kind, precision, c->stream, A, c->elem_block));
   }
   return AKX_OK;
@@ -548,6 +560,8 @@ int akx_ctx_destroy(akx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_mesh_buffers(c);
   for (auto& e : c->pool) (void)hipEventDestroy(e);
+  for (auto& e : c->net_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AKX_OK;
@@ -1044,7 +1058,7 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
   for (int s = 1; s <= nsteps; ++s) {
     const bool last = (s == nsteps);
     if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
-    rc = run_elements(c, precision);
+    rc = run_elements(c, precision, timing && last);
     if (rc) return rc;
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
     int mode = NODE_CORRECT;
@@ -1067,6 +1081,12 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
     }
     c->last_elem_ms = el / nsteps;
     c->last_node_ms = nd / nsteps;
+    if (c->net_timed) {
+      float a = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&a, c->net_ev[0], c->net_ev[1]));
+      c->last_net_ms = a;
+      c->net_timed = false;
+    }
   }
   return AKX_OK;
 }
@@ -1273,6 +1293,7 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "surrogate_samples") *value = double(c->sur.nsamples);
   else if (k == "surrogate_parameters") *value = double(akx::mlp::NPARAM);
   else if (k == "surrogate_enabled") *value = c->sur.enabled ? 1. : 0.;
+  else if (k == "surrogate_network_ms") *value = c->last_net_ms;
   else return fail(c, AKX_ERR_ARGUMENT, "unknown query key " + k);
   return AKX_OK;
 }
@@ -1386,6 +1407,8 @@ int akx_surrogate_enable(akx_ctx* c, int on) {
         (m.lambda != c->sur.lambda || m.mu != c->sur.mu || m.Sd != c->sur.Sd))
       return fail(c, AKX_ERR_UNSUPPORTED,
                   "every neohookean_damage material must be the one the surrogate was set up for");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, c->sur.ensure_points(c->E));
   c->sur.enabled = true;
   return AKX_OK;
 }

@@ -63,6 +63,9 @@ struct ElemArgs {
   const float* sur_b;
   float sur_eps, sur_sS, sur_sW;
   unsigned long long* sur_fallbacks;
+  float* sur_x;              // [count][8]: scaled network inputs + envelope flag
+  float* sur_y;              // [count][8]: network outputs
+  hipEvent_t net_ev0, net_ev1;  // recorded around the network launch when set
 };
 
 struct NodeArgs {
@@ -103,6 +106,11 @@ enum NodeMode {
 inline __host__ __device__ size_t patch_force_offset(int S, int d) {
   return (size_t(S) * (2 * d * 4 + 4) + size_t(S + 1) * 2 + 15) & ~size_t(15);
 }
+
+// Network evaluation on n points: scaled inputs x (sx floats per point) ->
+// outputs y (sy floats per point) (akx_surrogate.hip).
+hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int64_t n,
+                          const float* x, int sx, float* y, int sy);
 
 // Launchers (akx_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_precompute(int et, hipStream_t s, int64_t E, int64_t N, const double* X,
@@ -165,6 +173,9 @@ struct Surrogate {
   __bf16* dZT[4] = {};
   float *io_x = nullptr, *io_y = nullptr;
   int64_t io_cap = 0;
+  // per-point inputs / outputs of the element step (8 floats per point)
+  float *pt_x = nullptr, *pt_y = nullptr;
+  int64_t pt_cap = 0;
 
   ~Surrogate();
   void release();
@@ -172,6 +183,7 @@ struct Surrogate {
   hipError_t ensure_params();
   hipError_t ensure_batch(int B);
   hipError_t ensure_io(int64_t n);
+  hipError_t ensure_points(int64_t n);
   hipError_t refresh(hipStream_t s);
   hipError_t pass(hipStream_t s, int B, bool grad_only, float lr);
   hipError_t batch_loss(hipStream_t s, int B, double* loss);
@@ -286,4 +298,7 @@ struct akx_ctx {
   std::vector<hipEvent_t> pool;  // per-step timing events
   bool timing = false;
   double last_elem_ms = 0, last_node_ms = 0;
+  hipEvent_t net_ev[2] = {nullptr, nullptr};  // surrogate network of the last timed step
+  bool net_timed = false;
+  double last_net_ms = 0;
 };

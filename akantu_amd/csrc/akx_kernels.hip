@@ -244,22 +244,28 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
             J[i + j * d] = s;
           }
         }
-      const float det = detn<d>(J);
-      const float idet = 1.f / det;
-      float Ji[d * d];
+      // inverse through the adjugate: det from its first column, 1/det by
+      // the hardware reciprocal (about 1 ulp; this is the FP32 path)
+      float Ad[d * d], det;
       if constexpr (d == 2) {
-        Ji[0] = J[3] * idet; Ji[1] = -J[1] * idet; Ji[2] = -J[2] * idet; Ji[3] = J[0] * idet;
+        Ad[0] = J[3]; Ad[1] = -J[1]; Ad[2] = -J[2]; Ad[3] = J[0];
+        det = J[0] * Ad[0] + J[2] * Ad[1];
       } else {
-        Ji[0] = (J[4] * J[8] - J[7] * J[5]) * idet;
-        Ji[1] = (J[2] * J[7] - J[8] * J[1]) * idet;
-        Ji[2] = (J[1] * J[5] - J[4] * J[2]) * idet;
-        Ji[3] = (J[5] * J[6] - J[8] * J[3]) * idet;
-        Ji[4] = (J[0] * J[8] - J[6] * J[2]) * idet;
-        Ji[5] = (J[2] * J[3] - J[5] * J[0]) * idet;
-        Ji[6] = (J[3] * J[7] - J[6] * J[4]) * idet;
-        Ji[7] = (J[1] * J[6] - J[7] * J[0]) * idet;
-        Ji[8] = (J[0] * J[4] - J[3] * J[1]) * idet;
+        Ad[0] = J[4] * J[8] - J[7] * J[5];
+        Ad[1] = J[2] * J[7] - J[8] * J[1];
+        Ad[2] = J[1] * J[5] - J[4] * J[2];
+        Ad[3] = J[5] * J[6] - J[8] * J[3];
+        Ad[4] = J[0] * J[8] - J[6] * J[2];
+        Ad[5] = J[2] * J[3] - J[5] * J[0];
+        Ad[6] = J[3] * J[7] - J[6] * J[4];
+        Ad[7] = J[1] * J[6] - J[7] * J[0];
+        Ad[8] = J[0] * J[4] - J[3] * J[1];
+        det = J[0] * Ad[0] + J[3] * Ad[1] + J[6] * Ad[2];
       }
+      const float idet = __builtin_amdgcn_rcpf(det);
+      float Ji[d * d];
+#pragma unroll
+      for (int k = 0; k < d * d; ++k) Ji[k] = Ad[k] * idet;
       // B(i,a) = sum_l Ji(i,l) dN_a/dxi_l
 #pragma unroll
       for (int a = 0; a < n; ++a)
@@ -384,161 +390,190 @@ __global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
 }
 
 // ===========================================================================
-// element kernel, mixed precision, neo-Hookean + damage through the
-// constitutive surrogate (tetrahedron_4).  One wave = 32 elements; lanes j
-// and j + 32 both form element j's Green strain, the network (akx_mlp.hpp)
-// returns S_NH and W_NH for the 32 points together, lane j applies the damage
-// update and the B^T P integration.  A point outside the training envelope
-// (|E_k| > eps_max or damage outside [0, 1]) takes the plain FP32 law instead
-// and is counted.  Blocks are persistent: the network's weights are staged in
-// LDS once per block.
+// element step, mixed precision, neo-Hookean + damage through the
+// constitutive surrogate (tetrahedron_4), in three launches:
+//   k_sur_strain  one thread per element: Green strain of the point, network
+//                 inputs and the envelope flag (|E_k| <= eps_max, damage in
+//                 [0, 1]) to sur_x;
+//   network       persistent MFMA blocks, sur_x -> sur_y (akx_surrogate.hip);
+//   k_sur_forces  one thread per element: S_NH, W_NH from the network (or the
+//                 plain FP32 law outside the envelope, counted), Marigo update,
+//                 B^T (F S) det(J) w into the force slots.
+// The element gathers run at full occupancy; the network kernel streams its
+// inputs.
 // ===========================================================================
-__global__ void __launch_bounds__(256) k_element_surrogate(ElemArgs A) {
+struct SurPoint {
+  float Bq[12], jq, G[9];
+};
+
+// Geometry and displacement gradient of tetrahedron_4 element e (same
+// arithmetic as k_element_mixed<TET4, KIND, true>).
+__device__ __forceinline__ void sur_point(const ElemArgs& A, int64_t e, SurPoint& P) {
 #pragma clang fp contract(fast)
   constexpr int d = 3, n = 4, dn = 12;
-  __shared__ mlp::Image L;
-  mlp::stage_image(L, static_cast<const mlp::bf16x8*>(A.sur_img), A.sur_b);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, j = lane & 31, h = lane >> 5;
   const int64_t E = A.E;
-  const int64_t tiles = (A.count + 31) / 32;
-  const float eps = A.sur_eps, ieps = 1.f / A.sur_eps;
-  unsigned long long fallbacks = 0;
-  for (int64_t tile = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
-       tile += int64_t(gridDim.x) * (blockDim.x >> 6)) {
-    const int64_t t = tile * 32 + j;
-    const bool valid = t < A.count;
-    const int64_t e = valid ? (A.filter ? int64_t(A.filter[t]) : t) : 0;
+  int64_t nodes[n];
+#pragma unroll
+  for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
+  float du[dn], dX[dn];
+  double u0[d], x0[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    u0[i] = A.u[nodes[0] * d + i];
+    x0[i] = A.X[nodes[0] * d + i];
+    du[i] = 0.f;
+    dX[i] = 0.f;
+  }
+#pragma unroll
+  for (int a = 1; a < n; ++a)
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
+      dX[i + a * d] = float(A.X[nodes[a] * d + i] - x0[i]);
+    }
+  float Jm[9];
+#pragma unroll
+  for (int jj = 0; jj < d; ++jj)
+#pragma unroll
+    for (int i = 0; i < d; ++i) Jm[i + jj * d] = dX[jj + (i + 1) * d];
+  const float det = det3(Jm);
+  const float idet = 1.f / det;
+  float Ji[9];
+  Ji[0] = (Jm[4] * Jm[8] - Jm[7] * Jm[5]) * idet;
+  Ji[1] = (Jm[2] * Jm[7] - Jm[8] * Jm[1]) * idet;
+  Ji[2] = (Jm[1] * Jm[5] - Jm[4] * Jm[2]) * idet;
+  Ji[3] = (Jm[5] * Jm[6] - Jm[8] * Jm[3]) * idet;
+  Ji[4] = (Jm[0] * Jm[8] - Jm[6] * Jm[2]) * idet;
+  Ji[5] = (Jm[2] * Jm[3] - Jm[5] * Jm[0]) * idet;
+  Ji[6] = (Jm[3] * Jm[7] - Jm[6] * Jm[4]) * idet;
+  Ji[7] = (Jm[1] * Jm[6] - Jm[7] * Jm[0]) * idet;
+  Ji[8] = (Jm[0] * Jm[4] - Jm[3] * Jm[1]) * idet;
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    float sm = 0.f;
+#pragma unroll
+    for (int l = 0; l < d; ++l) sm -= Ji[i + l * d];
+    P.Bq[i] = sm;
+#pragma unroll
+    for (int a = 1; a < n; ++a) P.Bq[i + a * d] = Ji[i + (a - 1) * d];
+  }
+  P.jq = det * A.tabf.w[0];
+#pragma unroll
+  for (int jj = 0; jj < d; ++jj)
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      float sm = 0.f;
+#pragma unroll
+      for (int a = 1; a < n; ++a) sm += du[i + a * d] * P.Bq[jj + a * d];
+      P.G[i + jj * d] = sm;
+    }
+}
+
+// Green-Lagrange strain E = (G + G^T + G^T G) / 2, Voigt xx yy zz yz xz xy
+__device__ __forceinline__ void green_voigt(const float* G, float* Ev) {
+#pragma clang fp contract(fast)
+  const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = vi[k], b = vj[k];
+    float gtg = 0.f;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) gtg += G[l + a * 3] * G[l + b * 3];
+    Ev[k] = 0.5f * (G[a + b * 3] + G[b + a * 3] + gtg);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sur_strain(ElemArgs A) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  SurPoint P;
+  sur_point(A, e, P);
+  float Ev[6];
+  green_voigt(P.G, Ev);
+  const double dam = A.damage[e];
+  bool inside = dam >= 0. && dam <= 1.;
+  const float ieps = 1.f / A.sur_eps;
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    inside = inside && fabsf(Ev[k]) <= A.sur_eps;
+    x[k] = Ev[k] * ieps;
+  }
+  x[6] = float(dam);
+  x[7] = inside ? 1.f : 0.f;
+  float4* o = reinterpret_cast<float4*>(A.sur_x + t * 8);
+  o[0] = make_float4(x[0], x[1], x[2], x[3]);
+  o[1] = make_float4(x[4], x[5], x[6], x[7]);
+}
+
+__global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
+#pragma clang fp contract(fast)
+  constexpr int d = 3, n = 4, dn = 12;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = t < A.count;
+  unsigned long long fallback = 0;
+  if (live) {
+    const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
     const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
-    int64_t nodes[n];
-#pragma unroll
-    for (int a = 0; a < n; ++a) nodes[a] = A.conn[a * E + e];
-    float du[dn], dX[dn];
-    double u0[d], x0[d];
-#pragma unroll
-    for (int i = 0; i < d; ++i) {
-      u0[i] = A.u[nodes[0] * d + i];
-      x0[i] = A.X[nodes[0] * d + i];
-      du[i] = 0.f;
-      dX[i] = 0.f;
-    }
-#pragma unroll
-    for (int a = 1; a < n; ++a)
-#pragma unroll
-      for (int i = 0; i < d; ++i) {
-        du[i + a * d] = float(A.u[nodes[a] * d + i] - u0[i]);
-        dX[i + a * d] = float(A.X[nodes[a] * d + i] - x0[i]);
-      }
-    // B = J^-1 dN/dxi and det(J) w from the coordinates (tetrahedron_4)
-    float Jm[9];
-#pragma unroll
-    for (int jj = 0; jj < d; ++jj)
-#pragma unroll
-      for (int i = 0; i < d; ++i) Jm[i + jj * d] = dX[jj + (i + 1) * d];
-    const float det = det3(Jm);
-    const float idet = 1.f / det;
-    float Ji[9];
-    Ji[0] = (Jm[4] * Jm[8] - Jm[7] * Jm[5]) * idet;
-    Ji[1] = (Jm[2] * Jm[7] - Jm[8] * Jm[1]) * idet;
-    Ji[2] = (Jm[1] * Jm[5] - Jm[4] * Jm[2]) * idet;
-    Ji[3] = (Jm[5] * Jm[6] - Jm[8] * Jm[3]) * idet;
-    Ji[4] = (Jm[0] * Jm[8] - Jm[6] * Jm[2]) * idet;
-    Ji[5] = (Jm[2] * Jm[3] - Jm[5] * Jm[0]) * idet;
-    Ji[6] = (Jm[3] * Jm[7] - Jm[6] * Jm[4]) * idet;
-    Ji[7] = (Jm[1] * Jm[6] - Jm[7] * Jm[0]) * idet;
-    Ji[8] = (Jm[0] * Jm[4] - Jm[3] * Jm[1]) * idet;
-    float Bq[dn];
-#pragma unroll
-    for (int i = 0; i < d; ++i) {
-      float s = 0.f;
-#pragma unroll
-      for (int l = 0; l < d; ++l) s -= Ji[i + l * d];
-      Bq[i] = s;
-#pragma unroll
-      for (int a = 1; a < n; ++a) Bq[i + a * d] = Ji[i + (a - 1) * d];
-    }
-    const float jq = det * A.tabf.w[0];
-    float G[9];
-#pragma unroll
-    for (int jj = 0; jj < d; ++jj)
-#pragma unroll
-      for (int i = 0; i < d; ++i) {
-        float s = 0.f;
-#pragma unroll
-        for (int a = 1; a < n; ++a) s += du[i + a * d] * Bq[jj + a * d];
-        G[i + jj * d] = s;
-      }
-    // Green-Lagrange strain E = (G + G^T + G^T G) / 2, Voigt xx yy zz yz xz xy
-    float Ev[6];
+    SurPoint P;
+    sur_point(A, e, P);
+    const float* G = P.G;
+    const bool inside = A.sur_x[t * 8 + 7] != 0.f;
     const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int a = vi[k], b = vj[k];
-      float gtg = 0.f;
-#pragma unroll
-      for (int l = 0; l < d; ++l) gtg += G[l + a * d] * G[l + b * d];
-      Ev[k] = 0.5f * (G[a + b * d] + G[b + a * d] + gtg);
-    }
-    const double dam_old = valid ? A.damage[e] : 0.;
-    bool inside = dam_old >= 0. && dam_old <= 1.;
-    float x[7], y[7];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      inside = inside && fabsf(Ev[k]) <= eps;
-      x[k] = valid ? Ev[k] * ieps : 0.f;
-    }
-    x[6] = float(dam_old);
-    mlp::eval_wave(L, x, y);
-    if (!valid || h) continue;
     float S[9], W;
     if (inside) {
+      float Ev[6];
+      green_voigt(G, Ev);
+      const float4* yv = reinterpret_cast<const float4*>(A.sur_y + t * 8);
+      const float4 y0 = yv[0], y1 = yv[1];
+      const float y[7] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z};
       const float tr = Ev[0] + Ev[1] + Ev[2];
       const float ee = Ev[0] * Ev[0] + Ev[1] * Ev[1] + Ev[2] * Ev[2] +
                        2.f * (Ev[3] * Ev[3] + Ev[4] * Ev[4] + Ev[5] * Ev[5]);
-      float Sv[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        Sv[k] = 2.f * m.mu_f * Ev[k] + (k < 3 ? m.lambda_f * tr : 0.f) + A.sur_sS * y[k];
-      W = 0.5f * m.lambda_f * tr * tr + m.mu_f * ee + A.sur_sW * y[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        S[vi[k] + vj[k] * d] = Sv[k];
-        S[vj[k] + vi[k] * d] = Sv[k];
+        const float sv = 2.f * m.mu_f * Ev[k] + (k < 3 ? m.lambda_f * tr : 0.f) + A.sur_sS * y[k];
+        S[vi[k] + vj[k] * d] = sv;
+        S[vj[k] + vi[k] * d] = sv;
       }
+      W = 0.5f * m.lambda_f * tr * tr + m.mu_f * ee + A.sur_sW * y[6];
     } else {
       neohookean_stress_f32<d>(m.lambda_f, m.mu_f, G, S, &W);
-      ++fallbacks;
+      fallback = 1;
     }
     // Marigo update (material_marigo_inline_impl.hh:67-76) with Y = W
-    double dam = dam_old;
+    double dam = A.damage[e];
     const double Fd = double(W) - A.Yd[e] - m.Sd * dam;
     if (Fd > 0) dam = (double(W) - A.Yd[e]) / m.Sd;
     dam = fmin(dam, 1.);
     A.damage[e] = dam;
-    const float scale = jq * float(1. - dam);
-    float P[9];
+    const float scale = P.jq * float(1. - dam);
+    float Pk[9];
 #pragma unroll
     for (int i = 0; i < d; ++i)
 #pragma unroll
       for (int jj = 0; jj < d; ++jj) {
-        float s = S[i + jj * d];
+        float sm = S[i + jj * d];
 #pragma unroll
-        for (int l = 0; l < d; ++l) s += G[i + l * d] * S[l + jj * d];
-        P[i + jj * d] = s * scale;
+        for (int l = 0; l < d; ++l) sm += G[i + l * d] * S[l + jj * d];
+        Pk[i + jj * d] = sm * scale;
       }
     float fe[dn];
 #pragma unroll
     for (int a = 0; a < n; ++a)
 #pragma unroll
       for (int i = 0; i < d; ++i) {
-        float s = 0.f;
+        float sm = 0.f;
 #pragma unroll
-        for (int jj = 0; jj < d; ++jj) s += P[i + jj * d] * Bq[jj + a * d];
-        fe[i + a * d] = s;
+        for (int jj = 0; jj < d; ++jj) sm += Pk[i + jj * d] * P.Bq[jj + a * d];
+        fe[i + a * d] = sm;
       }
     store_slots<d, n>(A, e, fe);
   }
-  if (fallbacks) atomicAdd(A.sur_fallbacks, fallbacks);
+  // count the fallbacks of the wave with one atomic
+  const unsigned long long wave_fb = __popcll(__ballot(fallback != 0));
+  if ((threadIdx.x & 63) == 0 && wave_fb) atomicAdd(A.sur_fallbacks, wave_fb);
 }
 
 // ===========================================================================
@@ -1239,9 +1274,15 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
   } else if (A.sur_img) {
     if constexpr (ET == TET4) {
       if (kind != NH_DAMAGE) return hipErrorInvalidValue;
-      const int64_t tiles = (A.count + 31) / 32;
-      const unsigned gs = unsigned(std::min<int64_t>(256, (tiles + 3) / 4));
-      k_element_surrogate<<<gs, 256, 0, s>>>(A);
+      const unsigned ge = grid_for(A.count, 256);
+      k_sur_strain<<<ge, 256, 0, s>>>(A);
+      hipError_t err = hipGetLastError();
+      if (err != hipSuccess) return err;
+      if (A.net_ev0) (void)hipEventRecord(A.net_ev0, s);
+      err = launch_sur_mlp(s, A.sur_img, A.sur_b, A.count, A.sur_x, 8, A.sur_y, 8);
+      if (err != hipSuccess) return err;
+      if (A.net_ev1) (void)hipEventRecord(A.net_ev1, s);
+      k_sur_forces<<<ge, 256, 0, s>>>(A);
     } else {
       return hipErrorInvalidValue;
     }

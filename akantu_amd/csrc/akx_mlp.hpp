@@ -123,29 +123,45 @@ __device__ __forceinline__ void eval_wave(const Image& L, const float* x, float*
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const int v = 8 * (t & 1) + s;
-        const float z = acc[t >> 1][v] + L.b[layer * HID + chain_k(t, h, s)];
-        act[t][s] = to_bf16(silu(z));
+      for (int g = 0; g < 2; ++g) {
+        // slots 4g..4g+3 carry 4 consecutive features: one 16-byte bias read
+        const float4 bb =
+            *reinterpret_cast<const float4*>(&L.b[layer * HID + chain_k(t, h, 4 * g)]);
+        const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int s = 4 * g + r;
+          const float z = acc[t >> 1][8 * (t & 1) + s] + bv[r];
+          act[t][s] = to_bf16(silu(z));
+        }
       }
   };
   activate(0);
-  // one copy of the hidden-layer code: unrolling the three layers lets the
-  // scheduler hoist every weight load and spill
+  // One copy of the hidden-layer code (unrolling the three layers lets the
+  // scheduler hoist every weight load and spill).  K steps outermost: the
+  // four MFMAs of a step use independent accumulators; the LDS latency of
+  // the weight reads is covered by the other wave of the SIMD.
 #pragma unroll 1
   for (int layer = 1; layer <= 3; ++layer) {
-    const int base = PW_L1 + (layer - 1) * (PW_L2 - PW_L1);
+    const bf16x8* wl = L.w + PW_L1 + (layer - 1) * (PW_L2 - PW_L1) + lane;
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      acc[mb] = f32x16{};
+    for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x16{};
 #pragma unroll
-      for (int t = 0; t < 8; ++t) acc[mb] = mfma(L.w[base + (mb * 8 + t) * 64 + lane], act[t], acc[mb]);
+    for (int t = 0; t < 8; ++t) {
+      bf16x8 w[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) w[mb] = wl[(mb * 8 + t) * 64];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) acc[mb] = mfma(w[mb], act[t], acc[mb]);
     }
     activate(layer);
   }
   f32x16 o = f32x16{};
+  bf16x8 w4[8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) o = mfma(L.w[PW_L4 + t * 64 + lane], act[t], o);
+  for (int t = 0; t < 8; ++t) w4[t] = L.w[PW_L4 + t * 64 + lane];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) o = mfma(w4[t], act[t], o);
   // rows 0..3 sit in registers 0..3 of half 0, rows 4..7 in registers 0..3 of half 1
   float mine[4], other[4];
 #pragma unroll

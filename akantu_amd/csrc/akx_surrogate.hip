@@ -394,25 +394,39 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
   }
 }
 
-// evaluation of the network on n points (scaled inputs [n x 7] -> outputs
-// [n x 7]); persistent blocks of 4 waves, the image staged in LDS once
-__global__ void __launch_bounds__(256) k_sur_infer(const bf16x8* __restrict__ img,
+// evaluation of the network on n points (scaled inputs, sx floats per point
+// -> outputs, sy floats per point); persistent blocks of 8 waves, the image
+// staged in LDS once per block
+__global__ void __launch_bounds__(512) k_sur_infer(const bf16x8* __restrict__ img,
                                                    const float* __restrict__ imgb, int64_t n,
-                                                   const float* __restrict__ X, float* Y) {
+                                                   const float* __restrict__ X, int sx, float* Y,
+                                                   int sy) {
   __shared__ Image L;
   stage_image(L, img, imgb);
   __syncthreads();
   const int lane = threadIdx.x & 63, j = lane & 31, h = lane >> 5;
   const int64_t tiles = (n + 31) / 32;
   const int waves = blockDim.x >> 6;
-  for (int64_t tile = int64_t(blockIdx.x) * waves + (threadIdx.x >> 6); tile < tiles;
-       tile += int64_t(gridDim.x) * waves) {
-    const int64_t p = tile * 32 + j;
+  const int64_t stride = int64_t(gridDim.x) * waves;
+  int64_t tile = int64_t(blockIdx.x) * waves + (threadIdx.x >> 6);
+  // the next tile's inputs are loaded while the current one is evaluated
+  float xn[7];
+  auto load = [&](int64_t tl, float* x) {
+    const int64_t p = tl * 32 + j;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) x[k] = (tl < tiles && p < n) ? X[p * sx + k] : 0.f;
+  };
+  load(tile, xn);
+  for (; tile < tiles; tile += stride) {
     float x[7], y[7];
-    for (int k = 0; k < 7; ++k) x[k] = p < n ? X[p * 7 + k] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) x[k] = xn[k];
+    load(tile + stride, xn);
     eval_wave(L, x, y);
+    const int64_t p = tile * 32 + j;
     if (p < n && h == 0)
-      for (int k = 0; k < 7; ++k) Y[p * 7 + k] = y[k];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) Y[p * sy + k] = y[k];
   }
 }
 
@@ -450,8 +464,10 @@ void Surrogate::release() {
   dfree_p(X); dfree_p(T);
   dfree_p(fallbacks);
   dfree_p(io_x); dfree_p(io_y);
+  dfree_p(pt_x); dfree_p(pt_y);
   nsamples = 0;
   io_cap = 0;
+  pt_cap = 0;
 }
 
 hipError_t Surrogate::ensure_params() {
@@ -675,12 +691,31 @@ hipError_t Surrogate::infer(hipStream_t s, int64_t n, const float* x_host, float
   hipError_t e;
   if ((e = ensure_params()) || (e = ensure_io(n))) return e;
   if ((e = hipMemcpyAsync(io_x, x_host, size_t(n) * 7 * 4, hipMemcpyHostToDevice, s))) return e;
-  const int64_t tiles = (n + 31) / 32;
-  const unsigned g = unsigned(std::min<int64_t>(256, (tiles + 3) / 4));
-  if (n > 0) k_sur_infer<<<g, 256, 0, s>>>(img, imgb, n, io_x, io_y);
-  if ((e = hipGetLastError())) return e;
+  if ((e = launch_sur_mlp(s, img, imgb, n, io_x, 7, io_y, 7))) return e;
   if ((e = hipMemcpyAsync(y_host, io_y, size_t(n) * 7 * 4, hipMemcpyDeviceToHost, s))) return e;
   return hipStreamSynchronize(s);
+}
+
+hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int64_t n,
+                          const float* x, int sx, float* y, int sy) {
+  (void)hipGetLastError();
+  if (n <= 0) return hipSuccess;
+  // one block of 8 waves per CU (the weight image fills most of the LDS):
+  // two waves per SIMD, so one wave's SiLU overlaps the other's MFMAs
+  const int64_t tiles = (n + 31) / 32;
+  const unsigned g = unsigned(std::min<int64_t>(256, (tiles + 7) / 8));
+  k_sur_infer<<<g, 512, 0, s>>>(static_cast<const bf16x8*>(img), imgb, n, x, sx, y, sy);
+  return hipGetLastError();
+}
+
+hipError_t Surrogate::ensure_points(int64_t n) {
+  if (n <= pt_cap) return hipSuccess;
+  dfree_p(pt_x);
+  dfree_p(pt_y);
+  hipError_t e;
+  if ((e = dalloc_n(&pt_x, size_t(n) * 8)) || (e = dalloc_n(&pt_y, size_t(n) * 8))) return e;
+  pt_cap = n;
+  return hipSuccess;
 }
 
 hipError_t Surrogate::set_params(hipStream_t s, const float* host) {
