@@ -58,7 +58,7 @@ struct ElemArgs {
   const uint16_t* lconn;     // [a][E] patch-local node index
   const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
   ElemTablesF tabf;
-  // constitutive surrogate (k_element_surrogate); sur_img == nullptr: plain law
+  // constitutive surrogate (k_sur_strain / network / k_sur_forces); sur_img == nullptr: plain law
   const void* sur_img;
   const float* sur_b;
   float sur_eps, sur_sS, sur_sW;
