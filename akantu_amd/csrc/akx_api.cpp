@@ -71,6 +71,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->pnode_cnt); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
   dfree(c->lconn); dfree(c->lrank);
   dfree(c->pcsr_off); dfree(c->pentry);
+  dfree(c->tab64_d);
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
@@ -322,6 +323,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.lconn = c->lconn;
   A.lrank = c->lrank;
   A.tabf = c->tabf;
+  A.tab64 = c->tab64_d;
   A.sur_img = nullptr;
   A.sur_b = nullptr;
   A.sur_eps = A.sur_sS = A.sur_sW = 0.f;
@@ -863,6 +865,8 @@ int akx_init(akx_ctx* c) {
     for (int k = 0; k < 24; ++k) c->tabf.dnds[q][k] = float(tab.dnds[q][k]);
     c->tabf.w[q] = float(tab.w[q]);
   }
+  HIP_TRY(c, dalloc(&c->tab64_d, 1));
+  HIP_TRY(c, hipMemcpy(c->tab64_d, &tab, sizeof(ElemTables), hipMemcpyHostToDevice));
   HIP_TRY(c, launch_precompute(c->et, c->stream, E, N, c->X, c->conn_soa, tab, c->B64, c->J64,
                                c->B32, c->J32, c->flag));
   int bad = 0;
@@ -1472,7 +1476,7 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 4 : 2;
   for (int lay = 0; lay < nlay; ++lay)
     for (int stage = 0; stage < ((lay == 1 || lay == 2) ? 2 : 1); ++stage)
-      for (int rcmp = (lay >= 2 ? 1 : 0); rcmp < (precision == AKX_MIXED ? 2 : 1); ++rcmp)
+      for (int rcmp = (lay >= 2 ? 1 : 0); rcmp < 2; ++rcmp)
         for (int x = 0; x < 2; ++x)
           for (int eb : blocks) {
             if (lay >= 2 && eb != 256) continue;  // the patch kernel has its own block shape
@@ -1485,7 +1489,7 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
           const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;
           c->elem_block = eb; c->node_block = nb; c->xcd_remap = x; c->node_ordered = lay;
           c->node_stage = sh.stage;
-          c->mixed_recompute = precision == AKX_MIXED ? sh.rcmp : keep[3];
+          c->mixed_recompute = sh.rcmp;
           if ((rc = restore())) break;
           if ((rc = akx_solve_step(c, 1, precision))) break;  // warm-up
           if (hipEventRecord(t0, c->stream) != hipSuccess) { rc = AKX_ERR_HIP; break; }

@@ -58,6 +58,7 @@ struct ElemArgs {
   const uint16_t* lconn;     // [a][E] patch-local node index
   const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
   ElemTablesF tabf;
+  const ElemTables* tab64;   // FP64 element tables in device memory (FP64 recompute)
   // constitutive surrogate (k_sur_strain / network / k_sur_forces); sur_img == nullptr: plain law
   const void* sur_img;
   const float* sur_b;
@@ -268,6 +269,7 @@ struct akx_ctx {
   // node -> entries of the per-patch partial sums (slot layout 3)
   uint32_t *pcsr_off = nullptr, *pentry = nullptr;
   akx::ElemTablesF tabf;
+  akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal
   bool reorder = true;
   std::vector<uint32_t> node_perm, node_inv, elem_perm, elem_inv;

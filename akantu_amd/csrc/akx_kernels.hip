@@ -19,6 +19,22 @@
 
 namespace akx {
 
+// Shape derivatives and det(J) w of one point from the element's reference
+// coordinates: the arithmetic of k_precompute (element_class_tmpl.hh:282-355,
+// integrator_gauss_inline_impl.hh:208-252), so the values are bit-identical to
+// the tables it writes.
+template <int ET>
+__device__ __forceinline__ double point_shapes_ref(const ElemTables& tab, int q, const double* xe,
+                                                   double* B) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n;
+  double J[d * d], Ji[d * d];
+  matrix_matrixt<d, d, n>(tab.dnds[q], xe, J);
+  invn<d>(J, Ji);
+  matrix_matrix<d, n, d>(Ji, tab.dnds[q], B);
+  return detn<d>(J) * tab.w[q];
+}
+
 // ===========================================================================
 // precompute: B = J^{-1} dN/dxi and det(J) w
 //   ShapeLagrange::computeShapeDerivativesOnIntegrationPoints
@@ -45,11 +61,8 @@ __global__ void __launch_bounds__(256) k_precompute(int64_t E, const double* __r
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    double J[d * d], Ji[d * d], B[dn];
-    matrix_matrixt<d, d, n>(tab.dnds[q], xe, J);
-    invn<d>(J, Ji);
-    matrix_matrix<d, n, d>(Ji, tab.dnds[q], B);
-    const double jw = detn<d>(J) * tab.w[q];
+    double B[dn];
+    const double jw = point_shapes_ref<ET>(tab, q, xe, B);
 #pragma unroll
     for (int k = 0; k < dn; ++k) {
       B64[(int64_t(q) * dn + k) * E + e] = B[k];
@@ -144,7 +157,10 @@ __device__ __forceinline__ void store_slots(const ElemArgs& A, int64_t e, const 
 // ===========================================================================
 // element kernel, FP64, reference operation order
 // ===========================================================================
-template <int ET, int KIND>
+// RECOMPUTE = true: B and det(J) w rebuilt per point from the coordinates
+// (point_shapes_ref) instead of read from the FP64 tables: E*Q*(d*n+1)*8
+// bytes of table reads become one more gather of the nodes' coordinates.
+template <int ET, int KIND, bool RECOMPUTE>
 __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
   using T = Elem<ET>;
   constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
@@ -157,11 +173,15 @@ __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
   const double* __restrict__ Jt = static_cast<const double*>(A.J);
   // FEEngine::extractNodalToElementField (fe_engine_inline_impl.hh:104-143)
   double ue[dn];
+  double xe[RECOMPUTE ? dn : 1];
 #pragma unroll
   for (int a = 0; a < n; ++a) {
     const int64_t node = A.conn[a * E + e];
 #pragma unroll
-    for (int i = 0; i < d; ++i) ue[i + a * d] = A.u[node * d + i];
+    for (int i = 0; i < d; ++i) {
+      ue[i + a * d] = A.u[node * d + i];
+      if constexpr (RECOMPUTE) xe[i + a * d] = A.X[node * d + i];
+    }
   }
   double fe[dn];
 #pragma unroll
@@ -169,8 +189,13 @@ __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     double Bq[dn];
+    double jq_rc = 0.;
+    if constexpr (RECOMPUTE) {
+      jq_rc = point_shapes_ref<ET>(*A.tab64, q, xe, Bq);
+    } else {
 #pragma unroll
-    for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+      for (int k = 0; k < dn; ++k) Bq[k] = Bt[(int64_t(q) * dn + k) * E + e];
+    }
     double G[d * d], S[d * d], R[dn];
     // grad u = u_e B^T (shape_functions_inline_impl.hh:388, Matrix::mul<false, true>)
     matrix_matrixt<d, d, n>(ue, Bq, G);
@@ -190,7 +215,7 @@ __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
       finite_residual_ref<d, n>(Bq, G, S, R);
     }
     // IntegratorGauss::integrate (integrator_gauss_inline_impl.hh:446-452)
-    const double jq = Jt[int64_t(q) * E + e];
+    const double jq = RECOMPUTE ? jq_rc : Jt[int64_t(q) * E + e];
 #pragma unroll
     for (int k = 0; k < dn; ++k) fe[k] += R[k] * jq;
   }
@@ -1266,9 +1291,18 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
   const unsigned g = grid_for(A.count, block);
   if (precision == 0) {
     switch (kind) {
-    case ELASTIC: k_element_fp64<ET, ELASTIC><<<g, block, 0, s>>>(A); break;
-    case NEOHOOKEAN: k_element_fp64<ET, NEOHOOKEAN><<<g, block, 0, s>>>(A); break;
-    case NH_DAMAGE: k_element_fp64<ET, NH_DAMAGE><<<g, block, 0, s>>>(A); break;
+    case ELASTIC:
+      if (A.recompute) k_element_fp64<ET, ELASTIC, true><<<g, block, 0, s>>>(A);
+      else k_element_fp64<ET, ELASTIC, false><<<g, block, 0, s>>>(A);
+      break;
+    case NEOHOOKEAN:
+      if (A.recompute) k_element_fp64<ET, NEOHOOKEAN, true><<<g, block, 0, s>>>(A);
+      else k_element_fp64<ET, NEOHOOKEAN, false><<<g, block, 0, s>>>(A);
+      break;
+    case NH_DAMAGE:
+      if (A.recompute) k_element_fp64<ET, NH_DAMAGE, true><<<g, block, 0, s>>>(A);
+      else k_element_fp64<ET, NH_DAMAGE, false><<<g, block, 0, s>>>(A);
+      break;
     default: return hipErrorInvalidValue;
     }
   } else if (A.sur_img) {

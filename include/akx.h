@@ -205,13 +205,16 @@ int akx_enable_timing(akx_ctx* ctx, int on);
 int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
 
 /* Launch shape of the element and node kernels: threads per block (64..256,
- * 0 keeps the current value), XCD-aware block mapping (1/0, -1 keeps), for
- * the mixed path whether B and det(J) w are rebuilt from the coordinates (1)
- * or read from FP32 tables (0), the layout of the element force slots:
- * element-ordered (0), node-ordered (1) or node-ordered written by LDS
- * element patches (2, mixed path), and whether the node kernel copies its
- * block's contiguous slot range to LDS first (node_stage 1/0) (-1 keeps).
- * None of these changes a result bit. */
+ * 0 keeps the current value), XCD-aware block mapping (1/0, -1 keeps),
+ * whether B and det(J) w are rebuilt from the coordinates (1) or read from
+ * the tables (0) -- the FP64 rebuild repeats the precompute's arithmetic and
+ * is bit-identical to the tables --, the layout of the element force slots:
+ * element-ordered (0), node-ordered (1), node-ordered written by LDS element
+ * patches (2, mixed path) or one FP64 partial sum per (patch, node) (3, mixed
+ * path), and whether the node kernel copies its block's contiguous slot
+ * range to LDS first (node_stage 1/0) (-1 keeps).  On the FP64 path none of
+ * these changes a result bit; on the mixed path layouts 0/1, 2 and 3 differ
+ * in rounding only (patch-relative FP32 node values, FP64 partial sums). */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 

@@ -74,12 +74,13 @@ def test_patch_linear_elastic_explicit_gpu(name, precision):
         assert abs(ekin) < 1e-12 * model.get_energy("potential")
 
 
-@pytest.mark.parametrize("layout", [0, 1])
-def test_bar_c0_fp64_bitwise_1000_steps(layout):
+@pytest.mark.parametrize("layout,recompute", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_bar_c0_fp64_bitwise_1000_steps(layout, recompute):
     """c0 (4,000 triangle_3, 2,121 nodes), 1000 steps: FP64 GPU == oracle, with
-    element-ordered (0) and node-ordered (1) force slots."""
+    element-ordered (0) and node-ordered (1) force slots, B and det(J) w read
+    from the tables or rebuilt from the coordinates."""
     p = gh.bar_pair()
-    p.gpu.set_launch(slot_layout=layout)
+    p.gpu.set_launch(mixed_recompute=recompute, slot_layout=layout)
     p.step(1000, akx.FP64)
     for g, c in (("displacement", "u"), ("velocity", "v"), ("acceleration", "a"),
                  ("internal_force", "f_int")):
