@@ -44,6 +44,11 @@ SIGNATURES = {
     "akx_init": (_I, [_P]),
     "akx_field_upload": (_I, [_P, _I, _P, _SZ]),
     "akx_field_download": (_I, [_P, _I, _P, _SZ]),
+    "akx_host_alloc": (_I, [_SZ, ctypes.POINTER(_P)]),
+    "akx_host_free": (_I, [_P]),
+    "akx_field_download_async": (_I, [_P, _I, _P, _SZ, ctypes.POINTER(ctypes.c_uint64)]),
+    "akx_download_query": (_I, [_P, ctypes.c_uint64, ctypes.POINTER(_I)]),
+    "akx_download_wait": (_I, [_P, ctypes.c_uint64]),
     "akx_field_device_pointer": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "akx_node_numbering": (_I, [_P, _P]),
     "akx_set_time_step": (_I, [_P, _D]),
@@ -94,6 +99,34 @@ lib = _load()
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class PinnedBuffer:
+    """Page-locked host memory (akx_host_alloc) for asynchronous downloads."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        if lib.akx_host_alloc(self.nbytes, ctypes.byref(p)) != 0:
+            raise AkxError("akx_host_alloc(%d) failed" % self.nbytes)
+        self.ptr = p
+
+    def array(self, dtype, shape):
+        """numpy view of the buffer (valid while the buffer lives)."""
+        dtype = np.dtype(dtype)
+        count = int(np.prod(shape))
+        if count * dtype.itemsize > self.nbytes:
+            raise ValueError("view larger than the buffer")
+        raw = (ctypes.c_char * (count * dtype.itemsize)).from_address(self.ptr.value)
+        return np.frombuffer(raw, dtype=dtype, count=count).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            lib.akx_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:
@@ -287,9 +320,11 @@ class SolidMechanicsModel:
         return v.value
 
     # -- output (SolidMechanicsModel::setBaseName / addDumpField / dump) --
-    def set_base_name(self, name, directory="paraview"):
+    def set_base_name(self, name, directory="paraview", asynchronous=True):
         from .dumper import Dumper
-        self._dumper = Dumper(self, name, directory)
+        if getattr(self, "_dumper", None) is not None:
+            self._dumper.close()
+        self._dumper = Dumper(self, name, directory, asynchronous)
 
     def add_dump_field(self, name):
         if getattr(self, "_dumper", None) is None:
@@ -321,6 +356,32 @@ class SolidMechanicsModel:
             return out.astype(bool)
         return out
 
+    def field_layout(self, name):
+        """(shape, dtype) of a field as get() returns it before any conversion."""
+        return self._shape(name), (np.uint8 if name == "blocked_dofs" else np.float64)
+
+    def get_async(self, name, buf):
+        """Snapshot `name` now; its bytes arrive in `buf` (a PinnedBuffer) while
+        later steps run.  Returns a ticket for wait()."""
+        shape, dtype = self.field_layout(name)
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        if buf.nbytes < nbytes:
+            raise ValueError("pinned buffer too small for %s" % name)
+        t = ctypes.c_uint64()
+        self.ctx.check(lib.akx_field_download_async(self.ctx.h, FIELDS[name], buf.ptr, nbytes,
+                                                    ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Block until the asynchronous download `ticket` has landed (may be
+        called from another thread than the one that steps the model)."""
+        self.ctx.check(lib.akx_download_wait(self.ctx.h, ticket))
+
+    def done(self, ticket):
+        flag = ctypes.c_int()
+        self.ctx.check(lib.akx_download_query(self.ctx.h, ticket, ctypes.byref(flag)))
+        return bool(flag.value)
+
     def set(self, name, values):
         dtype = np.uint8 if name == "blocked_dofs" else np.float64
         arr = np.ascontiguousarray(np.broadcast_to(np.asarray(values).astype(dtype),
@@ -346,5 +407,13 @@ class SolidMechanicsModel:
     def get_blocked_dofs(self):
         return self.get("blocked_dofs")
 
+    def flush_dumps(self):
+        """Wait until every dump so far is written."""
+        if getattr(self, "_dumper", None) is not None:
+            self._dumper.flush()
+
     def close(self):
+        if getattr(self, "_dumper", None) is not None:
+            self._dumper.close()
+            self._dumper = None
         self.ctx.close()

@@ -72,6 +72,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->lconn); dfree(c->lrank);
   dfree(c->pcsr_off); dfree(c->pentry);
   dfree(c->tab64_d);
+  dfree(c->node_inv_d); dfree(c->elem_inv_d);
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
@@ -466,11 +467,9 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
       }
     poff[b + 1] = poff[b] + uint32_t(m);
   }
-  // node state (u | X, FP64), run starts, run bases, then the forces with
-  // 4 floats per 3D contribution (2 in 2D) -- the layout of k_element_patch
   // LDS of k_element_patch: the node lists (patch_force_offset) and the
-  // forces, d floats per contribution
-  const size_t lds = patch_force_offset(maxm, d) + size_t(PATCH) * n * d * 4;
+  // forces, d floats per contribution, plus the run-sum padding
+  const size_t lds = patch_force_offset(maxm, d) + size_t(PATCH * n + RUNCHUNK - 1) * d * 4;
   if (lds > 64 * 1024) return AKX_OK;
   // per-patch lists padded to the stride S = maxm; padding repeats the
   // patch's first node (a valid address) and its run data
@@ -564,6 +563,13 @@ int akx_ctx_destroy(akx_ctx* c) {
   for (auto& e : c->pool) (void)hipEventDestroy(e);
   for (auto& e : c->net_ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  for (auto& sl : c->aslots) {
+    dfree(sl.buf);
+    if (sl.snap) (void)hipEventDestroy(sl.snap);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AKX_OK;
@@ -1004,6 +1010,151 @@ int akx_field_download(akx_ctx* c, int field, void* host, size_t bytes) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   to_caller_nodal(c, tmp.data(), static_cast<double*>(host));
   return AKX_OK;
+}
+
+int akx_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return AKX_ERR_ARGUMENT;
+  *ptr = nullptr;
+  if (bytes == 0) return AKX_ERR_ARGUMENT;
+  return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? AKX_OK : AKX_ERR_HIP;
+}
+
+int akx_host_free(void* ptr) {
+  if (!ptr) return AKX_OK;
+  return hipHostFree(ptr) == hipSuccess ? AKX_OK : AKX_ERR_HIP;
+}
+
+int akx_field_download_async(akx_ctx* c, int field, void* host, size_t bytes, uint64_t* ticket) {
+  if (!c || !ticket) return AKX_ERR_ARGUMENT;
+  *ticket = 0;
+  if (!c->initialized) return fail(c, AKX_ERR_STATE, "akx_init has not been called");
+  if (!host) return fail(c, AKX_ERR_ARGUMENT, "null host pointer");
+  (void)hipSetDevice(c->device);
+  // caller -> internal numbering on the device, built on first use
+  if (c->reorder && !c->node_inv_d) {
+    HIP_TRY(c, dalloc(&c->node_inv_d, c->node_inv.size()));
+    HIP_TRY(c, hipMemcpy(c->node_inv_d, c->node_inv.data(), c->node_inv.size() * 4,
+                         hipMemcpyHostToDevice));
+    if (!c->elem_inv.empty()) {
+      HIP_TRY(c, dalloc(&c->elem_inv_d, c->elem_inv.size()));
+      HIP_TRY(c, hipMemcpy(c->elem_inv_d, c->elem_inv.data(), c->elem_inv.size() * 4,
+                           hipMemcpyHostToDevice));
+    }
+  }
+  const uint32_t* ninv = c->reorder ? c->node_inv_d : nullptr;
+  const uint32_t* einv = c->reorder ? c->elem_inv_d : nullptr;
+  const int d = c->d;
+  const void* src = nullptr;
+  int64_t rows = c->N;
+  int dw = d, sw = d, eb = 8;
+  const uint32_t* inv = ninv;
+  switch (field) {
+  case AKX_BLOCKED_DOFS: src = c->blocked; eb = 1; break;
+  case AKX_MASS: src = c->mass; sw = 1; break;
+  case AKX_CURRENT_POSITION:
+    HIP_TRY(c, launch_add(c->stream, c->N * d, c->X, c->u, c->work));
+    src = c->work;
+    break;
+  case AKX_DAMAGE:
+    if (!c->damage) return fail(c, AKX_ERR_STATE, "no damage material");
+    src = c->damage; rows = c->E; dw = sw = c->Q; inv = einv;
+    break;
+  case AKX_GRAD_U:
+  case AKX_STRESS: {
+    if (int rc = check_ready(c)) return rc;
+    const size_t ne = field_elems(c, field);
+    if (!c->gradu) {
+      HIP_TRY(c, dalloc(&c->gradu, ne));
+      HIP_TRY(c, dalloc(&c->stress, ne));
+    }
+    for (auto& g : c->groups) {
+      ElemArgs A = elem_args(c, g, AKX_FP64);
+      HIP_TRY(c, launch_point_fields(c->et, g.kind, c->stream, A, c->gradu, c->stress));
+    }
+    src = field == AKX_GRAD_U ? c->gradu : c->stress;
+    rows = c->E; dw = sw = c->Q * d * d; inv = einv;
+    break;
+  }
+  case AKX_SHAPES_DERIVATIVES:
+  case AKX_JACOBIANS:
+    return fail(c, AKX_ERR_ARGUMENT, "field not available asynchronously (akx_field_download)");
+  default:
+    src = nodal_ptr(c, field);
+    if (!src) return fail(c, AKX_ERR_ARGUMENT, "unknown field");
+  }
+  const size_t need = size_t(rows) * dw * eb;
+  if (bytes != need) return fail(c, AKX_ERR_ARGUMENT, "size mismatch");
+  if (!c->copy_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  std::lock_guard<std::mutex> lk(c->amu);
+  // a free slot (its copy done), else a new one, else wait for the oldest
+  constexpr size_t MAX_SLOTS = 16;
+  int pick = -1;
+  for (size_t k = 0; k < c->aslots.size() && pick < 0; ++k) {
+    auto& sl = c->aslots[k];
+    if (sl.busy && hipEventQuery(sl.done) == hipSuccess) sl.busy = false;
+    if (!sl.busy) pick = int(k);
+  }
+  if (pick < 0 && c->aslots.size() < MAX_SLOTS) {
+    c->aslots.emplace_back();
+    auto& sl = c->aslots.back();
+    HIP_TRY(c, hipEventCreateWithFlags(&sl.snap, hipEventDisableTiming));
+    HIP_TRY(c, hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    pick = int(c->aslots.size()) - 1;
+  }
+  if (pick < 0) {
+    pick = 0;
+    for (size_t k = 1; k < c->aslots.size(); ++k)
+      if (c->aslots[k].ticket < c->aslots[pick].ticket) pick = int(k);
+    HIP_TRY(c, hipEventSynchronize(c->aslots[pick].done));
+    c->aslots[pick].busy = false;
+  }
+  auto& sl = c->aslots[pick];
+  if (sl.cap < need) {
+    dfree(sl.buf);
+    sl.cap = 0;
+    HIP_TRY(c, hipMalloc(&sl.buf, need));
+    sl.cap = need;
+  }
+  HIP_TRY(c, launch_snapshot(c->stream, rows, dw, sw, inv, src, sl.buf, eb));
+  HIP_TRY(c, hipEventRecord(sl.snap, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->copy_stream, sl.snap, 0));
+  HIP_TRY(c, hipMemcpyAsync(host, sl.buf, need, hipMemcpyDeviceToHost, c->copy_stream));
+  HIP_TRY(c, hipEventRecord(sl.done, c->copy_stream));
+  sl.ticket = c->next_ticket++;
+  sl.busy = true;
+  *ticket = sl.ticket;
+  return AKX_OK;
+}
+
+int akx_download_query(akx_ctx* c, uint64_t ticket, int* done) {
+  if (!c || !done) return AKX_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> lk(c->amu);
+  if (ticket == 0 || ticket >= c->next_ticket) return AKX_ERR_ARGUMENT;
+  *done = 1;
+  for (auto& sl : c->aslots)
+    if (sl.busy && sl.ticket == ticket) {
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q != hipSuccess && q != hipErrorNotReady) return AKX_ERR_HIP;
+      *done = q == hipSuccess;
+    }
+  return AKX_OK;
+}
+
+int akx_download_wait(akx_ctx* c, uint64_t ticket) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  hipEvent_t ev = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->amu);
+    if (ticket == 0 || ticket >= c->next_ticket) return AKX_ERR_ARGUMENT;
+    for (auto& sl : c->aslots)
+      if (sl.busy && sl.ticket == ticket) ev = sl.done;
+  }
+  // not found: completed and its slot recycled.  A slot is recycled only
+  // after its copy completed, so waiting on a re-recorded event waits for a
+  // later copy, never for less than this one.
+  if (!ev) return AKX_OK;
+  (void)hipSetDevice(c->device);
+  return hipEventSynchronize(ev) == hipSuccess ? AKX_OK : AKX_ERR_HIP;
 }
 
 int akx_field_device_pointer(akx_ctx* c, int field, void** ptr) {

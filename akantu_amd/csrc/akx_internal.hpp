@@ -5,6 +5,7 @@
 
 #include <stdint.h>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "akx_device.hpp"
@@ -104,6 +105,9 @@ enum NodeMode {
 
 // Byte offset of the forces in the LDS of k_element_patch with list stride S:
 // S nodes x 2d floats, S run starts (u32), S + 1 run bases (u16), 16-aligned.
+// Entries the patch kernel reads per step when it sums a node's run; the LDS
+// force area carries RUNCHUNK - 1 entries of padding after its end.
+constexpr int RUNCHUNK = 4;
 inline __host__ __device__ size_t patch_force_offset(int S, int d) {
   return (size_t(S) * (2 * d * 4 + 4) + size_t(S + 1) * 2 + 15) & ~size_t(15);
 }
@@ -142,6 +146,8 @@ hipError_t launch_energy(int kind_energy, int et, int mat_kind_uniform, hipStrea
 hipError_t launch_point_fields(int et, int kind, hipStream_t s, const ElemArgs& A, double* gradu,
                                double* stress);
 hipError_t launch_add(hipStream_t s, int64_t n, const double* x, const double* y, double* out);
+hipError_t launch_snapshot(hipStream_t s, int64_t rows, int dw, int sw, const uint32_t* inv,
+                           const void* src, void* dst, int elem_bytes);
 
 // The constitutive surrogate of one context: training set, FP32 master
 // weights with Adam moments, BF16 working copies, batch buffers
@@ -296,6 +302,23 @@ struct akx_ctx {
   hipGraphExec_t graph_exec = nullptr;
 
   akx::Surrogate sur;
+
+  // asynchronous downloads: a device snapshot taken on `stream`, copied to
+  // page-locked host memory on `copy_stream`; slots are recycled once their
+  // copy has completed.  `amu` guards the slot table (akx_download_wait may be
+  // called from another host thread).
+  struct AsyncSlot {
+    void* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t snap = nullptr, done = nullptr;
+    uint64_t ticket = 0;
+    bool busy = false;
+  };
+  std::vector<AsyncSlot> aslots;
+  hipStream_t copy_stream = nullptr;
+  uint64_t next_ticket = 1;
+  std::mutex amu;
+  uint32_t *node_inv_d = nullptr, *elem_inv_d = nullptr;  // caller -> internal
 
   std::vector<hipEvent_t> pool;  // per-step timing events
   bool timing = false;

@@ -245,6 +245,69 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
   const float* __restrict__ Jt = static_cast<const float*>(A.J);
 #pragma unroll
   for (int k = 0; k < dn; ++k) fe[k] = 0.f;
+  if constexpr (simplex && RECOMPUTE && KIND == ELASTIC) {
+    // Linear simplex, one point, linear law: with A = adj(J),
+    // B_a = A(:, a-1) / det for a > 0, grad u = G' / det where
+    // G' = sum_a du_a A(:, a-1)^T, and det(J) w cancels one 1/det:
+    //   f_a = (w / det) sigma(G') A(:, a-1),   f_0 = -(f_1 + ... + f_{n-1}).
+    float Ad[d * d], det;
+    if constexpr (d == 2) {
+      // J(i,j) = dX[j + (i+1) d]
+      const float j00 = dX[2], j10 = dX[4], j01 = dX[3], j11 = dX[5];
+      Ad[0] = j11; Ad[1] = -j10; Ad[2] = -j01; Ad[3] = j00;
+      det = j00 * Ad[0] + j01 * Ad[1];
+    } else {
+      float J[9];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) J[i + j * 3] = dX[j + (i + 1) * 3];
+      Ad[0] = J[4] * J[8] - J[7] * J[5];
+      Ad[1] = J[2] * J[7] - J[8] * J[1];
+      Ad[2] = J[1] * J[5] - J[4] * J[2];
+      Ad[3] = J[5] * J[6] - J[8] * J[3];
+      Ad[4] = J[0] * J[8] - J[6] * J[2];
+      Ad[5] = J[2] * J[3] - J[5] * J[0];
+      Ad[6] = J[3] * J[7] - J[6] * J[4];
+      Ad[7] = J[1] * J[6] - J[7] * J[0];
+      Ad[8] = J[0] * J[4] - J[3] * J[1];
+      det = J[0] * Ad[0] + J[3] * Ad[1] + J[6] * Ad[2];
+    }
+    const float s = A.tabf.w[0] * __builtin_amdgcn_rcpf(det);
+    float G[d * d];
+#pragma unroll
+    for (int j = 0; j < d; ++j)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float g = 0.f;
+#pragma unroll
+        for (int a = 1; a < n; ++a) g += du[i + a * d] * Ad[j + (a - 1) * d];
+        G[i + j * d] = g;
+      }
+    float tr = 0.f;
+#pragma unroll
+    for (int i = 0; i < d; ++i) tr += G[i + i * d];
+    float P[d * d];
+#pragma unroll
+    for (int j = 0; j < d; ++j)
+#pragma unroll
+      for (int i = 0; i <= j; ++i) {
+        float v = m.mu_f * (G[i + j * d] + G[j + i * d]);
+        if (i == j) v += m.lambda_f * tr;
+        P[i + j * d] = P[j + i * d] = v * s;
+      }
+#pragma unroll
+    for (int a = 1; a < n; ++a)
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        float f = 0.f;
+#pragma unroll
+        for (int j = 0; j < d; ++j) f += P[i + j * d] * Ad[j + (a - 1) * d];
+        fe[i + a * d] = f;
+        fe[i] -= f;
+      }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     float Bq[dn];
@@ -716,9 +779,19 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
       double f[d];
 #pragma unroll
       for (int i = 0; i < d; ++i) f[i] = 0.;
-      for (int q = s_rb[k]; q < s_rb[k + 1]; ++q)
+      // chunks of RUNCHUNK entries: a chunk's LDS reads are issued together
+      // (the force area has RUNCHUNK - 1 entries of padding after its end);
+      // entries past the run add +0, so the sum is the sequential one
+      const int q1 = s_rb[k + 1];
+      for (int q = s_rb[k]; q < q1; q += RUNCHUNK) {
+        float v[RUNCHUNK * d];
 #pragma unroll
-        for (int i = 0; i < d; ++i) f[i] += double(s_f[q * d + i]);
+        for (int j = 0; j < RUNCHUNK * d; ++j) v[j] = s_f[q * d + j];
+#pragma unroll
+        for (int j = 0; j < RUNCHUNK; ++j)
+#pragma unroll
+          for (int i = 0; i < d; ++i) f[i] += double(q + j < q1 ? v[j * d + i] : 0.f);
+      }
 #pragma unroll
       for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = f[i];
     }
@@ -1464,6 +1537,41 @@ hipError_t launch_point_fields(int et, int kind, hipStream_t s, const ElemArgs& 
   AKX_ET_SWITCH(et, F)
 #undef F
 #undef G
+  return hipGetLastError();
+}
+
+// Snapshot of a per-row device array in the caller's numbering, for the
+// asynchronous downloads: dst[r * dw + j] = src[inv[r] * sw + (sw == 1 ? 0 : j)]
+// (rows = nodes or elements; inv = caller -> internal, nullptr = identity;
+// sw = 1 broadcasts one value per row, as the lumped mass is reported per dof).
+template <typename T>
+__global__ void __launch_bounds__(256) k_snapshot(int64_t rows, int dw, int sw,
+                                                  const uint32_t* __restrict__ inv,
+                                                  const T* __restrict__ src, T* __restrict__ dst,
+                                                  int mask) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= rows * dw) return;
+  const int64_t r = k / dw;
+  const int j = int(k - r * dw);
+  const int64_t from = (inv ? int64_t(inv[r]) : r) * sw + (sw == 1 ? 0 : j);
+  T v = src[from];
+  if constexpr (sizeof(T) == 1) v = T(v & mask);
+  dst[k] = v;
+}
+
+hipError_t launch_snapshot(hipStream_t s, int64_t rows, int dw, int sw, const uint32_t* inv,
+                           const void* src, void* dst, int elem_bytes) {
+  clear_error();
+  const int64_t n = rows * dw;
+  if (n == 0) return hipSuccess;
+  if (elem_bytes == 8)
+    k_snapshot<double><<<grid_for(n, 256), 256, 0, s>>>(rows, dw, sw, inv,
+                                                        static_cast<const double*>(src),
+                                                        static_cast<double*>(dst), 0);
+  else
+    k_snapshot<uint8_t><<<grid_for(n, 256), 256, 0, s>>>(rows, dw, sw, inv,
+                                                         static_cast<const uint8_t*>(src),
+                                                         static_cast<uint8_t*>(dst), 1);
   return hipGetLastError();
 }
 

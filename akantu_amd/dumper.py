@@ -8,10 +8,19 @@ src/model/model.cc and src/io/dumper/), which writes nodal fields
 `paraview/<base>_<step>.vtu`.  Nodal fields are point data; per-point fields
 of elements with several quadrature points are averaged per element as cell
 data.  Files are ASCII so that they can be read without any VTK library.
+
+Asynchronous by default: dump() takes a device snapshot of every field
+(akx_field_download_async), and one writer thread waits for the copies and
+writes the files while the caller keeps stepping.  Page-locked buffers are
+recycled once their file is written; at most `depth` dumps are in flight
+(dump() waits for the oldest beyond that).  flush() waits for all of them.
 """
 from __future__ import annotations
 
+import collections
+import concurrent.futures
 import os
+import threading
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -86,13 +95,19 @@ def write_pvd(path, entries):
 class Dumper:
     """model.set_base_name / add_dump_field / dump, for akx.SolidMechanicsModel."""
 
-    def __init__(self, model, base_name, directory="paraview"):
+    def __init__(self, model, base_name, directory="paraview", asynchronous=True, depth=2):
         self.model = model
         self.base = base_name
         self.dir = directory
         self.fields = []
         self.count = 0
         self.entries = []
+        self.asynchronous = asynchronous
+        self.depth = max(1, int(depth))
+        self._pool = None
+        self._pending = collections.deque()
+        self._free = collections.defaultdict(list)  # field -> idle PinnedBuffers
+        self._lock = threading.Lock()
 
     def add_dump_field(self, name):
         if name not in NODAL + POINTWISE:
@@ -100,21 +115,76 @@ class Dumper:
         if name not in self.fields:
             self.fields.append(name)
 
-    def dump(self, time=None):
+    def _data(self, arrays):
+        """point and cell data from the fields' arrays (as model.get returns them)."""
         m = self.model.mesh
         pd, cd = {}, {}
-        for f in self.fields:
-            a = self.model.get(f)
+        for f, a in arrays:
             if f in NODAL:
                 pd[f] = a.astype(np.uint8) if a.dtype == bool else a
             else:
-                Q = m.nb_qp
-                per = a.reshape(m.nb_elements, Q, -1).mean(axis=1)
+                per = a.reshape(m.nb_elements, m.nb_qp, -1).mean(axis=1)
                 cd[f] = per[:, 0] if per.shape[1] == 1 else per
+        return pd, cd
+
+    def _write(self, name, arrays, entries):
+        write_vtu(os.path.join(self.dir, name), self.model.mesh, *self._data(arrays))
+        write_pvd(os.path.join(self.dir, self.base + ".pvd"), entries)
+
+    def dump(self, time=None):
         name = "%s_%04d.vtu" % (self.base, self.count)
-        write_vtu(os.path.join(self.dir, name), m, pd, cd)
         t = self.count * self.model.time_step if time is None else time
         self.entries.append((t, name))
-        write_pvd(os.path.join(self.dir, self.base + ".pvd"), self.entries)
         self.count += 1
-        return os.path.join(self.dir, name)
+        path = os.path.join(self.dir, name)
+        if not self.asynchronous:
+            self._write(name, [(f, self.model.get(f)) for f in self.fields], list(self.entries))
+            return path
+        from .akx import PinnedBuffer
+        while len(self._pending) >= self.depth:
+            self._pending.popleft().result()
+        jobs = []
+        for f in self.fields:
+            shape, dtype = self.model.field_layout(f)
+            nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+            with self._lock:
+                free = self._free[f]
+                buf = free.pop() if free else None
+            if buf is None or buf.nbytes < nbytes:
+                buf = PinnedBuffer(nbytes)
+            jobs.append((f, buf, self.model.get_async(f, buf), shape, dtype))
+        if self._pool is None:
+            self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self._pending.append(self._pool.submit(self._write_async, name, jobs, list(self.entries)))
+        return path
+
+    def _write_async(self, name, jobs, entries):
+        arrays = []
+        for f, buf, ticket, shape, dtype in jobs:
+            self.model.wait(ticket)
+            a = buf.array(dtype, shape)
+            arrays.append((f, a.astype(bool) if f == "blocked_dofs" else a))
+        try:
+            self._write(name, arrays, entries)
+        finally:
+            with self._lock:
+                for f, buf, _, _, _ in jobs:
+                    self._free[f].append(buf)
+
+    def flush(self):
+        """Wait until every dump so far is on disk (re-raises a writer error)."""
+        while self._pending:
+            self._pending.popleft().result()
+
+    def close(self):
+        try:
+            self.flush()
+        finally:
+            if self._pool is not None:
+                self._pool.shutdown(wait=True)
+                self._pool = None
+            with self._lock:
+                for bufs in self._free.values():
+                    for b in bufs:
+                        b.close()
+                self._free.clear()

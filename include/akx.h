@@ -137,6 +137,23 @@ int akx_init(akx_ctx* ctx);
  * solid_mechanics_model.hh:363-397).  bytes must equal the field size. */
 int akx_field_upload(akx_ctx* ctx, int field, const void* host, size_t bytes);
 int akx_field_download(akx_ctx* ctx, int field, void* host, size_t bytes);
+/* Asynchronous output (the reference's dumpers read the same arrays
+ * synchronously: Model::dump, src/model/model.cc, through
+ * src/io/dumper/dumper_iohelper.cc).  akx_field_download_async snapshots
+ * `field` as it stands after the work already enqueued on the context's
+ * stream -- in the caller's numbering and layout, the same bytes
+ * akx_field_download writes -- and copies it to `host` on a second stream
+ * while later steps run.  `host` must be page-locked (akx_host_alloc) and
+ * left untouched until akx_download_wait(ticket) has returned or
+ * akx_download_query reports it done.  Not for the static tables
+ * (AKX_SHAPES_DERIVATIVES, AKX_JACOBIANS).  akx_download_wait and
+ * akx_download_query may be called from another host thread than the one
+ * that steps the context. */
+int akx_host_alloc(size_t bytes, void** host);
+int akx_host_free(void* host);
+int akx_field_download_async(akx_ctx* ctx, int field, void* host, size_t bytes, uint64_t* ticket);
+int akx_download_query(akx_ctx* ctx, uint64_t ticket, int* done);
+int akx_download_wait(akx_ctx* ctx, uint64_t ticket);
 /* Device address of a nodal field (for zero-copy interop on the same device).
  * Device arrays are in the internal node numbering (see akx_node_numbering). */
 int akx_field_device_pointer(akx_ctx* ctx, int field, void** device_ptr);

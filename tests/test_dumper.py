@@ -62,11 +62,85 @@ def test_model_dump_writes_the_device_state(tmp_path):
         model.add_dump_field(f)
     model.solve_step(20, akx.FP64)
     path = model.dump()
+    model.flush_dumps()
     piece, arr = _arrays(path)
     assert np.array_equal(arr["displacement"][:, :2], model.get_displacement())
     assert np.array_equal(arr["velocity"][:, :2], model.get_velocity())
     assert np.array_equal(arr["stress"], model.get("stress"))
     model.solve_step(20, akx.FP64)
     model.dump()
+    model.flush_dumps()
     ds = ET.parse(tmp_path / "paraview" / "bar.pvd").getroot().findall("Collection/DataSet")
     assert len(ds) == 2
+    model.close()
+
+
+def _hex_damage_model():
+    from akantu_amd import akx
+    m = akm.box_hex8(4, 3, 2)
+    model = akx.SolidMechanicsModel(m)
+    model.add_material(kind=akx.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1000.0, Sd=1e4,
+                       Yd=np.full(m.nb_elements * m.nb_qp, 50.0))
+    model.init_full()
+    rng = np.random.default_rng(1)
+    model.set("velocity", rng.normal(scale=0.5, size=m.nodes.shape))
+    blocked = np.zeros(m.nodes.shape, dtype=bool)
+    blocked[m.nodes[:, 2] == 0.0] = True
+    model.set("blocked_dofs", blocked)
+    model.set_time_step(0.5 * model.get_stable_time_step())
+    return model
+
+
+ASYNC_FIELDS = ("displacement", "velocity", "acceleration", "internal_force", "mass",
+                "blocked_dofs", "current_position", "grad_u", "stress", "damage")
+
+
+@pytest.mark.gpu
+def test_async_download_is_a_snapshot_of_the_enqueued_state():
+    """Every asynchronous field equals the synchronous download taken at the
+    same point, although more steps are enqueued before the copy is awaited."""
+    from akantu_amd import akx
+    model = _hex_damage_model()
+    model.solve_step(30, akx.FP64)
+    ref = {f: model.get(f) for f in ASYNC_FIELDS}
+    bufs, tickets = {}, {}
+    for f in ASYNC_FIELDS:
+        shape, dtype = model.field_layout(f)
+        bufs[f] = akx.PinnedBuffer(int(np.prod(shape)) * np.dtype(dtype).itemsize)
+        tickets[f] = model.get_async(f, bufs[f])
+    model.solve_step(30, akx.FP64)          # enqueued behind the snapshots
+    for f in ASYNC_FIELDS:
+        model.wait(tickets[f])
+        assert model.done(tickets[f])
+        shape, dtype = model.field_layout(f)
+        got = bufs[f].array(dtype, shape)
+        want = ref[f].astype(np.uint8) if f == "blocked_dofs" else ref[f]
+        assert np.array_equal(got, want), f
+    assert not np.array_equal(model.get_displacement(), ref["displacement"])
+    with pytest.raises(akx.AkxError):
+        model.get_async("jacobians", bufs["damage"])
+    for b in bufs.values():
+        b.close()
+    model.close()
+
+
+@pytest.mark.gpu
+def test_async_dumps_equal_synchronous_dumps(tmp_path):
+    """Dumps written by the writer thread while the model keeps stepping are
+    the files the synchronous dumper writes at the same steps."""
+    from akantu_amd import akx
+    texts = {}
+    for mode in (False, True):
+        model = _hex_damage_model()
+        model.set_base_name("h", str(tmp_path / str(mode)), asynchronous=mode)
+        for f in ("displacement", "velocity", "blocked_dofs", "stress", "damage"):
+            model.add_dump_field(f)
+        model.dump()
+        for _ in range(5):                  # more dumps than the pinned-buffer depth
+            model.solve_step(10, akx.FP64)
+            model.dump()
+        model.flush_dumps()
+        texts[mode] = [(tmp_path / str(mode) / ("h_%04d.vtu" % k)).read_text() for k in range(6)]
+        texts[mode].append((tmp_path / str(mode) / "h.pvd").read_text())
+        model.close()
+    assert texts[True] == texts[False]
