@@ -69,7 +69,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->slot_pos);
   dfree(c->csr2_off); dfree(c->csr2_slot); dfree(c->slot2_pos);
   dfree(c->pnode_cnt); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
-  dfree(c->lconn); dfree(c->lrank);
+  dfree(c->lconn); dfree(c->lrank); dfree(c->lcr);
   dfree(c->pcsr_off); dfree(c->pentry);
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
@@ -278,11 +278,14 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
     std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
 }
 
-// Mixed path with per-patch partial sums (slot layout 3): the element kernel
-// reduces each node's contributions inside its patch, the node kernel sums
-// FP64 partials.  Needs the LDS patches and a single launch group.
+// Mixed path with per-patch partial sums (slot layouts 3 and 4): the element
+// kernel reduces each node's contributions inside its patch in FP64 and
+// stores one FP64 (3) or FP32 (4) partial per (patch, node); the node kernel
+// sums a node's partials in FP64.  Needs the LDS patches and a single launch
+// group.
 bool uses_partials(const akx_ctx* c, int precision) {
-  return precision == AKX_MIXED && c->node_ordered == 3 && c->patch_ok && c->groups.size() == 1 &&
+  return precision == AKX_MIXED && (c->node_ordered == 3 || c->node_ordered == 4) &&
+         c->patch_ok && c->groups.size() == 1 &&
          !c->groups[0].filter &&
          !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
          // the partials (padded stride) live in the slot buffer
@@ -313,7 +316,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.X = c->X;
   A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
   A.patch = (precision == AKX_MIXED && c->node_ordered >= 2 && c->patch_ok && !g.filter) ? 1 : 0;
-  A.partials = (A.patch && uses_partials(c, precision)) ? 1 : 0;
+  A.partials = (A.patch && uses_partials(c, precision)) ? (c->node_ordered == 4 ? 2 : 1) : 0;
   A.patch_max_nodes = c->patch_max_nodes;
   A.patch_lds = c->patch_lds;
   A.patch_blocks = c->patch_blocks;
@@ -323,6 +326,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.run_base = c->run_base;
   A.lconn = c->lconn;
   A.lrank = c->lrank;
+  A.lcr = c->lcr;
   A.tabf = c->tabf;
   A.tab64 = c->tab64_d;
   A.sur_img = nullptr;
@@ -368,8 +372,8 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.stage_lds = 0;
   A.partials = 0;
   if (uses_partials(c, precision)) {
-    A.partials = 1;
-    // one FP64 partial sum per (patch, node): a short gather per node
+    A.partials = c->node_ordered == 4 ? 2 : 1;
+    // one partial sum per (patch, node): a short gather per node
     A.csr_off = c->pcsr_off;
     A.csr_slot = c->pentry;
     A.node_ordered = 0;
@@ -492,14 +496,27 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
   HIP_TRY(c, dalloc(&c->pnode_ids, pids_p.size()));
   HIP_TRY(c, dalloc(&c->run_start, rstart_p.size()));
   HIP_TRY(c, dalloc(&c->run_base, rbase_p.size()));
-  HIP_TRY(c, dalloc(&c->lconn, lconn.size()));
-  HIP_TRY(c, dalloc(&c->lrank, lrank.size()));
+  // patch-local node and rank packed in 16 bits per (e, a) when every patch
+  // has at most 256 nodes: one vector load per element
+  if (maxm <= 256) {
+    const int np = n == 3 ? 4 : n;
+    std::vector<uint16_t> lcr(size_t(E) * np, 0);
+    for (int64_t e = 0; e < E; ++e)
+      for (int a = 0; a < n; ++a)
+        lcr[size_t(e) * np + a] =
+            uint16_t(lconn[size_t(a) * E + e] | (uint16_t(lrank[size_t(a) * E + e]) << 8));
+    HIP_TRY(c, dalloc(&c->lcr, lcr.size()));
+    HIP_TRY(c, hipMemcpy(c->lcr, lcr.data(), lcr.size() * 2, hipMemcpyHostToDevice));
+  } else {
+    HIP_TRY(c, dalloc(&c->lconn, lconn.size()));
+    HIP_TRY(c, dalloc(&c->lrank, lrank.size()));
+    HIP_TRY(c, hipMemcpy(c->lconn, lconn.data(), lconn.size() * 2, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->lrank, lrank.data(), lrank.size(), hipMemcpyHostToDevice));
+  }
   HIP_TRY(c, hipMemcpy(c->pnode_cnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->pnode_ids, pids_p.data(), pids_p.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->run_start, rstart_p.data(), rstart_p.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->run_base, rbase_p.data(), rbase_p.size() * 2, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->lconn, lconn.data(), lconn.size() * 2, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->lrank, lrank.data(), lrank.size(), hipMemcpyHostToDevice));
   // node -> patch-node entries b * S + k (ascending patch), for the per-patch
   // partial sums
   {
@@ -1622,9 +1639,9 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   struct Shape { int eb, nb, x, rcmp, lay, stage; };
   std::vector<Shape> cand;
   const int blocks[3] = {64, 128, 256};
-  // layouts 2 and 3 (LDS patches) exist on the mixed path only; staging the
-  // node kernel's slot range applies to node-ordered slots (layouts 1, 2)
-  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 4 : 2;
+  // layouts 2, 3 and 4 (LDS patches) exist on the mixed path only; staging
+  // the node kernel's slot range applies to node-ordered slots (layouts 1, 2)
+  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 5 : 2;
   for (int lay = 0; lay < nlay; ++lay)
     for (int stage = 0; stage < ((lay == 1 || lay == 2) ? 2 : 1); ++stage)
       for (int rcmp = (lay >= 2 ? 1 : 0); rcmp < 2; ++rcmp)
@@ -1676,7 +1693,7 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
     return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
-  if (slot_layout > 3) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1, 2 or 3");
+  if (slot_layout > 4) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1, 2, 3 or 4");
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;

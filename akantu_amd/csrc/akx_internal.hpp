@@ -48,7 +48,7 @@ struct ElemArgs {
   const uint32_t* slot_pos;  // [a][E] CSR position of (e, a); nullptr = element-ordered slots
   // LDS-patch variant of the mixed kernel (k_element_patch)
   int patch;                 // 1: use it
-  int partials;              // 1: write one FP64 sum per (patch, node) instead of the runs
+  int partials;              // write one sum per (patch, node) instead of the runs: 1 FP64, 2 FP32
   int patch_max_nodes;       // most distinct nodes of one patch = stride S of the lists
   int patch_lds;             // dynamic LDS bytes per block
   int64_t patch_blocks;      // number of patches
@@ -56,6 +56,9 @@ struct ElemArgs {
   const uint32_t* pnode_ids; // [b * S + k] patch nodes (internal ids, ascending; padded)
   const uint32_t* run_start; // [b * S + k] first mixed-CSR position of the node's run
   const uint16_t* run_base;  // [b * (S + 1) + k] prefix of run lengths (padded)
+  // patch-local node | rank << 8 per (e, a), element-major with stride
+  // lcr_stride (n, 4 for 3-node elements); nullptr: lconn / lrank instead
+  const uint16_t* lcr;
   const uint16_t* lconn;     // [a][E] patch-local node index
   const uint8_t* lrank;      // [a][E] rank of (e, a) inside the node's run
   ElemTablesF tabf;
@@ -91,7 +94,7 @@ struct NodeArgs {
   int xcd_remap;
   int node_ordered;       // slots stored in CSR order (contiguous per node)
   int stage_lds;          // > 0: bytes of LDS for the block's slot range (node-ordered only)
-  int partials;           // slots are FP64 per-patch partial sums (csr_off/csr_slot: node -> entries)
+  int partials;           // slots are per-patch partial sums, 1 FP64, 2 FP32 (csr_off/csr_slot: node -> entries)
 };
 
 enum NodeMode {
@@ -270,7 +273,7 @@ struct akx_ctx {
   int patch_max_nodes = 0, patch_lds = 0;
   int64_t patch_blocks = 0, patch_nodes_total = 0;
   uint32_t *pnode_cnt = nullptr, *pnode_ids = nullptr, *run_start = nullptr;
-  uint16_t *run_base = nullptr, *lconn = nullptr;
+  uint16_t *run_base = nullptr, *lconn = nullptr, *lcr = nullptr;
   uint8_t* lrank = nullptr;
   // node -> entries of the per-patch partial sums (slot layout 3)
   uint32_t *pcsr_off = nullptr, *pentry = nullptr;

@@ -704,10 +704,30 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   const int t = threadIdx.x;
   const int64_t e = e0 + min(t, ne - 1);
   int ln[n], rk[n];
+  if (A.lcr) {
+    // packed: patch-local node | rank << 8, one 8- or 16-byte load per element
+    constexpr int np = n == 3 ? 4 : n;
+    uint32_t w[np / 2];
+    if constexpr (np == 4) {
+      const uint2 v = reinterpret_cast<const uint2*>(A.lcr)[e];
+      w[0] = v.x; w[1] = v.y;
+    } else {
+      static_assert(np == 8, "packed patch indices: 4 or 8 per element");
+      const uint4 v = reinterpret_cast<const uint4*>(A.lcr)[e];
+      w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    }
 #pragma unroll
-  for (int a = 0; a < n; ++a) {
-    ln[a] = A.lconn[a * A.E + e];
-    rk[a] = A.lrank[a * A.E + e];
+    for (int a = 0; a < n; ++a) {
+      const uint32_t h = (w[a >> 1] >> (16 * (a & 1))) & 0xffffu;
+      ln[a] = int(h & 0xffu);
+      rk[a] = int(h >> 8);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      ln[a] = A.lconn[a * A.E + e];
+      rk[a] = A.lrank[a * A.E + e];
+    }
   }
   const int mat = A.elem_mat ? A.elem_mat[e] : 0;
   const uint32_t m_nodes = A.pnode_cnt[b];
@@ -773,8 +793,8 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
   }
   __syncthreads();
   if (A.partials) {
-    // one FP64 sum per patch node, its run summed in element order
-    double* __restrict__ part = static_cast<double*>(A.slots);
+    // one sum per patch node, its run summed in element order in FP64 and
+    // stored in FP64 (partials 1) or rounded once to FP32 (partials 2)
     for (uint32_t k = threadIdx.x; k < m_nodes; k += blockDim.x) {
       double f[d];
 #pragma unroll
@@ -792,8 +812,15 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
 #pragma unroll
           for (int i = 0; i < d; ++i) f[i] += double(q + j < q1 ? v[j * d + i] : 0.f);
       }
+      if (A.partials == 2) {
+        float* __restrict__ part = static_cast<float*>(A.slots);
 #pragma unroll
-      for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = f[i];
+        for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = float(f[i]);
+      } else {
+        double* __restrict__ part = static_cast<double*>(A.slots);
+#pragma unroll
+        for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = f[i];
+      }
     }
     return;
   }
@@ -827,10 +854,11 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
 // scatter).  FP32 slots: FP64 Neumaier-compensated sum.
 // ===========================================================================
 // SRC: what the node gathers -- 0 FP64 element slots, 1 FP32 element slots,
-// 2 FP64 per-patch partial sums.
+// 2 FP64 per-patch partial sums, 3 FP32 per-patch partial sums (FP32 values
+// are summed in FP64 with Neumaier compensation).
 template <int D, int SRC, bool HAS_FEXT>
 __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
-  using TS = typename std::conditional<SRC == 1, float, double>::type;
+  using TS = typename std::conditional<SRC == 1 || SRC == 3, float, double>::type;
   // The block's nodal state is staged through LDS: global loads and stores of
   // u, v, a, f_ext and the blocked flags are contiguous runs of doubles (fully
   // coalesced, each array written once), while each thread works on one node.
@@ -1437,7 +1465,10 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
                                 int block) {
   const unsigned g = grid_for(A.N, block);
   if (precision == 0) {
-    if (A.partials) {
+    if (A.partials == 2) {
+      if (has_fext) k_node<D, 3, true><<<g, block, A.stage_lds, s>>>(A);
+      else k_node<D, 3, false><<<g, block, A.stage_lds, s>>>(A);
+    } else if (A.partials) {
       if (has_fext) k_node<D, 2, true><<<g, block, A.stage_lds, s>>>(A);
       else k_node<D, 2, false><<<g, block, A.stage_lds, s>>>(A);
     } else {

@@ -227,11 +227,12 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * the tables (0) -- the FP64 rebuild repeats the precompute's arithmetic and
  * is bit-identical to the tables --, the layout of the element force slots:
  * element-ordered (0), node-ordered (1), node-ordered written by LDS element
- * patches (2, mixed path) or one FP64 partial sum per (patch, node) (3, mixed
- * path), and whether the node kernel copies its block's contiguous slot
- * range to LDS first (node_stage 1/0) (-1 keeps).  On the FP64 path none of
- * these changes a result bit; on the mixed path layouts 0/1, 2 and 3 differ
- * in rounding only (patch-relative FP32 node values, FP64 partial sums). */
+ * patches (2, mixed path) or one partial sum per (patch, node), summed in
+ * FP64 and stored in FP64 (3, mixed path) or FP32 (4, mixed path), and
+ * whether the node kernel copies its block's contiguous slot range to LDS
+ * first (node_stage 1/0) (-1 keeps).  On the FP64 path none of these changes
+ * a result bit; on the mixed path layouts 0/1, 2, 3 and 4 differ in rounding
+ * only (patch-relative FP32 node values, partial sums). */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 
