@@ -1453,6 +1453,7 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "patch_nodes_total") *value = double(c->patch_ok ? c->patch_nodes_total : 0);
   else if (k == "patch_max_nodes") *value = double(c->patch_ok ? c->patch_max_nodes : 0);
   else if (k == "patch_available") *value = c->patch_ok ? 1. : 0.;
+  else if (k == "patch_packed") *value = (c->patch_ok && c->lcr) ? 1. : 0.;
   else if (k == "element_block") *value = c->elem_block;
   else if (k == "node_block") *value = c->node_block;
   else if (k == "xcd_remap") *value = c->xcd_remap;

@@ -52,7 +52,7 @@ def log(msg):
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
                       patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0,
-                      partials=False):
+                      partials=0, packed=False):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -62,18 +62,21 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     connectivity + 8-bit run ranks per (e, a) and, per patch node, its id,
     run start and run offset (4 + 4 + 2 bytes); dt_per_node: one time step
     per node (ensembles); state_bytes: per-point history read + written;
-    partials: the patch kernel writes one FP64 sum per patch node (no run
-    start needed) and the node kernel gathers those through a node -> entry
-    CSR instead of reading E*n slots."""
+    partials: bytes per value of the one sum per patch node that the patch
+    kernel writes (8 FP64, 4 FP32; no run start needed) and the node kernel
+    gathers through a node -> entry CSR instead of reading E*n slots;
+    packed: local node and run rank in one 16-bit word per (e, a), 3-node
+    elements padded to 4."""
+    local = (E * (4 if n == 3 else n) * 2) if packed else E * n * (2 + 1)
     if partials:
-        elem = (E * n * (2 + 1) + patch_nodes * (4 + 2) + patch_blocks * (4 + 2)
-                + 2 * N * d * 8 + patch_nodes * d * 8 + state_bytes)
-        node = ((N + 1) * 4 + patch_nodes * 4 + patch_nodes * d * 8
+        elem = (local + patch_nodes * (4 + 2) + patch_blocks * (4 + 2)
+                + 2 * N * d * 8 + patch_nodes * d * partials + state_bytes)
+        node = ((N + 1) * 4 + patch_nodes * 4 + patch_nodes * d * partials
                 + 6 * N * d * 8 + N * 8 + N * d
                 + (N * d * 8 if has_fext else 0) + (N * 8 if dt_per_node else 0))
         return elem, node
     if patch_nodes is not None:
-        elem = (E * n * (2 + 1)                            # local connectivity, run ranks
+        elem = (local                                      # local connectivity, run ranks
                 + patch_nodes * (4 + 4 + 2) + patch_blocks * (4 + 2)  # patch node lists
                 + 2 * N * d * 8                            # u and X of every node
                 + E * n * d * pe)                          # force slots (write)
@@ -231,19 +234,21 @@ def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
     """Roofline of the slower of the two mixed-path kernels."""
     lay = shapes["mixed"]["slot_layout"]
     patch = lay >= 2 and model.query("patch_available") == 1
-    partials = patch and lay == 3
+    partials = {3: 8, 4: 4}.get(lay, 0) if patch else 0
     eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
                                tables=not (shapes["mixed"]["mixed_recompute"] or patch),
                                node_ordered=bool(lay),
                                patch_nodes=model.query("patch_nodes_total") if patch else None,
                                patch_blocks=model.query("patch_blocks") if patch else 0,
-                               partials=partials, **kw)
+                               partials=partials,
+                               packed=patch and model.query("patch_packed") == 1, **kw)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms
     else:
         # per-patch partials: the node kernel's partial-sum instance
-        kname = node_kernel.replace(", 1, ", ", 2, ") if partials else node_kernel
+        kname = (node_kernel.replace(", 1, ", ", %d, " % (2 if partials == 8 else 3))
+                 if partials else node_kernel)
         kbytes, kms = nb, nd_ms
     achieved = kbytes / (kms * 1e-3) / 1e9
     step_gbs = (eb + nb) / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
@@ -336,6 +341,7 @@ def bench_cube(args, akx):
         E, N, n, Q, d)
     traffic = traffic_for(kname, shapes, "cube_tet4_traffic.json") if args.n == 100 else None
     eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
+                                   tables=not shapes["fp64"]["mixed_recompute"],
                                    node_ordered=bool(shapes["fp64"]["slot_layout"]))
     copy_gbs = copy_bandwidth()
     log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))

@@ -239,7 +239,8 @@ int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_rema
 /* Scalar facts about the context, for diagnostics and byte models:
  *   "nb_nodes", "nb_elements", "csr_entries", "patch_blocks",
  *   "patch_nodes_total" (sum over LDS patches of their distinct nodes),
- *   "patch_max_nodes", "patch_available" (1/0),
+ *   "patch_max_nodes", "patch_available" (1/0), "patch_packed" (1/0: 16-bit
+ *   patch-local node | rank words),
  *   "element_block", "node_block", "xcd_remap", "mixed_recompute", "slot_layout". */
 int akx_query(akx_ctx* ctx, const char* key, double* value);
 
