@@ -74,7 +74,7 @@ def build_facade(verbose: bool = False) -> str:
     the package directory, so the tree can be moved as a whole)."""
     cxx = os.environ.get("CXX", "g++")
     inc = ["-I", os.path.join(FACADE, "include"), "-I", os.path.join(ROOT, "include")]
-    flags = ["-O2", "-std=c++17", "-Wall", "-Wextra"]
+    flags = ["-O2", "-std=c++17", "-pthread", "-Wall", "-Wextra"]
     tmp = FACADE_LIB + ".tmp"
     _run([cxx] + flags + ["-fPIC", "-shared"] + inc +
          [os.path.join(FACADE, "src", "akantu.cc"), "-o", tmp, "-L", HERE, "-lakx",

@@ -97,8 +97,9 @@ int main(int argc, char* argv[]) {
       energy << s << "," << s * time_step << "," << epot << "," << ekin << "," << epot + ekin
              << std::endl;
       if (s % 100 == 0) std::cout << "passing step " << s << "/" << max_steps << std::endl;
-      if (dump_every && s % dump_every == 0) model.dump();
+      if (dump_every && s % dump_every == 0) model.dump();  // written in the background
     }
+    model.flushDumps();
 
     std::ofstream dt_out(prefix + "dt.txt");
     dt_out.precision(17);

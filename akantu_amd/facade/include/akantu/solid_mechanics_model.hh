@@ -21,6 +21,7 @@
  */
 #pragma once
 
+#include <future>
 #include <memory>
 #include <string>
 #include <vector>
@@ -97,10 +98,14 @@ class SolidMechanicsModel {
   /// paraview/<base>_<NNNN>.vtu per dump and paraview/<base>.pvd listing them.
   /// Nodal fields: displacement, velocity, acceleration, external_force,
   /// internal_force, mass, blocked_dofs; per-element (averaged over the
-  /// quadrature points): stress, grad_u, damage.
+  /// quadrature points): stress, grad_u, damage.  dump() snapshots the
+  /// fields on the device and returns; a background thread waits for the
+  /// copies and writes the files while the caller keeps stepping.
   void setBaseName(const std::string& name, const std::string& directory = "paraview");
   void addDumpField(const std::string& field);
   void dump();
+  /// wait until every dump so far is on disk (rethrows a write error)
+  void flushDumps();
 
   /// extensions: arithmetic of the element kernels, launch autotuning, kernel times
   void setPrecision(Precision p) { precision_ = p; }
@@ -135,6 +140,12 @@ class SolidMechanicsModel {
   std::string dump_base_ = "model", dump_dir_ = "paraview";
   std::vector<std::string> dump_fields_;
   std::vector<std::pair<Real, std::string>> dumps_;
+  struct DumpBuffer {
+    void* host = nullptr;  // page-locked (akx_host_alloc)
+    size_t bytes = 0;
+  };
+  std::vector<DumpBuffer> dump_bufs_;  // one per dump field, reused
+  std::future<void> dump_job_;         // the file writer of the last dump
   UInt steps_done_ = 0;
   bool host_stale_[kCount] = {};
   bool host_dirty_[kCount] = {};

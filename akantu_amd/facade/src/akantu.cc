@@ -6,6 +6,7 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <future>
 #include <limits>
 #include <map>
 #include <random>
@@ -384,6 +385,11 @@ SolidMechanicsModel::SolidMechanicsModel(Mesh& mesh, UInt spatial_dimension, con
 }
 
 SolidMechanicsModel::~SolidMechanicsModel() {
+  try {
+    flushDumps();
+  } catch (...) {
+  }
+  for (auto& b : dump_bufs_) akx_host_free(b.host);
   if (ctx_) akx_ctx_destroy(ctx_);
 }
 
@@ -665,6 +671,7 @@ void SolidMechanicsModel::getLastStepTiming(Real& element_ms, Real& node_ms) {
 /* ParaView output                                                           */
 /* ------------------------------------------------------------------------ */
 void SolidMechanicsModel::setBaseName(const std::string& name, const std::string& directory) {
+  flushDumps();
   dump_base_ = name;
   dump_dir_ = directory;
   dumps_.clear();
@@ -694,8 +701,14 @@ void vtu_array(std::ostream& o, const std::string& name, const Real* v, size_t c
 }
 }  // namespace
 
+void SolidMechanicsModel::flushDumps() {
+  if (dump_job_.valid()) dump_job_.get();
+}
+
 void SolidMechanicsModel::dump() {
   if (!initialized_) AKANTU_EXCEPTION("initFull first");
+  flushDumps();  // the previous dump's buffers are free again
+  pushDirty();
   const ElementType t = mesh_.getElementType();
   const UInt N = mesh_.getNbNodes(), E = mesh_.getNbElement(), n = getNbNodesPerElement(t);
   const UInt Q = getNbIntegrationPoints(t), d = dim_;
@@ -704,67 +717,106 @@ void SolidMechanicsModel::dump() {
   char num[16];
   std::snprintf(num, sizeof(num), "%04u", unsigned(dumps_.size()));
   const std::string file = dump_base_ + "_" + num + ".vtu";
-  std::error_code ec;
-  std::filesystem::create_directories(dump_dir_, ec);
-  std::ofstream o(dump_dir_ + "/" + file);
-  if (!o) AKANTU_EXCEPTION("cannot write " << dump_dir_ << "/" << file);
-  o.precision(17);
-  o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
-       "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\""
-    << N << "\" NumberOfCells=\"" << E << "\">\n<Points>\n";
-  vtu_array(o, "positions", mesh_.getNodes().storage(), N, d, 3);
-  o << "</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
-  const auto& conn = mesh_.getConnectivity(t);
-  for (UInt e = 0; e < E; ++e) {
-    for (UInt a = 0; a < n; ++a) o << conn(e, a) << ' ';
-    o << '\n';
-  }
-  o << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
-  for (UInt e = 0; e < E; ++e) o << size_t(e + 1) * n << '\n';
-  o << "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
-  for (UInt e = 0; e < E; ++e) o << vtk.at(t) << '\n';
-  o << "</DataArray>\n</Cells>\n<PointData>\n";
-  for (const auto& f : dump_fields_) {
-    const Array<Real>* a = nullptr;
-    if (f == "displacement") a = &getDisplacement();
-    else if (f == "velocity") a = &getVelocity();
-    else if (f == "acceleration") a = &getAcceleration();
-    else if (f == "external_force") a = &getExternalForce();
-    else if (f == "internal_force") a = &getInternalForce();
-    else if (f == "mass") a = &getMass();
-    if (a) vtu_array(o, f, a->storage(), N, d, d == 2 ? 3 : d);
-    if (f == "blocked_dofs") {
-      const auto& b = getBlockedDOFs();
-      std::vector<Real> v(size_t(N) * d);
-      for (size_t k = 0; k < v.size(); ++k) v[k] = b[k] ? 1. : 0.;
-      vtu_array(o, f, v.data(), N, d, d == 2 ? 3 : d);
+  // snapshots of every field, enqueued behind the steps so far; the copies
+  // to the page-locked buffers overlap the steps that follow
+  struct Item {
+    std::string name;
+    bool point, bytes_u8;
+    UInt comp;
+    const void* host;
+    uint64_t ticket;
+  };
+  std::vector<Item> items;
+  dump_bufs_.resize(dump_fields_.size());
+  for (size_t k = 0; k < dump_fields_.size(); ++k) {
+    const std::string& f = dump_fields_[k];
+    int id = -1;
+    bool point = false;
+    if (f == "displacement") id = AKX_DISPLACEMENT;
+    else if (f == "velocity") id = AKX_VELOCITY;
+    else if (f == "acceleration") id = AKX_ACCELERATION;
+    else if (f == "external_force") id = AKX_EXTERNAL_FORCE;
+    else if (f == "internal_force") id = AKX_INTERNAL_FORCE;
+    else if (f == "mass") id = AKX_MASS;
+    else if (f == "blocked_dofs") id = AKX_BLOCKED_DOFS;
+    else if (f == "stress") { id = AKX_STRESS; point = true; }
+    else if (f == "grad_u") { id = AKX_GRAD_U; point = true; }
+    else if (f == "damage") { id = AKX_DAMAGE; point = true; }
+    const bool u8 = id == AKX_BLOCKED_DOFS;
+    const UInt comp = id == AKX_DAMAGE ? 1 : point ? d * d : d;
+    const size_t bytes = (point ? size_t(E) * Q : size_t(N)) * comp * (u8 ? 1 : 8);
+    DumpBuffer& b = dump_bufs_[k];
+    if (b.bytes < bytes) {
+      akx_host_free(b.host);
+      b = DumpBuffer{};
+      check(akx_host_alloc(bytes, &b.host), "akx_host_alloc");
+      b.bytes = bytes;
     }
+    uint64_t ticket = 0;
+    check(akx_field_download_async(ctx_, id, b.host, bytes, &ticket), "akx_field_download_async");
+    items.push_back({f, point, u8, comp, b.host, ticket});
   }
-  o << "</PointData>\n<CellData>\n";
-  pushDirty();
-  for (const auto& f : dump_fields_) {
-    int field = -1;
-    if (f == "stress") field = AKX_STRESS;
-    else if (f == "grad_u") field = AKX_GRAD_U;
-    else if (f == "damage") field = AKX_DAMAGE;
-    if (field < 0) continue;
-    const UInt comp = field == AKX_DAMAGE ? 1 : d * d;
-    std::vector<Real> all(size_t(E) * Q * comp), avg(size_t(E) * comp, 0.);
-    check(akx_field_download(ctx_, field, all.data(), all.size() * 8), "akx_field_download");
-    for (UInt e = 0; e < E; ++e)
-      for (UInt q = 0; q < Q; ++q)
-        for (UInt c = 0; c < comp; ++c) avg[size_t(e) * comp + c] += all[(size_t(e) * Q + q) * comp + c] / Q;
-    vtu_array(o, f, avg.data(), E, comp, comp);
-  }
-  o << "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
   dumps_.push_back({Real(steps_done_) * time_step_, file});
-  std::ofstream pvd(dump_dir_ + "/" + dump_base_ + ".pvd");
-  pvd.precision(17);
-  pvd << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n<Collection>\n";
-  for (const auto& dd : dumps_)
-    pvd << "<DataSet timestep=\"" << dd.first << "\" group=\"\" part=\"0\" file=\"" << dd.second
-        << "\"/>\n";
-  pvd << "</Collection>\n</VTKFile>\n";
+  const auto entries = dumps_;
+  const std::string dir = dump_dir_, pvd_name = dump_base_ + ".pvd";
+  akx_ctx* ctx = ctx_;
+  const Mesh* mesh = &mesh_;
+  const int cell = vtk.at(t);
+  dump_job_ = std::async(std::launch::async, [=]() {
+    for (const auto& it : items)
+      if (akx_download_wait(ctx, it.ticket) != AKX_OK)
+        AKANTU_EXCEPTION("akx_download_wait failed for " << it.name);
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    std::ofstream o(dir + "/" + file);
+    if (!o) AKANTU_EXCEPTION("cannot write " << dir << "/" << file);
+    o.precision(17);
+    o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+         "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\""
+      << N << "\" NumberOfCells=\"" << E << "\">\n<Points>\n";
+    vtu_array(o, "positions", mesh->getNodes().storage(), N, d, 3);
+    o << "</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    const auto& conn = mesh->getConnectivity(t);
+    for (UInt e = 0; e < E; ++e) {
+      for (UInt a = 0; a < n; ++a) o << conn(e, a) << ' ';
+      o << '\n';
+    }
+    o << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+    for (UInt e = 0; e < E; ++e) o << size_t(e + 1) * n << '\n';
+    o << "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
+    for (UInt e = 0; e < E; ++e) o << cell << '\n';
+    o << "</DataArray>\n</Cells>\n<PointData>\n";
+    for (const auto& it : items) {
+      if (it.point) continue;
+      if (it.bytes_u8) {
+        const auto* b = static_cast<const uint8_t*>(it.host);
+        std::vector<Real> v(size_t(N) * d);
+        for (size_t k = 0; k < v.size(); ++k) v[k] = b[k] ? 1. : 0.;
+        vtu_array(o, it.name, v.data(), N, d, d == 2 ? 3 : d);
+      } else {
+        vtu_array(o, it.name, static_cast<const Real*>(it.host), N, d, d == 2 ? 3 : d);
+      }
+    }
+    o << "</PointData>\n<CellData>\n";
+    for (const auto& it : items) {
+      if (!it.point) continue;
+      const auto* all = static_cast<const Real*>(it.host);
+      std::vector<Real> avg(size_t(E) * it.comp, 0.);
+      for (UInt e = 0; e < E; ++e)
+        for (UInt q = 0; q < Q; ++q)
+          for (UInt c = 0; c < it.comp; ++c)
+            avg[size_t(e) * it.comp + c] += all[(size_t(e) * Q + q) * it.comp + c] / Q;
+      vtu_array(o, it.name, avg.data(), E, it.comp, it.comp);
+    }
+    o << "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+    std::ofstream pvd(dir + "/" + pvd_name);
+    pvd.precision(17);
+    pvd << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n<Collection>\n";
+    for (const auto& dd : entries)
+      pvd << "<DataSet timestep=\"" << dd.first << "\" group=\"\" part=\"0\" file=\"" << dd.second
+          << "\"/>\n";
+    pvd << "</Collection>\n</VTKFile>\n";
+  });
 }
 
 void SolidMechanicsModel::downloadPointField(int field, Array<Real>& out, const Material& m) {

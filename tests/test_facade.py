@@ -108,6 +108,12 @@ def test_explicit_dynamic_driver_matches_oracle(precision, tmp_path):
     assert (tmp_path / "paraview" / "explicit_dynamic.pvd").exists()
     u0 = np.loadtxt(tmp_path / "u0.txt")
     u = np.loadtxt(tmp_path / "u.txt")
+    # the last dump (written by the background writer) holds the final state
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(tmp_path / "paraview" / dumps[-1]).getroot().find("UnstructuredGrid/Piece")
+    arrays = {da.get("Name"): da.text for da in piece.iter("DataArray")}
+    assert np.array_equal(np.array(arrays["displacement"].split(), dtype=float).reshape(-1, 3), u)
+    assert len(arrays["stress"].split()) == 9 * m.nb_elements
     dt = float((tmp_path / "dt.txt").read_text())
     en = np.loadtxt(tmp_path / "energy.csv", delimiter=",", skiprows=1, usecols=(0, 2, 3))
     assert en.shape == (steps, 3)
