@@ -70,7 +70,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->csr2_off); dfree(c->csr2_slot); dfree(c->slot2_pos);
   dfree(c->pnode_cnt); dfree(c->pnode_ids); dfree(c->run_start); dfree(c->run_base);
   dfree(c->lconn); dfree(c->lrank); dfree(c->lcr);
-  dfree(c->pcsr_off); dfree(c->pentry);
+  dfree(c->pcsr_off); dfree(c->pentry); dfree(c->ppos);
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
   c->patch_ok = false;
@@ -278,13 +278,14 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
     std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
 }
 
-// Mixed path with per-patch partial sums (slot layouts 3 and 4): the element
-// kernel reduces each node's contributions inside its patch in FP64 and
-// stores one FP64 (3) or FP32 (4) partial per (patch, node); the node kernel
-// sums a node's partials in FP64.  Needs the LDS patches and a single launch
-// group.
+// Mixed path with per-patch partial sums (slot layouts 3, 4 and 5): the
+// element kernel reduces each node's contributions inside its patch in FP64
+// and stores one FP64 (3) or FP32 (4, 5) partial per (patch, node), in patch
+// order (3, 4) or at the node's place in a node-major list (5); the node
+// kernel sums a node's partials in FP64.  Needs the LDS patches and a single
+// launch group.
 bool uses_partials(const akx_ctx* c, int precision) {
-  return precision == AKX_MIXED && (c->node_ordered == 3 || c->node_ordered == 4) &&
+  return precision == AKX_MIXED && c->node_ordered >= 3 && c->node_ordered <= 5 &&
          c->patch_ok && c->groups.size() == 1 &&
          !c->groups[0].filter &&
          !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
@@ -316,7 +317,8 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.X = c->X;
   A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
   A.patch = (precision == AKX_MIXED && c->node_ordered >= 2 && c->patch_ok && !g.filter) ? 1 : 0;
-  A.partials = (A.patch && uses_partials(c, precision)) ? (c->node_ordered == 4 ? 2 : 1) : 0;
+  A.partials = (A.patch && uses_partials(c, precision)) ? (c->node_ordered == 3 ? 1 : 2) : 0;
+  A.part_pos = (A.partials && c->node_ordered == 5) ? c->ppos : nullptr;
   A.patch_max_nodes = c->patch_max_nodes;
   A.patch_lds = c->patch_lds;
   A.patch_blocks = c->patch_blocks;
@@ -372,11 +374,12 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.stage_lds = 0;
   A.partials = 0;
   if (uses_partials(c, precision)) {
-    A.partials = c->node_ordered == 4 ? 2 : 1;
-    // one partial sum per (patch, node): a short gather per node
+    A.partials = c->node_ordered == 3 ? 1 : 2;
+    // one partial sum per (patch, node): a short gather per node, or a
+    // contiguous run per node when the partials are node-ordered
     A.csr_off = c->pcsr_off;
     A.csr_slot = c->pentry;
-    A.node_ordered = 0;
+    A.node_ordered = c->node_ordered == 5 ? 1 : 0;
     return A;
   }
   if (c->node_ordered && c->node_stage) {
@@ -527,10 +530,16 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
     for (int64_t b = 0; b < nb; ++b)
       for (uint32_t k = 0; k < cnt[b]; ++k)
         ent[fill[pids[poff[b] + k]]++] = uint32_t(b * S + k);
+    // the inverse, for node-ordered partials: entry b * S + k -> its position
+    // in the node-major list (padding entries are never written)
+    std::vector<uint32_t> ppos(size_t(nb) * S, 0);
+    for (size_t q = 0; q < ent.size(); ++q) ppos[ent[q]] = uint32_t(q);
     HIP_TRY(c, dalloc(&c->pcsr_off, off.size()));
     HIP_TRY(c, dalloc(&c->pentry, ent.size()));
+    HIP_TRY(c, dalloc(&c->ppos, ppos.size()));
     HIP_TRY(c, hipMemcpy(c->pcsr_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->pentry, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->ppos, ppos.data(), ppos.size() * 4, hipMemcpyHostToDevice));
   }
   c->patch_max_nodes = maxm;
   c->patch_nodes_total = int64_t(pids.size());
@@ -1640,9 +1649,9 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
   struct Shape { int eb, nb, x, rcmp, lay, stage; };
   std::vector<Shape> cand;
   const int blocks[3] = {64, 128, 256};
-  // layouts 2, 3 and 4 (LDS patches) exist on the mixed path only; staging
-  // the node kernel's slot range applies to node-ordered slots (layouts 1, 2)
-  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 5 : 2;
+  // layouts 2 to 5 (LDS patches) exist on the mixed path only; staging the
+  // node kernel's slot range applies to node-ordered slots (layouts 1, 2)
+  const int nlay = (precision == AKX_MIXED && c->patch_ok) ? 6 : 2;
   for (int lay = 0; lay < nlay; ++lay)
     for (int stage = 0; stage < ((lay == 1 || lay == 2) ? 2 : 1); ++stage)
       for (int rcmp = (lay >= 2 ? 1 : 0); rcmp < 2; ++rcmp)
@@ -1694,7 +1703,7 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
     return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
-  if (slot_layout > 4) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0, 1, 2, 3 or 4");
+  if (slot_layout > 5) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0 to 5");
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;

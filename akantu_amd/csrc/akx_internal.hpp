@@ -49,6 +49,7 @@ struct ElemArgs {
   // LDS-patch variant of the mixed kernel (k_element_patch)
   int patch;                 // 1: use it
   int partials;              // write one sum per (patch, node) instead of the runs: 1 FP64, 2 FP32
+  const uint32_t* part_pos;  // [b * S + k] node-major position of the partial, nullptr = b * S + k
   int patch_max_nodes;       // most distinct nodes of one patch = stride S of the lists
   int patch_lds;             // dynamic LDS bytes per block
   int64_t patch_blocks;      // number of patches
@@ -276,7 +277,7 @@ struct akx_ctx {
   uint16_t *run_base = nullptr, *lconn = nullptr, *lcr = nullptr;
   uint8_t* lrank = nullptr;
   // node -> entries of the per-patch partial sums (slot layout 3)
-  uint32_t *pcsr_off = nullptr, *pentry = nullptr;
+  uint32_t *pcsr_off = nullptr, *pentry = nullptr, *ppos = nullptr;
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal

@@ -751,7 +751,9 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
     float2* dst = reinterpret_cast<float2*>(s_ux + size_t(k) * 2 * d);
 #pragma unroll
     for (int j = 0; j < d; ++j) dst[j] = make_float2(v[2 * j], v[2 * j + 1]);
+    // run start (layout 2) or the partial's node-major position (layout 5)
     if (!A.partials) s_rs[k] = A.run_start[k0 + k];
+    else if (A.part_pos) s_rs[k] = A.part_pos[k0 + k];
     s_rb[k] = A.run_base[b * int64_t(S + 1) + k];
   }
   if (threadIdx.x == 0) s_rb[S] = A.run_base[b * int64_t(S + 1) + S];
@@ -812,14 +814,15 @@ __global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
 #pragma unroll
           for (int i = 0; i < d; ++i) f[i] += double(q + j < q1 ? v[j * d + i] : 0.f);
       }
+      const int64_t at = A.part_pos ? int64_t(s_rs[k]) : k0 + k;
       if (A.partials == 2) {
         float* __restrict__ part = static_cast<float*>(A.slots);
 #pragma unroll
-        for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = float(f[i]);
+        for (int i = 0; i < d; ++i) part[at * d + i] = float(f[i]);
       } else {
         double* __restrict__ part = static_cast<double*>(A.slots);
 #pragma unroll
-        for (int i = 0; i < d; ++i) part[(k0 + k) * d + i] = f[i];
+        for (int i = 0; i < d; ++i) part[at * d + i] = f[i];
       }
     }
     return;

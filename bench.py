@@ -52,7 +52,7 @@ def log(msg):
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
                       patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0,
-                      partials=0, packed=False):
+                      partials=0, packed=False, partials_node_major=False):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -66,12 +66,16 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     kernel writes (8 FP64, 4 FP32; no run start needed) and the node kernel
     gathers through a node -> entry CSR instead of reading E*n slots;
     packed: local node and run rank in one 16-bit word per (e, a), 3-node
-    elements padded to 4."""
+    elements padded to 4; partials_node_major: the patch kernel reads each
+    partial's node-major position and the node kernel streams its partials
+    (no entry list)."""
     local = (E * (4 if n == 3 else n) * 2) if packed else E * n * (2 + 1)
     if partials:
+        pos = patch_nodes * 4
         elem = (local + patch_nodes * (4 + 2) + patch_blocks * (4 + 2)
-                + 2 * N * d * 8 + patch_nodes * d * partials + state_bytes)
-        node = ((N + 1) * 4 + patch_nodes * 4 + patch_nodes * d * partials
+                + 2 * N * d * 8 + patch_nodes * d * partials + state_bytes
+                + (pos if partials_node_major else 0))
+        node = ((N + 1) * 4 + (0 if partials_node_major else pos) + patch_nodes * d * partials
                 + 6 * N * d * 8 + N * 8 + N * d
                 + (N * d * 8 if has_fext else 0) + (N * 8 if dt_per_node else 0))
         return elem, node
@@ -234,14 +238,15 @@ def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
     """Roofline of the slower of the two mixed-path kernels."""
     lay = shapes["mixed"]["slot_layout"]
     patch = lay >= 2 and model.query("patch_available") == 1
-    partials = {3: 8, 4: 4}.get(lay, 0) if patch else 0
+    partials = {3: 8, 4: 4, 5: 4}.get(lay, 0) if patch else 0
     eb, nb = algorithmic_bytes(E, N, n, Q, d, 4,
                                tables=not (shapes["mixed"]["mixed_recompute"] or patch),
                                node_ordered=bool(lay),
                                patch_nodes=model.query("patch_nodes_total") if patch else None,
                                patch_blocks=model.query("patch_blocks") if patch else 0,
                                partials=partials,
-                               packed=patch and model.query("patch_packed") == 1, **kw)
+                               packed=patch and model.query("patch_packed") == 1,
+                               partials_node_major=patch and lay == 5, **kw)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms

@@ -228,11 +228,12 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * is bit-identical to the tables --, the layout of the element force slots:
  * element-ordered (0), node-ordered (1), node-ordered written by LDS element
  * patches (2, mixed path) or one partial sum per (patch, node), summed in
- * FP64 and stored in FP64 (3, mixed path) or FP32 (4, mixed path), and
- * whether the node kernel copies its block's contiguous slot range to LDS
- * first (node_stage 1/0) (-1 keeps).  On the FP64 path none of these changes
- * a result bit; on the mixed path layouts 0/1, 2, 3 and 4 differ in rounding
- * only (patch-relative FP32 node values, partial sums). */
+ * FP64 and stored in FP64 (3, mixed path) or FP32 (4, mixed path) in patch
+ * order, or in FP32 at the node's place in a node-major list (5, mixed
+ * path), and whether the node kernel copies its block's contiguous slot
+ * range to LDS first (node_stage 1/0) (-1 keeps).  On the FP64 path none of
+ * these changes a result bit; on the mixed path layouts 0/1, 2, 3 and 4/5
+ * differ in rounding only (patch-relative FP32 node values, partial sums). */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 
