@@ -1,0 +1,116 @@
+"""Edge cases of the GPU path: single-element meshes of every element type
+through every kernel variant (one partly filled patch, one node block), and
+the C ABI's argument and state errors.  Ragged patches (element counts that
+are not a multiple of the patch size) and patches too large for the packed
+16-bit indices are covered by the c0 bar (4,000 triangles) and the hex8 cube
+in test_gpu_parity.py."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from akantu_amd import akx
+from akantu_amd import mesh as akm
+from tests import gpu_helpers as gh
+
+pytestmark = pytest.mark.gpu
+
+ONE_ELEMENT = {
+    "triangle_3": (akm.TRIANGLE_3, [[0.0, 0.0], [1.0, 0.1], [0.2, 0.9]], [[0, 1, 2]]),
+    "quadrangle_4": (akm.QUADRANGLE_4, [[0.0, 0.0], [1.0, 0.0], [1.1, 0.9], [0.0, 1.0]],
+                     [[0, 1, 2, 3]]),
+    "tetrahedron_4": (akm.TETRAHEDRON_4,
+                      [[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.1, 1.0, 0.1], [0.0, 0.2, 1.0]],
+                      [[0, 1, 2, 3]]),
+    "hexahedron_8": (akm.HEXAHEDRON_8,
+                     [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                      [0, 0, 1], [1, 0, 1], [1, 1, 1.1], [0, 1, 1]],
+                     [[0, 1, 2, 3, 4, 5, 6, 7]]),
+}
+
+
+def _one_element_pair(name, kind=akx.ELASTIC):
+    et, nodes, conn = ONE_ELEMENT[name]
+    m = akm.Mesh(et, np.array(nodes, dtype=float), np.array(conn))
+    p = gh.Pair(m, [dict(kind=kind, E=1e7, nu=0.3, rho=1000.0)])
+    rng = np.random.default_rng(5)
+    p.set_nodal("velocity", rng.normal(scale=0.1, size=m.nodes.shape))
+    mask = np.zeros_like(m.nodes, dtype=bool)
+    mask[0] = True                      # one node held: a forced, non-rigid motion
+    p.block(mask)
+    p.set_time_step(0.5 * p.cpu.stable_time_step())
+    return p
+
+
+@pytest.mark.parametrize("name", sorted(ONE_ELEMENT))
+def test_single_element_fp64_every_layout(name):
+    """FP64 on one element == oracle bit for bit, whatever the launch shape
+    (layouts 2-5 fall back to node-ordered slots on the FP64 path)."""
+    for layout in range(6):
+        for recompute in (0, 1):
+            p = _one_element_pair(name)
+            p.gpu.set_launch(64, 64, 1, recompute, layout, 0)
+            p.step(40, akx.FP64)
+            assert np.array_equal(p.gpu.get_displacement(), p.cpu.nodal("u")), (layout, recompute)
+            assert np.array_equal(p.gpu.get_velocity(), p.cpu.nodal("v")), (layout, recompute)
+
+
+@pytest.mark.parametrize("name", sorted(ONE_ELEMENT))
+@pytest.mark.parametrize("kind", [akx.ELASTIC, akx.NEOHOOKEAN])
+def test_single_element_mixed_every_layout(name, kind):
+    """Mixed path on one element (a patch of one element, a node block of a
+    few nodes): <= 1e-5 from the oracle after 40 steps for every layout."""
+    for layout in range(6):
+        for recompute in ((0, 1) if layout < 2 else (1,)):
+            p = _one_element_pair(name, kind)
+            p.gpu.set_launch(256, 256, 1, recompute, layout, 0)
+            p.step(40, akx.MIXED)
+            err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
+            assert err <= 1e-5, (layout, recompute, err)
+
+
+def _check(rc, expected):
+    assert rc == expected, (rc, expected)
+
+
+def test_abi_rejects_bad_arguments_and_states():
+    lib = akx.lib
+    h = ctypes.c_void_p()
+    _check(lib.akx_ctx_create(0, 0, ctypes.byref(h)), 0)
+    try:
+        nodes = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+        bad = np.array([[0, 1, 3]], dtype=np.uint32)       # node 3 does not exist
+        _check(lib.akx_mesh_set(h, akm.TRIANGLE_3, 2, 3, nodes.ctypes.data_as(ctypes.c_void_p),
+                                1, bad.ctypes.data_as(ctypes.c_void_p)), -1)
+        _check(lib.akx_mesh_set(h, akm.TRIANGLE_3, 3, 3, nodes.ctypes.data_as(ctypes.c_void_p),
+                                1, bad.ctypes.data_as(ctypes.c_void_p)), -1)   # wrong dimension
+        conn = np.array([[0, 1, 2]], dtype=np.uint32)
+        _check(lib.akx_mesh_set(h, akm.TRIANGLE_3, 2, 3, nodes.ctypes.data_as(ctypes.c_void_p),
+                                1, conn.ctypes.data_as(ctypes.c_void_p)), 0)
+        # stepping and asynchronous downloads need akx_init
+        _check(lib.akx_solve_step(h, 1, akx.FP64), -2)
+        out = np.zeros((3, 2))
+        t = ctypes.c_uint64()
+        _check(lib.akx_field_download_async(h, 0, out.ctypes.data_as(ctypes.c_void_p),
+                                            out.nbytes, ctypes.byref(t)), -2)
+        mid = ctypes.c_int()
+        _check(lib.akx_material_add(h, akx.ELASTIC, 1e7, 0.3, 1000.0, 0, ctypes.byref(mid)), 0)
+        _check(lib.akx_init(h), 0)
+        _check(lib.akx_solve_step(h, 1, akx.FP64), -2)        # no time step yet
+        _check(lib.akx_set_time_step(h, -1.0), -1)
+        _check(lib.akx_set_time_step(h, 1e-6), 0)
+        # sizes must match exactly; layouts are 0..5; tickets must exist
+        _check(lib.akx_field_upload(h, 0, out.ctypes.data_as(ctypes.c_void_p), out.nbytes - 8), -1)
+        _check(lib.akx_field_download(h, 0, out.ctypes.data_as(ctypes.c_void_p), out.nbytes + 8), -1)
+        _check(lib.akx_set_launch(h, 0, 0, -1, -1, 6, -1), -1)
+        _check(lib.akx_download_wait(h, 12345), -1)
+        done = ctypes.c_int()
+        _check(lib.akx_download_query(h, 0, ctypes.byref(done)), -1)
+        # a static table is not available asynchronously
+        jac = np.zeros(1)
+        _check(lib.akx_field_download_async(h, 13, jac.ctypes.data_as(ctypes.c_void_p), 8,
+                                            ctypes.byref(t)), -1)
+        assert b"asynchronously" in lib.akx_last_error(h)
+        _check(lib.akx_solve_step(h, 1, 7), -1)             # unknown precision
+    finally:
+        _check(lib.akx_ctx_destroy(h), 0)
