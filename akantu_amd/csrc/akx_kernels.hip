@@ -308,9 +308,18 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
       }
     return;
   }
-#pragma unroll
+  // With the damage law, the quadrature loop stays rolled and non-simplex
+  // elements rebuilding B form its columns B_a = Ji dN_a/dxi where they are
+  // used (once for grad u, once for the forces) instead of holding them: the
+  // unrolled 8-point hexahedron needed 448 VGPRs + 192 AGPRs (one wave per
+  // SIMD), the rolled one 168.  The other laws keep the unrolled form, which
+  // is the leaner one for them.
+  constexpr bool on_the_fly = RECOMPUTE && !simplex && KIND == NH_DAMAGE;
+  constexpr int QUNROLL = KIND == NH_DAMAGE ? 1 : Q;
+#pragma unroll QUNROLL
   for (int q = 0; q < Q; ++q) {
-    float Bq[dn];
+    float Bq[on_the_fly ? 1 : dn];
+    float Jiq[on_the_fly ? d * d : 1];
     float jq;
     if constexpr (!RECOMPUTE) {
 #pragma unroll
@@ -354,37 +363,58 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
       float Ji[d * d];
 #pragma unroll
       for (int k = 0; k < d * d; ++k) Ji[k] = Ad[k] * idet;
-      // B(i,a) = sum_l Ji(i,l) dN_a/dxi_l
+      if constexpr (on_the_fly) {
 #pragma unroll
-      for (int a = 0; a < n; ++a)
+        for (int k = 0; k < d * d; ++k) Jiq[k] = Ji[k];
+      } else {
+        // B(i,a) = sum_l Ji(i,l) dN_a/dxi_l
 #pragma unroll
-        for (int i = 0; i < d; ++i) {
-          float s = 0.f;
-          if constexpr (simplex) {
-            if (a == 0) {
+        for (int a = 0; a < n; ++a)
 #pragma unroll
-              for (int l = 0; l < d; ++l) s -= Ji[i + l * d];
+          for (int i = 0; i < d; ++i) {
+            float s = 0.f;
+            if constexpr (simplex) {
+              if (a == 0) {
+#pragma unroll
+                for (int l = 0; l < d; ++l) s -= Ji[i + l * d];
+              } else {
+                s = Ji[i + (a - 1) * d];
+              }
             } else {
-              s = Ji[i + (a - 1) * d];
-            }
-          } else {
 #pragma unroll
-            for (int l = 0; l < d; ++l) s += Ji[i + l * d] * A.tabf.dnds[q][l + a * d];
+              for (int l = 0; l < d; ++l) s += Ji[i + l * d] * A.tabf.dnds[q][l + a * d];
+            }
+            Bq[i + a * d] = s;
           }
-          Bq[i + a * d] = s;
-        }
+      }
       jq = det * A.tabf.w[q];
     }
-    float G[d * d];
-#pragma unroll
-    for (int j = 0; j < d; ++j)
+    // column a of B: B(i,a) = sum_l Ji(i,l) dN_a/dxi_l, or the stored one
+    auto bcol = [&](int a, float* b) {
 #pragma unroll
       for (int i = 0; i < d; ++i) {
-        float s = 0.f;
+        if constexpr (on_the_fly) {
+          float s = 0.f;
 #pragma unroll
-        for (int a = 1; a < n; ++a) s += du[i + a * d] * Bq[j + a * d];
-        G[i + j * d] = s;
+          for (int l = 0; l < d; ++l) s += Jiq[i + l * d] * A.tabf.dnds[q][l + a * d];
+          b[i] = s;
+        } else {
+          b[i] = Bq[i + a * d];
+        }
       }
+    };
+    float G[d * d];
+#pragma unroll
+    for (int k = 0; k < d * d; ++k) G[k] = 0.f;
+#pragma unroll
+    for (int a = 1; a < n; ++a) {
+      float b[d];
+      bcol(a, b);
+#pragma unroll
+      for (int j = 0; j < d; ++j)
+#pragma unroll
+        for (int i = 0; i < d; ++i) G[i + j * d] += du[i + a * d] * b[j];
+    }
     float P[d * d];  // first Piola-Kirchhoff (Cauchy at small strain) times det(J) w
     if constexpr (KIND == ELASTIC) {
       float tr = 0.f;
@@ -425,14 +455,17 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
     }
     // f_a,i += sum_j P_ij dN_a/dX_j
 #pragma unroll
-    for (int a = 0; a < n; ++a)
+    for (int a = 0; a < n; ++a) {
+      float b[d];
+      bcol(a, b);
 #pragma unroll
       for (int i = 0; i < d; ++i) {
         float s = fe[i + a * d];
 #pragma unroll
-        for (int j = 0; j < d; ++j) s += P[i + j * d] * Bq[j + a * d];
+        for (int j = 0; j < d; ++j) s += P[i + j * d] * b[j];
         fe[i + a * d] = s;
       }
+    }
   }
 }
 
