@@ -165,6 +165,7 @@ class SolidMechanicsModel:
                                         _ptr(self._X), mesh.nb_elements, _ptr(self._conn)))
         self.materials = []
         self.time_step = 0.0
+        self._inflight = {}  # ticket -> PinnedBuffer of an asynchronous download
 
     # -- materials (MaterialFactory / input file `material elastic [...]`) --
     def add_material(self, kind=ELASTIC, E=None, nu=0.0, rho=None, plane_stress=False,
@@ -370,16 +371,22 @@ class SolidMechanicsModel:
         t = ctypes.c_uint64()
         self.ctx.check(lib.akx_field_download_async(self.ctx.h, FIELDS[name], buf.ptr, nbytes,
                                                     ctypes.byref(t)))
+        # the buffer stays referenced until its copy is known to be done, so
+        # that dropping it early cannot free memory the copy engine writes
+        self._inflight[t.value] = buf
         return t.value
 
     def wait(self, ticket):
         """Block until the asynchronous download `ticket` has landed (may be
         called from another thread than the one that steps the model)."""
         self.ctx.check(lib.akx_download_wait(self.ctx.h, ticket))
+        self._inflight.pop(ticket, None)
 
     def done(self, ticket):
         flag = ctypes.c_int()
         self.ctx.check(lib.akx_download_query(self.ctx.h, ticket, ctypes.byref(flag)))
+        if flag.value:
+            self._inflight.pop(ticket, None)
         return bool(flag.value)
 
     def set(self, name, values):
