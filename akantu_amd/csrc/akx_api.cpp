@@ -1231,9 +1231,11 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       c->pool.push_back(e);
     }
   }
+  // the mixed path carries the half-step velocity between the steps of a run
+  const int leap = precision == AKX_MIXED ? NODE_LEAP : 0;
   // predictor of step 1
   {
-    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0), precision);
+    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0) | leap, precision);
     HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
   }
   for (int s = 1; s <= nsteps; ++s) {
@@ -1242,7 +1244,7 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
     rc = run_elements(c, precision, timing && last);
     if (rc) return rc;
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
-    int mode = NODE_CORRECT;
+    int mode = NODE_CORRECT | leap;
     if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
     else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
     NodeArgs A = node_args(c, mode, precision);

@@ -104,7 +104,15 @@ enum NodeMode {
   NODE_STORE_FINT = 4,  // write the assembled internal force
   NODE_SAVE_PREV = 8,   // previous = u before the predictor
   NODE_STORE_INC = 16,  // increment = u - previous after the corrector
-  NODE_GATHER = 32      // gather forces only (with NODE_STORE_FINT)
+  NODE_GATHER = 32,     // gather forces only (with NODE_STORE_FINT)
+  // Half-step velocity form (mixed path): between the first predictor and the
+  // last corrector of a run, v holds v_{n+1/2} = v_n + dt/2 a_n and a is
+  // neither read nor written: a_n = (f_ext + f_int) / M, v_{n+1/2} =
+  // v_{n-1/2} + dt a_n, u_{n+1} = u_n + dt v_{n+1/2} -- the same central
+  // difference as the predictor / corrector pair, with one array less of
+  // traffic.  The first predictor forms v_{1/2} from (v, a), the last
+  // corrector returns v_n = v_{n-1/2} + dt/2 a_n and a_n.
+  NODE_LEAP = 64
 };
 
 // Byte offset of the forces in the LDS of k_element_patch with list stride S:

@@ -909,10 +909,13 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   const int nk = cnt * D;
   const int mode = A.mode;
   const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
+  const bool leap = mode & NODE_LEAP;
+  // the half-step form between two steps touches neither a nor its loads
+  const bool leap_mid = leap && correct && predict;
   for (int t = threadIdx.x; t < nk; t += nb) {
     s_u[t] = A.u[k0 + t];
     s_v[t] = A.v[k0 + t];
-    s_a[t] = A.a[k0 + t];
+    if (!leap_mid) s_a[t] = A.a[k0 + t];
     s_b[t] = A.blocked[k0 + t];
     if constexpr (HAS_FEXT) s_f[t] = A.fext[k0 + t];
   }
@@ -1021,14 +1024,26 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
         double r = 0.;
         if constexpr (HAS_FEXT) r += s_f[l];
         r += fi;
-        double cache = 0.;
-        cache += -1. * s_a[l];
-        r += M * cache;
-        if (!(s_b[l] & 1)) {
-          const double delta = r / M;
-          s_u[l] += ecoef * delta;
-          s_v[l] += dcoef * delta;
-          s_a[l] += delta;
+        if (leap) {
+          if (!(s_b[l] & 1)) {
+            const double an = r / M;
+            if (predict) {
+              s_v[l] += dt * an;    // v_{n+1/2}
+            } else {
+              s_v[l] += dcoef * an; // v_n
+              s_a[l] = an;
+            }
+          }
+        } else {
+          double cache = 0.;
+          cache += -1. * s_a[l];
+          r += M * cache;
+          if (!(s_b[l] & 1)) {
+            const double delta = r / M;
+            s_u[l] += ecoef * delta;
+            s_v[l] += dcoef * delta;
+            s_a[l] += delta;
+          }
         }
         if (mode & NODE_STORE_INC) {
           double inc = s_u[l];
@@ -1046,10 +1061,16 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
         if (b & 2) u += A.bc_inc[i];
         if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = u;
         if (!(b & 1)) {
-          const double v = s_v[l], acc = s_a[l];
-          const double dt_a_n = dt * acc;
-          u += dt * v + 0.5 * dt * dt_a_n;
-          s_v[l] = v + dt_a_n;
+          if (leap) {
+            // v_{n+1/2}: formed here on the first step, by the corrector after
+            if (!correct) s_v[l] += 0.5 * dt * s_a[l];
+            u += dt * s_v[l];
+          } else {
+            const double v = s_v[l], acc = s_a[l];
+            const double dt_a_n = dt * acc;
+            u += dt * v + 0.5 * dt * dt_a_n;
+            s_v[l] = v + dt_a_n;
+          }
         }
         s_u[l] = u;
       }
@@ -1060,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
     for (int t = threadIdx.x; t < nk; t += nb) {
       A.u[k0 + t] = s_u[t];
       A.v[k0 + t] = s_v[t];
-      if (correct) A.a[k0 + t] = s_a[t];
+      if (correct && !leap_mid) A.a[k0 + t] = s_a[t];
     }
   }
 }
