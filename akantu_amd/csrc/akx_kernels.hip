@@ -713,8 +713,12 @@ __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
 //   Valid with the mixed-path CSR, which lists each node's slots by element
 //   id: the patch's contributions to one node are then one contiguous run.
 // ===========================================================================
+// At least 4 waves per SIMD: the 8-point hexahedron with finite strain would
+// otherwise take 151 VGPRs (3 waves); capped at 128 it spills a few values and
+// runs 6 % faster.  The other instantiations are far below the cap.
 template <int ET, int KIND>
-__global__ void __launch_bounds__(256) k_element_patch(ElemArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_element_patch(ElemArgs A) {
   using T = Elem<ET>;
   constexpr int d = T::d, n = T::n, dn = d * n;
   extern __shared__ __align__(16) unsigned char smem[];
