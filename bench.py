@@ -52,7 +52,7 @@ def log(msg):
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
                       patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0,
-                      partials=0, packed=False, partials_node_major=False):
+                      partials=0, packed=False, partials_node_major=False, leap=False):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -68,7 +68,9 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     packed: local node and run rank in one 16-bit word per (e, a), 3-node
     elements padded to 4; partials_node_major: the patch kernel reads each
     partial's node-major position and the node kernel streams its partials
-    (no entry list)."""
+    (no entry list); leap: the node kernel carries the half-step velocity
+    between steps (mixed path) and reads / writes u and v only, not a."""
+    state = (4 if leap else 6) * N * d * 8
     local = (E * (4 if n == 3 else n) * 2) if packed else E * n * (2 + 1)
     if partials:
         pos = patch_nodes * 4
@@ -76,7 +78,7 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
                 + 2 * N * d * 8 + patch_nodes * d * partials + state_bytes
                 + (pos if partials_node_major else 0))
         node = ((N + 1) * 4 + (0 if partials_node_major else pos) + patch_nodes * d * partials
-                + 6 * N * d * 8 + N * 8 + N * d
+                + state + N * 8 + N * d
                 + (N * d * 8 if has_fext else 0) + (N * 8 if dt_per_node else 0))
         return elem, node
     if patch_nodes is not None:
@@ -95,7 +97,7 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     elem += state_bytes
     node = ((N + 1) * 4 + (0 if node_ordered else E * n * 4)  # node -> slot CSR
             + E * n * d * pe         # force slots (read)
-            + 6 * N * d * 8          # u, v, a read + write
+            + state                  # u, v (and a) read + write
             + N * 8                  # lumped mass (one value per node)
             + N * d                  # blocked flags
             + (N * d * 8 if has_fext else 0)
@@ -246,7 +248,7 @@ def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
                                patch_blocks=model.query("patch_blocks") if patch else 0,
                                partials=partials,
                                packed=patch and model.query("patch_packed") == 1,
-                               partials_node_major=patch and lay == 5, **kw)
+                               partials_node_major=patch and lay == 5, leap=True, **kw)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms
