@@ -46,6 +46,87 @@ def test_pvd_lists_the_dumps(tmp_path):
     assert float(ds[1].get("timestep")) == 1e-3
 
 
+class _StubBuffer:
+    """Host stand-in for akx.PinnedBuffer (no device needed)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.raw = np.zeros(nbytes, dtype=np.uint8)
+
+    def array(self, dtype, shape):
+        return self.raw[:int(np.prod(shape)) * np.dtype(dtype).itemsize].view(dtype).reshape(shape)
+
+    def close(self):
+        pass
+
+
+class _StubModel:
+    """Answers the dumper's calls from numpy arrays; an asynchronous download
+    completes only when wait() is called, with the values of the moment the
+    download was requested (a snapshot), like akx_field_download_async."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.time_step = 1e-3
+        self.state = {"displacement": np.zeros(mesh.nodes.shape),
+                      "blocked_dofs": np.zeros(mesh.nodes.shape, dtype=bool),
+                      "stress": np.zeros((mesh.nb_elements * mesh.nb_qp, mesh.dim ** 2))}
+        self.pending = {}
+        self.waited = []
+
+    def field_layout(self, name):
+        a = self.state[name]
+        return a.shape, (np.uint8 if a.dtype == bool else np.float64)
+
+    def get(self, name):
+        return self.state[name].copy()
+
+    def get_async(self, name, buf):
+        ticket = len(self.pending) + 1
+        self.pending[ticket] = (buf, name, self.state[name].copy())
+        return ticket
+
+    def wait(self, ticket):
+        buf, name, snap = self.pending[ticket]
+        shape, dtype = self.field_layout(name)
+        buf.array(dtype, shape)[...] = snap.astype(dtype)
+        self.waited.append(ticket)
+
+    def step(self, k):
+        self.state["displacement"] += k
+        self.state["stress"] -= k
+        self.state["blocked_dofs"][: k % 3] = True
+
+
+def test_async_dumper_writes_snapshots_in_order(tmp_path, monkeypatch):
+    """The writer thread waits for every download, writes each file from the
+    state of its own dump() call although the model moved on, keeps the .pvd
+    in dump order and reuses the page-locked buffers."""
+    from akantu_amd import akx
+    monkeypatch.setattr(akx, "PinnedBuffer", _StubBuffer, raising=False)
+    m = akm.rectangle_tri3(3, 2, 1.0, 0.5)
+    model = _StubModel(m)
+    d = dumper.Dumper(model, "s", str(tmp_path), asynchronous=True, depth=2)
+    for f in ("displacement", "blocked_dofs", "stress"):
+        d.add_dump_field(f)
+    expected = []
+    for k in range(5):
+        expected.append({f: model.get(f) for f in ("displacement", "stress")})
+        d.dump()
+        model.step(k + 1)
+    d.flush()
+    assert sorted(model.waited) == list(range(1, 16))
+    for k in range(5):
+        _, arr = _arrays(tmp_path / ("s_%04d.vtu" % k))
+        assert np.array_equal(arr["displacement"][:, :2], expected[k]["displacement"])
+        assert np.array_equal(arr["stress"], expected[k]["stress"])
+    ds = ET.parse(tmp_path / "s.pvd").getroot().findall("Collection/DataSet")
+    assert [x.get("file") for x in ds] == ["s_%04d.vtu" % k for k in range(5)]
+    # at most `depth` dumps in flight: never more than depth + 1 buffers per field
+    assert all(len(v) <= 3 for v in d._free.values())
+    d.close()
+
+
 @pytest.mark.gpu
 def test_model_dump_writes_the_device_state(tmp_path):
     from akantu_amd import akx
