@@ -308,12 +308,15 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
       }
     return;
   }
-  // With the damage law, the quadrature loop stays rolled and non-simplex
-  // elements rebuilding B form its columns B_a = Ji dN_a/dxi where they are
-  // used (once for grad u, once for the forces) instead of holding them: the
-  // unrolled 8-point hexahedron needed 448 VGPRs + 192 AGPRs (one wave per
-  // SIMD), the rolled one 168.  The other laws keep the unrolled form, which
-  // is the leaner one for them.
+  // With the damage law the quadrature loop stays rolled (unrolled, its
+  // 8-point hexahedron needed 448 VGPRs + 192 AGPRs: one wave per SIMD) and
+  // non-simplex elements rebuilding the geometry never form B = Ji dN/dxi:
+  //   G(i,j) = sum_l H(i,l) Ji(j,l),    H(i,l) = sum_a du_a,i dN_a/dxi_l;
+  //   f_a,i += sum_l Qm(i,l) dN_a/dxi_l, Qm(i,l) = sum_j P(i,j) Ji(j,l),
+  // which holds fewer values live (18 spilled values instead of 39 under the
+  // 128-VGPR cap).  For the other laws the B form is the faster one (320 vs
+  // 335 us on the 128^3 neo-Hookean hexahedron mesh), though it needs more
+  // multiply-adds.
   constexpr bool on_the_fly = RECOMPUTE && !simplex && KIND == NH_DAMAGE;
   constexpr int QUNROLL = KIND == NH_DAMAGE ? 1 : Q;
 #pragma unroll QUNROLL
@@ -390,30 +393,34 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
       jq = det * A.tabf.w[q];
     }
     // column a of B: B(i,a) = sum_l Ji(i,l) dN_a/dxi_l, or the stored one
-    auto bcol = [&](int a, float* b) {
-#pragma unroll
-      for (int i = 0; i < d; ++i) {
-        if constexpr (on_the_fly) {
-          float s = 0.f;
-#pragma unroll
-          for (int l = 0; l < d; ++l) s += Jiq[i + l * d] * A.tabf.dnds[q][l + a * d];
-          b[i] = s;
-        } else {
-          b[i] = Bq[i + a * d];
-        }
-      }
-    };
     float G[d * d];
 #pragma unroll
     for (int k = 0; k < d * d; ++k) G[k] = 0.f;
+    if constexpr (on_the_fly) {
+      float H[d * d];
 #pragma unroll
-    for (int a = 1; a < n; ++a) {
-      float b[d];
-      bcol(a, b);
+      for (int k = 0; k < d * d; ++k) H[k] = 0.f;
+#pragma unroll
+      for (int a = 1; a < n; ++a)
+#pragma unroll
+        for (int l = 0; l < d; ++l) {
+          const float dn_al = A.tabf.dnds[q][l + a * d];
+#pragma unroll
+          for (int i = 0; i < d; ++i) H[i + l * d] += du[i + a * d] * dn_al;
+        }
 #pragma unroll
       for (int j = 0; j < d; ++j)
 #pragma unroll
-        for (int i = 0; i < d; ++i) G[i + j * d] += du[i + a * d] * b[j];
+        for (int l = 0; l < d; ++l)
+#pragma unroll
+          for (int i = 0; i < d; ++i) G[i + j * d] += H[i + l * d] * Jiq[j + l * d];
+    } else {
+#pragma unroll
+      for (int a = 1; a < n; ++a)
+#pragma unroll
+        for (int j = 0; j < d; ++j)
+#pragma unroll
+          for (int i = 0; i < d; ++i) G[i + j * d] += du[i + a * d] * Bq[j + a * d];
     }
     float P[d * d];  // first Piola-Kirchhoff (Cauchy at small strain) times det(J) w
     if constexpr (KIND == ELASTIC) {
@@ -454,17 +461,36 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
         }
     }
     // f_a,i += sum_j P_ij dN_a/dX_j
+    if constexpr (on_the_fly) {
+      float Qm[d * d];
 #pragma unroll
-    for (int a = 0; a < n; ++a) {
-      float b[d];
-      bcol(a, b);
+      for (int l = 0; l < d; ++l)
 #pragma unroll
-      for (int i = 0; i < d; ++i) {
-        float s = fe[i + a * d];
+        for (int i = 0; i < d; ++i) {
+          float s = 0.f;
 #pragma unroll
-        for (int j = 0; j < d; ++j) s += P[i + j * d] * b[j];
-        fe[i + a * d] = s;
-      }
+          for (int j = 0; j < d; ++j) s += P[i + j * d] * Jiq[j + l * d];
+          Qm[i + l * d] = s;
+        }
+#pragma unroll
+      for (int a = 0; a < n; ++a)
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          float s = fe[i + a * d];
+#pragma unroll
+          for (int l = 0; l < d; ++l) s += Qm[i + l * d] * A.tabf.dnds[q][l + a * d];
+          fe[i + a * d] = s;
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < n; ++a)
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          float s = fe[i + a * d];
+#pragma unroll
+          for (int j = 0; j < d; ++j) s += P[i + j * d] * Bq[j + a * d];
+          fe[i + a * d] = s;
+        }
     }
   }
 }
