@@ -78,11 +78,13 @@ def test_ensemble_mixed_error_per_specimen():
         assert err < 1e-5, (k, err)
 
 
-@pytest.mark.parametrize("precision", [akx.FP64, akx.MIXED])
-def test_graph_replays_match_eager(precision):
+@pytest.mark.parametrize("precision,layout", [(akx.FP64, 0), (akx.MIXED, 0), (akx.MIXED, 4),
+                                              (akx.MIXED, 5)])
+def test_graph_replays_match_eager(precision, layout):
     """Replays of one captured graph, with host work and synchronisations in
     between, give what the eager launches give, every time; changing the time
-    step re-captures."""
+    step re-captures.  Layouts 4 and 5: the patch kernel and the per-patch
+    partial sums inside the graph."""
     m = akm.bar_tri3(60, 12)
     X = m.nodes
     mask = np.zeros_like(X, dtype=bool)
@@ -97,6 +99,7 @@ def test_graph_replays_match_eager(precision):
         g.set("blocked_dofs", mask)
         g.set("velocity", v0)
         g.set_time_step(0.8 * g.get_stable_time_step())
+        g.set_launch(slot_layout=layout)
         models.append(g)
     eager, graph = models
     for r in range(4):
