@@ -73,6 +73,9 @@ void free_derived(akx_ctx* c) {
   dfree(c->pcsr_off); dfree(c->pentry); dfree(c->ppos);
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
+  dfree(c->fel_off); dfree(c->fel); dfree(c->flslot); dfree(c->fch_off); dfree(c->fcbase);
+  dfree(c->u_alt);
+  c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
@@ -278,14 +281,14 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
     std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
 }
 
-// Mixed path with per-patch partial sums (slot layouts 3, 4 and 5): the
-// element kernel reduces each node's contributions inside its patch in FP64
-// and stores one FP64 (3) or FP32 (4, 5) partial per (patch, node), in patch
-// order (3, 4) or at the node's place in a node-major list (5); the node
-// kernel sums a node's partials in FP64.  Needs the LDS patches and a single
-// launch group.
+// Mixed path with per-patch partial sums (slot layouts 3, 4 and 5; 6, the
+// fused FP64 step, runs as 5 on the mixed path): the element kernel reduces
+// each node's contributions inside its patch in FP64 and stores one FP64 (3)
+// or FP32 (4, 5) partial per (patch, node), in patch order (3, 4) or at the
+// node's place in a node-major list (5); the node kernel sums a node's
+// partials in FP64.  Needs the LDS patches and a single launch group.
 bool uses_partials(const akx_ctx* c, int precision) {
-  return precision == AKX_MIXED && c->node_ordered >= 3 && c->node_ordered <= 5 &&
+  return precision == AKX_MIXED && c->node_ordered >= 3 && c->node_ordered <= 6 &&
          c->patch_ok && c->groups.size() == 1 &&
          !c->groups[0].filter &&
          !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
@@ -296,6 +299,13 @@ bool uses_partials(const akx_ctx* c, int precision) {
 // per-patch partials, FP32 otherwise
 int node_precision(const akx_ctx* c, int precision) {
   return uses_partials(c, precision) ? AKX_FP64 : precision;
+}
+
+// FP64 step fused over node blocks (slot layout 6): one launch per step, no
+// force slots; stateless laws of a single kind.
+bool uses_fused(const akx_ctx* c, int precision) {
+  return precision == AKX_FP64 && c->node_ordered == 6 && c->fused_ok && c->groups.size() == 1 &&
+         !c->groups[0].filter;
 }
 
 ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
@@ -318,7 +328,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.slot_pos = !c->node_ordered ? nullptr : precision == AKX_MIXED ? c->slot2_pos : c->slot_pos;
   A.patch = (precision == AKX_MIXED && c->node_ordered >= 2 && c->patch_ok && !g.filter) ? 1 : 0;
   A.partials = (A.patch && uses_partials(c, precision)) ? (c->node_ordered == 3 ? 1 : 2) : 0;
-  A.part_pos = (A.partials && c->node_ordered == 5) ? c->ppos : nullptr;
+  A.part_pos = (A.partials && c->node_ordered >= 5) ? c->ppos : nullptr;
   A.patch_max_nodes = c->patch_max_nodes;
   A.patch_lds = c->patch_lds;
   A.patch_blocks = c->patch_blocks;
@@ -379,7 +389,7 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
     // contiguous run per node when the partials are node-ordered
     A.csr_off = c->pcsr_off;
     A.csr_slot = c->pentry;
-    A.node_ordered = c->node_ordered == 5 ? 1 : 0;
+    A.node_ordered = c->node_ordered >= 5 ? 1 : 0;
     return A;
   }
   if (c->node_ordered && c->node_stage) {
@@ -546,6 +556,90 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
   c->patch_lds = int(lds);
   c->patch_blocks = nb;
   c->patch_ok = true;
+  return AKX_OK;
+}
+
+// Node blocks of the fused FP64 step: for block b, the elements touching its
+// nodes in assembly order, the LDS slot of each (element, node) of the block
+// and, per chunk of FUSED_CH elements, where each node's run starts.  `order`
+// is the assembly order (caller element ids), `off`/`slot` the FP64 CSR.
+// Leaves c->fused_ok false for laws with history (the step recomputes
+// elements) and for several laws in one mesh.
+int build_fused(akx_ctx* c, const std::vector<uint32_t>& order, const std::vector<uint32_t>& off,
+                const std::vector<uint32_t>& slot) {
+  c->fused_ok = false;
+  const int64_t E = c->E, N = c->N;
+  const int n = c->n, d = c->d;
+  if (E == 0) return AKX_OK;
+  const int kind = c->mats[c->elem_mat[0]].kind;
+  if (kind != AKX_MATERIAL_ELASTIC && kind != AKX_MATERIAL_NEOHOOKEAN) return AKX_OK;
+  for (int64_t e = 1; e < E; ++e)
+    if (c->mats[c->elem_mat[e]].kind != kind) return AKX_OK;
+  if (int64_t(FUSED_CH) * n > 0xFFFF) return AKX_OK;
+  std::vector<uint32_t> rank(E);  // assembly rank of each internal element
+  for (int64_t r = 0; r < E; ++r) rank[c->elem_inv[order[r]]] = uint32_t(r);
+  const int64_t nblk = (N + FUSED_NB - 1) / FUSED_NB;
+  std::vector<uint32_t> el_off(nblk + 1, 0), ch_off(nblk + 1, 0), el;
+  std::vector<uint16_t> lslot, cbase;
+  std::vector<uint32_t> L;
+  std::vector<uint32_t> cnt, run;
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t n0 = b * FUSED_NB, n1 = std::min<int64_t>(N, n0 + FUSED_NB);
+    L.clear();
+    for (int64_t k = n0; k < n1; ++k)
+      for (uint32_t p = off[k]; p < off[k + 1]; ++p) L.push_back(slot[p] / n);
+    std::sort(L.begin(), L.end(), [&](uint32_t x, uint32_t y) { return rank[x] < rank[y]; });
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    const size_t nch = (L.size() + FUSED_CH - 1) / FUSED_CH;
+    auto pos = [&](uint32_t ei) {  // index of ei in L (sorted by rank)
+      return size_t(std::lower_bound(L.begin(), L.end(), ei,
+                                     [&](uint32_t x, uint32_t y) { return rank[x] < rank[y]; }) -
+                    L.begin());
+    };
+    cnt.assign(nch * FUSED_NB, 0);
+    for (int64_t k = n0; k < n1; ++k)
+      for (uint32_t p = off[k]; p < off[k + 1]; ++p)
+        cnt[(pos(slot[p] / n) / FUSED_CH) * FUSED_NB + size_t(k - n0)]++;
+    const size_t cb0 = cbase.size();
+    for (size_t ch = 0; ch < nch; ++ch) {
+      uint32_t acc = 0;
+      for (int k = 0; k < FUSED_NB; ++k) {
+        cbase.push_back(uint16_t(acc));
+        acc += cnt[ch * FUSED_NB + k];
+      }
+      cbase.push_back(uint16_t(acc));
+    }
+    const size_t l0 = lslot.size();
+    lslot.resize(l0 + L.size() * n, 0xFFFF);
+    // each node's entries in CSR (assembly) order fill its run of each chunk
+    run.assign(nch * FUSED_NB, 0);
+    for (int64_t k = n0; k < n1; ++k)
+      for (uint32_t p = off[k]; p < off[k + 1]; ++p) {
+        const uint32_t ei = slot[p] / n, a = slot[p] % n;
+        const size_t j = pos(ei), ch = j / FUSED_CH, kl = size_t(k - n0);
+        const uint32_t at = cbase[cb0 + ch * (FUSED_NB + 1) + kl] + run[ch * FUSED_NB + kl]++;
+        lslot[l0 + j * n + a] = uint16_t(at);
+      }
+    el.insert(el.end(), L.begin(), L.end());
+    el_off[b + 1] = uint32_t(el.size());
+    ch_off[b + 1] = ch_off[b] + uint32_t(nch);
+  }
+  if (el.size() >= (size_t(1) << 32)) return AKX_OK;
+  HIP_TRY(c, dalloc(&c->fel_off, el_off.size()));
+  HIP_TRY(c, dalloc(&c->fel, el.size()));
+  HIP_TRY(c, dalloc(&c->flslot, lslot.size()));
+  HIP_TRY(c, dalloc(&c->fch_off, ch_off.size()));
+  HIP_TRY(c, dalloc(&c->fcbase, cbase.size()));
+  HIP_TRY(c, dalloc(&c->u_alt, size_t(N) * d));
+  HIP_TRY(c, hipMemcpy(c->fel_off, el_off.data(), el_off.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->fel, el.data(), el.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->flslot, lslot.data(), lslot.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->fch_off, ch_off.data(), ch_off.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->fcbase, cbase.data(), cbase.size() * 2, hipMemcpyHostToDevice));
+  c->fused_blocks = nblk;
+  c->fused_elements = int64_t(el.size());
+  c->fused_lds = FUSED_CH * n * d * 8;
+  c->fused_ok = true;
   return AKX_OK;
 }
 
@@ -794,6 +888,7 @@ int akx_init(akx_ctx* c) {
     return AKX_OK;
   };
   if (int rc = upload_pos(slot, &c->slot_pos)) return rc;
+  if (int rc = build_fused(c, order, off, slot)) return rc;
   {
     // The mixed path needs a fixed summation order but not the reference's:
     // its CSR lists each node's contributions by internal element id, so that
@@ -1238,6 +1333,40 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
     NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0) | leap, precision);
     HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
   }
+  if (uses_fused(c, precision)) {
+    // one fused launch per step; u alternates between c->u and c->u_alt
+    double* ucur = c->u;
+    double* unext = c->u_alt;
+    const KindGroup& g = c->groups[0];
+    for (int s = 1; s <= nsteps; ++s) {
+      const bool last = (s == nsteps);
+      if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
+      int mode = NODE_CORRECT;
+      if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
+      else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
+      NodeArgs NA = node_args(c, mode, precision);
+      NA.u = ucur;
+      ElemArgs EA = elem_args(c, g, precision);
+      FusedArgs F;
+      F.el_off = c->fel_off;
+      F.el = c->fel;
+      F.lslot = c->flslot;
+      F.ch_off = c->fch_off;
+      F.cbase = c->fcbase;
+      F.u_out = unext;
+      F.nblk = c->fused_blocks;
+      HIP_TRY(c, launch_fused(c->et, g.kind, c->mixed_recompute != 0, c->has_fext, c->stream, EA,
+                              NA, F, c->fused_lds));
+      if (timing) {
+        HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
+        HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
+      }
+      std::swap(ucur, unext);
+    }
+    // the displacement ends where the last step wrote it
+    if (ucur != c->u)
+      HIP_TRY(c, launch_snapshot(c->stream, int64_t(c->N) * d, 1, 1, nullptr, ucur, c->u, 8));
+  } else
   for (int s = 1; s <= nsteps; ++s) {
     const bool last = (s == nsteps);
     if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
@@ -1465,6 +1594,8 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "patch_max_nodes") *value = double(c->patch_ok ? c->patch_max_nodes : 0);
   else if (k == "patch_available") *value = c->patch_ok ? 1. : 0.;
   else if (k == "patch_packed") *value = (c->patch_ok && c->lcr) ? 1. : 0.;
+  else if (k == "fused_available") *value = c->fused_ok ? 1. : 0.;
+  else if (k == "fused_elements") *value = double(c->fused_ok ? c->fused_elements : 0);
   else if (k == "element_block") *value = c->elem_block;
   else if (k == "node_block") *value = c->node_block;
   else if (k == "xcd_remap") *value = c->xcd_remap;
@@ -1665,6 +1796,10 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
               cand.push_back(Shape{eb, nb, x, rcmp, lay, stage});
             }
           }
+  // the fused FP64 step (layout 6) has its own block shapes
+  if (precision == AKX_FP64 && c->fused_ok)
+    for (int rcmp = 0; rcmp < 2; ++rcmp)
+      for (int x = 0; x < 2; ++x) cand.push_back(Shape{256, 256, x, rcmp, 6, 0});
   for (const Shape& sh : cand) {
           const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;
           c->elem_block = eb; c->node_block = nb; c->xcd_remap = x; c->node_ordered = lay;
@@ -1705,7 +1840,7 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
   if (!ok(element_block) || !ok(node_block))
     return fail(c, AKX_ERR_ARGUMENT, "block size must be 64, 128, 192 or 256");
-  if (slot_layout > 5) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0 to 5");
+  if (slot_layout > 6) return fail(c, AKX_ERR_ARGUMENT, "slot layout is 0 to 6");
   if (element_block) c->elem_block = element_block;
   if (node_block) c->node_block = node_block;
   if (xcd_remap >= 0) c->xcd_remap = xcd_remap ? 1 : 0;

@@ -98,6 +98,20 @@ struct NodeArgs {
   int partials;           // slots are per-patch partial sums, 1 FP64, 2 FP32 (csr_off/csr_slot: node -> entries)
 };
 
+// Fused FP64 step over node blocks (stateless laws, slot layout 6): block b
+// owns nodes [b * FUSED_NB, (b + 1) * FUSED_NB) and recomputes the elements
+// around them in chunks of FUSED_CH, in the reference's assembly order.
+constexpr int FUSED_NB = 256, FUSED_CH = 256;
+struct FusedArgs {
+  const uint32_t* el_off;  // [nblk + 1] offsets into el
+  const uint32_t* el;      // internal element ids of each block, assembly order
+  const uint16_t* lslot;   // [j * n + a] LDS slot of (element j, node a); 0xFFFF: not a block node
+  const uint32_t* ch_off;  // [nblk + 1] first chunk table of each block
+  const uint16_t* cbase;   // [chunk * (FUSED_NB + 1) + k] first slot of block node k in the chunk
+  double* u_out;           // displacement written by the step (read from NodeArgs::u)
+  int64_t nblk;
+};
+
 enum NodeMode {
   NODE_CORRECT = 1,     // gather forces, residual, lumped solve, corrector
   NODE_PREDICT = 2,     // Dirichlet increments, save previous, predictor
@@ -138,6 +152,8 @@ hipError_t launch_mass(int et, hipStream_t s, int64_t N, int64_t E, const uint32
                        const uint16_t* elem_mat, const MatParams* mats, double* mass);
 hipError_t launch_element(int et, int kind, int precision, hipStream_t s, const ElemArgs& A,
                           int block);
+hipError_t launch_fused(int et, int kind, bool recompute, bool has_fext, hipStream_t s,
+                        const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F, int lds);
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
                        int block);
 hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
@@ -286,6 +302,13 @@ struct akx_ctx {
   uint8_t* lrank = nullptr;
   // node -> entries of the per-patch partial sums (slot layout 3)
   uint32_t *pcsr_off = nullptr, *pentry = nullptr, *ppos = nullptr;
+  // fused FP64 step (slot layout 6)
+  bool fused_ok = false;
+  int fused_lds = 0;
+  int64_t fused_blocks = 0, fused_elements = 0;
+  uint32_t *fel_off = nullptr, *fel = nullptr, *fch_off = nullptr;
+  uint16_t *flslot = nullptr, *fcbase = nullptr;
+  double* u_alt = nullptr;  // second displacement buffer of the fused step
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal

@@ -160,30 +160,18 @@ __device__ __forceinline__ void store_slots(const ElemArgs& A, int64_t e, const 
 // RECOMPUTE = true: B and det(J) w rebuilt per point from the coordinates
 // (point_shapes_ref) instead of read from the FP64 tables: E*Q*(d*n+1)*8
 // bytes of table reads become one more gather of the nodes' coordinates.
+// Element forces in the reference's FP64 operation order from the element's
+// nodal displacements ue and (RECOMPUTE) coordinates xe, d x n column-major.
+// Shared by the element kernel and the fused FP64 step.
 template <int ET, int KIND, bool RECOMPUTE>
-__global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
+__device__ __forceinline__ void element_forces_fp64(const ElemArgs& A, int64_t e,
+                                                    const MatParams& m, const double* ue,
+                                                    const double* xe, double* fe) {
   using T = Elem<ET>;
   constexpr int d = T::d, n = T::n, Q = T::Q, dn = d * n;
-  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
-  if (t >= A.count) return;
   const int64_t E = A.E;
-  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
-  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
   const double* __restrict__ Bt = static_cast<const double*>(A.B);
   const double* __restrict__ Jt = static_cast<const double*>(A.J);
-  // FEEngine::extractNodalToElementField (fe_engine_inline_impl.hh:104-143)
-  double ue[dn];
-  double xe[RECOMPUTE ? dn : 1];
-#pragma unroll
-  for (int a = 0; a < n; ++a) {
-    const int64_t node = A.conn[a * E + e];
-#pragma unroll
-    for (int i = 0; i < d; ++i) {
-      ue[i + a * d] = A.u[node * d + i];
-      if constexpr (RECOMPUTE) xe[i + a * d] = A.X[node * d + i];
-    }
-  }
-  double fe[dn];
 #pragma unroll
   for (int k = 0; k < dn; ++k) fe[k] = 0.;
 #pragma unroll
@@ -219,6 +207,31 @@ __global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
 #pragma unroll
     for (int k = 0; k < dn; ++k) fe[k] += R[k] * jq;
   }
+}
+
+template <int ET, int KIND, bool RECOMPUTE>
+__global__ void __launch_bounds__(256) k_element_fp64(ElemArgs A) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, dn = d * n;
+  const int64_t t = logical_block(A.xcd_remap) * blockDim.x + threadIdx.x;
+  if (t >= A.count) return;
+  const int64_t E = A.E;
+  const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
+  const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
+  // FEEngine::extractNodalToElementField (fe_engine_inline_impl.hh:104-143)
+  double ue[dn];
+  double xe[RECOMPUTE ? dn : 1];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    const int64_t node = A.conn[a * E + e];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      ue[i + a * d] = A.u[node * d + i];
+      if constexpr (RECOMPUTE) xe[i + a * d] = A.X[node * d + i];
+    }
+  }
+  double fe[dn];
+  element_forces_fp64<ET, KIND, RECOMPUTE>(A, e, m, ue, xe, fe);
   store_slots<d, n>(A, e, fe);
 }
 
@@ -919,6 +932,86 @@ k_element_patch(ElemArgs A) {
 // FP64 slots: plain sum in slot order (bit-identical to the sequential
 // scatter).  FP32 slots: FP64 Neumaier-compensated sum.
 // ===========================================================================
+// One node's residual, lumped solve and corrector (NODE_CORRECT), with the
+// assembled internal force fi (sum of -fe) and the node's own state u, v, a
+// (d values each), external force fext (nullptr: none) and blocked flags b.
+// Also writes f_int (NODE_STORE_FINT) and the increment (NODE_STORE_INC).
+// Shared by the node kernel and the fused FP64 step.
+template <int D, bool HAS_FEXT>
+__device__ __forceinline__ void node_correct(const NodeArgs& A, int64_t node, double dt,
+                                             const double* fi, const double* fext,
+                                             const uint8_t* b, double* u, double* v, double* a) {
+  const int mode = A.mode;
+  const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
+  const bool leap = mode & NODE_LEAP;
+  const double M = A.mass[node];
+  const double dcoef = 0.5 * dt;            // beta dt
+  const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    if (mode & NODE_STORE_FINT) A.fint_out[node * D + i] = fi[i];
+    if (!correct) continue;
+    double r = 0.;
+    if constexpr (HAS_FEXT) r += fext[i];
+    r += fi[i];
+    if (leap) {
+      if (!(b[i] & 1)) {
+        const double an = r / M;
+        if (predict) {
+          v[i] += dt * an;    // v_{n+1/2}
+        } else {
+          v[i] += dcoef * an; // v_n
+          a[i] = an;
+        }
+      }
+    } else {
+      double cache = 0.;
+      cache += -1. * a[i];
+      r += M * cache;
+      if (!(b[i] & 1)) {
+        const double delta = r / M;
+        u[i] += ecoef * delta;
+        v[i] += dcoef * delta;
+        a[i] += delta;
+      }
+    }
+    if (mode & NODE_STORE_INC) {
+      double inc = u[i];
+      inc -= A.prev[node * D + i];
+      A.inc[node * D + i] = inc;
+    }
+  }
+}
+
+// Dirichlet increments, previous displacement and predictor (NODE_PREDICT)
+// of one node.
+template <int D>
+__device__ __forceinline__ void node_predict(const NodeArgs& A, int64_t node, double dt,
+                                             const uint8_t* b, double* u, double* v, double* a) {
+  const int mode = A.mode;
+  const bool correct = mode & NODE_CORRECT;
+  const bool leap = mode & NODE_LEAP;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double ui = u[i];
+    if (b[i] & 2) ui += A.bc_inc[i];
+    if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = ui;
+    if (!(b[i] & 1)) {
+      if (leap) {
+        // v_{n+1/2}: formed here on the first step, by the corrector after
+        if (!correct) v[i] += 0.5 * dt * a[i];
+        ui += dt * v[i];
+      } else {
+        const double vi = v[i], acc = a[i];
+        const double dt_a_n = dt * acc;
+        ui += dt * vi + 0.5 * dt * dt_a_n;
+        v[i] = vi + dt_a_n;
+      }
+    }
+    u[i] = ui;
+  }
+}
+
 // SRC: what the node gathers -- 0 FP64 element slots, 1 FP32 element slots,
 // 2 FP64 per-patch partial sums, 3 FP32 per-patch partial sums (FP32 values
 // are summed in FP64 with Neumaier compensation).
@@ -1041,69 +1134,17 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
 #pragma unroll
             for (int i = 0; i < D; ++i) add(i, x[j][i]);
       }
-      const double M = A.mass[node];
-      const double dcoef = 0.5 * dt;            // beta dt
-      const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
+      double fi[D];
 #pragma unroll
       for (int i = 0; i < D; ++i) {
-        const int l = ln * D + i;
-        double fi = f[i];
-        if constexpr (sizeof(TS) != 8) fi = f[i] + c[i];
-        if (mode & NODE_STORE_FINT) A.fint_out[node * D + i] = fi;
-        if (!correct) continue;
-        double r = 0.;
-        if constexpr (HAS_FEXT) r += s_f[l];
-        r += fi;
-        if (leap) {
-          if (!(s_b[l] & 1)) {
-            const double an = r / M;
-            if (predict) {
-              s_v[l] += dt * an;    // v_{n+1/2}
-            } else {
-              s_v[l] += dcoef * an; // v_n
-              s_a[l] = an;
-            }
-          }
-        } else {
-          double cache = 0.;
-          cache += -1. * s_a[l];
-          r += M * cache;
-          if (!(s_b[l] & 1)) {
-            const double delta = r / M;
-            s_u[l] += ecoef * delta;
-            s_v[l] += dcoef * delta;
-            s_a[l] += delta;
-          }
-        }
-        if (mode & NODE_STORE_INC) {
-          double inc = s_u[l];
-          inc -= A.prev[node * D + i];
-          A.inc[node * D + i] = inc;
-        }
+        fi[i] = f[i];
+        if constexpr (sizeof(TS) != 8) fi[i] = f[i] + c[i];
       }
+      node_correct<D, HAS_FEXT>(A, node, dt, fi, HAS_FEXT ? &s_f[ln * D] : nullptr, &s_b[ln * D],
+                                &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
     }
     if (predict) {
-#pragma unroll
-      for (int i = 0; i < D; ++i) {
-        const int l = ln * D + i;
-        const uint8_t b = s_b[l];
-        double u = s_u[l];
-        if (b & 2) u += A.bc_inc[i];
-        if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = u;
-        if (!(b & 1)) {
-          if (leap) {
-            // v_{n+1/2}: formed here on the first step, by the corrector after
-            if (!correct) s_v[l] += 0.5 * dt * s_a[l];
-            u += dt * s_v[l];
-          } else {
-            const double v = s_v[l], acc = s_a[l];
-            const double dt_a_n = dt * acc;
-            u += dt * v + 0.5 * dt * dt_a_n;
-            s_v[l] = v + dt_a_n;
-          }
-        }
-        s_u[l] = u;
-      }
+      node_predict<D>(A, node, dt, &s_b[ln * D], &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
     }
   }
   __syncthreads();
@@ -1113,6 +1154,108 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
       A.v[k0 + t] = s_v[t];
       if (correct && !leap_mid) A.a[k0 + t] = s_a[t];
     }
+  }
+}
+
+// ===========================================================================
+// fused FP64 step over node blocks (stateless laws, slot layout 6)
+//   Block b owns FUSED_NB consecutive nodes.  The elements touching them are
+//   listed per block in the reference's assembly order; the block walks that
+//   list in chunks of FUSED_CH elements: every thread computes one element's
+//   forces (element_forces_fp64, the element kernel's arithmetic), writes the
+//   contributions to the block's own nodes into LDS at their place in the
+//   node's run for this chunk, and each node thread adds its run in order --
+//   the chunks follow the assembly order, so every node's sum is the same
+//   sequence of additions as the slot path's.  Then each node thread runs the
+//   node kernel's update on its node.  Elements on block boundaries are
+//   computed by every block they touch; no force slot goes through HBM.
+//   u is read from NodeArgs::u and written to FusedArgs::u_out: other blocks
+//   still read the old displacements while this one updates its nodes.
+// ===========================================================================
+template <int ET, int KIND, bool RECOMPUTE, bool HAS_FEXT>
+__global__ void __launch_bounds__(256) k_step_fused(ElemArgs EA, NodeArgs NA, FusedArgs F) {
+  static_assert(KIND != NH_DAMAGE, "the fused step recomputes elements: stateless laws only");
+  static_assert(FUSED_NB <= 256 && FUSED_CH == 256, "one node and one element per thread of 256");
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, dn = d * n;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* s_f = reinterpret_cast<double*>(smem);  // FUSED_CH * n contributions, d each
+  const int64_t b = logical_block(NA.xcd_remap);
+  if (b >= F.nblk) return;  // uniform over the block
+  const int t = threadIdx.x;
+  const int64_t n0 = b * FUSED_NB;
+  const int cnt = int(min<int64_t>(FUSED_NB, NA.N - n0));
+  const bool owner = t < cnt;
+  const int64_t node = n0 + t;
+  const int64_t E = EA.E;
+  const double* __restrict__ u_in = NA.u;
+  double f[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) f[i] = 0.;
+  const uint32_t j0 = F.el_off[b], j1 = F.el_off[b + 1];
+  const uint32_t c0 = F.ch_off[b];
+  uint32_t c = 0;
+  for (uint32_t jc = j0; jc < j1; jc += FUSED_CH, ++c) {
+    const uint32_t j = jc + uint32_t(t);
+    if (j < j1) {
+      const int64_t e = F.el[j];
+      const MatParams m = EA.mats[EA.elem_mat ? EA.elem_mat[e] : 0];
+      double ue[dn];
+      double xe[RECOMPUTE ? dn : 1];
+#pragma unroll
+      for (int a = 0; a < n; ++a) {
+        const int64_t nd = EA.conn[a * E + e];
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          ue[i + a * d] = u_in[nd * d + i];
+          if constexpr (RECOMPUTE) xe[i + a * d] = EA.X[nd * d + i];
+        }
+      }
+      double fe[dn];
+      element_forces_fp64<ET, KIND, RECOMPUTE>(EA, e, m, ue, xe, fe);
+#pragma unroll
+      for (int a = 0; a < n; ++a) {
+        const uint32_t ls = F.lslot[size_t(j) * n + a];
+        if (ls != 0xFFFFu)
+#pragma unroll
+          for (int i = 0; i < d; ++i) s_f[ls * d + i] = fe[i + a * d];
+      }
+    }
+    __syncthreads();
+    if (owner) {
+      // this chunk's run of the node, in assembly order (the node kernel's
+      // sum: f += -1 * contribution)
+      const uint16_t* cb = F.cbase + size_t(c0 + c) * (FUSED_NB + 1);
+      const int q0 = cb[t], q1 = cb[t + 1];
+      for (int q = q0; q < q1; ++q)
+#pragma unroll
+        for (int i = 0; i < d; ++i) {
+          const double x = -1. * s_f[q * d + i];
+          f[i] += x;
+        }
+    }
+    __syncthreads();
+  }
+  if (!owner) return;
+  const double dt = NA.dt_node ? NA.dt_node[node] : NA.dt;
+  double u[d], v[d], acc[d], fx[d];
+  uint8_t bl[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    u[i] = u_in[node * d + i];
+    v[i] = NA.v[node * d + i];
+    acc[i] = NA.a[node * d + i];
+    bl[i] = NA.blocked[node * d + i];
+    fx[i] = HAS_FEXT ? NA.fext[node * d + i] : 0.;
+  }
+  if (NA.mode & (NODE_CORRECT | NODE_GATHER))
+    node_correct<d, HAS_FEXT>(NA, node, dt, f, fx, bl, u, v, acc);
+  if (NA.mode & NODE_PREDICT) node_predict<d>(NA, node, dt, bl, u, v, acc);
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    F.u_out[node * d + i] = u[i];
+    NA.v[node * d + i] = v[i];
+    NA.a[node * d + i] = acc[i];
   }
 }
 
@@ -1567,6 +1710,34 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
     else k_node<D, 1, false><<<g, block, A.stage_lds, s>>>(A);
   }
   return hipGetLastError();
+}
+
+template <int ET, int KIND>
+static hipError_t launch_fused_et(bool recompute, bool has_fext, hipStream_t s,
+                                  const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F,
+                                  int lds) {
+  const unsigned g = unsigned(F.nblk);
+  if (recompute) {
+    if (has_fext) k_step_fused<ET, KIND, true, true><<<g, 256, lds, s>>>(EA, NA, F);
+    else k_step_fused<ET, KIND, true, false><<<g, 256, lds, s>>>(EA, NA, F);
+  } else {
+    if (has_fext) k_step_fused<ET, KIND, false, true><<<g, 256, lds, s>>>(EA, NA, F);
+    else k_step_fused<ET, KIND, false, false><<<g, 256, lds, s>>>(EA, NA, F);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(int et, int kind, bool recompute, bool has_fext, hipStream_t s,
+                        const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F, int lds) {
+  clear_error();
+  if (F.nblk == 0) return hipSuccess;
+  if (kind != ELASTIC && kind != NEOHOOKEAN) return hipErrorInvalidValue;
+#define F_(ET)                                                                       \
+  return kind == ELASTIC ? launch_fused_et<ET, ELASTIC>(recompute, has_fext, s, EA, NA, F, lds) \
+                         : launch_fused_et<ET, NEOHOOKEAN>(recompute, has_fext, s, EA, NA, F, lds);
+  AKX_ET_SWITCH(et, F_)
+#undef F_
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,

@@ -230,10 +230,13 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * patches (2, mixed path) or one partial sum per (patch, node), summed in
  * FP64 and stored in FP64 (3, mixed path) or FP32 (4, mixed path) in patch
  * order, or in FP32 at the node's place in a node-major list (5, mixed
- * path), and whether the node kernel copies its block's contiguous slot
- * range to LDS first (node_stage 1/0) (-1 keeps).  On the FP64 path none of
- * these changes a result bit; on the mixed path layouts 0/1, 2, 3 and 4/5
- * differ in rounding only (patch-relative FP32 node values, partial sums). */
+ * path), or no slots at all (6, FP64 path: one fused launch per step over
+ * node blocks that recompute the elements around their nodes; stateless
+ * laws of one kind; the mixed path runs 6 as 5), and whether the node
+ * kernel copies its block's contiguous slot range to LDS first (node_stage
+ * 1/0) (-1 keeps).  On the FP64 path none of these changes a result bit; on
+ * the mixed path layouts 0/1, 2, 3 and 4/5 differ in rounding only
+ * (patch-relative FP32 node values, partial sums). */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 
@@ -241,7 +244,8 @@ int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_rema
  *   "nb_nodes", "nb_elements", "csr_entries", "patch_blocks",
  *   "patch_nodes_total" (sum over LDS patches of their distinct nodes),
  *   "patch_max_nodes", "patch_available" (1/0), "patch_packed" (1/0: 16-bit
- *   patch-local node | rank words),
+ *   patch-local node | rank words), "fused_available" (1/0: slot layout 6),
+ *   "fused_elements" (element evaluations of one fused step),
  *   "element_block", "node_block", "xcd_remap", "mixed_recompute", "slot_layout". */
 int akx_query(akx_ctx* ctx, const char* key, double* value);
 

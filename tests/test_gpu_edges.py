@@ -45,8 +45,9 @@ def _one_element_pair(name, kind=akx.ELASTIC):
 @pytest.mark.parametrize("name", sorted(ONE_ELEMENT))
 def test_single_element_fp64_every_layout(name):
     """FP64 on one element == oracle bit for bit, whatever the launch shape
-    (layouts 2-5 fall back to node-ordered slots on the FP64 path)."""
-    for layout in range(6):
+    (layouts 2-5 fall back to node-ordered slots on the FP64 path, 6 is the
+    fused node-block step)."""
+    for layout in range(7):
         for recompute in (0, 1):
             p = _one_element_pair(name)
             p.gpu.set_launch(64, 64, 1, recompute, layout, 0)
@@ -60,7 +61,7 @@ def test_single_element_fp64_every_layout(name):
 def test_single_element_mixed_every_layout(name, kind):
     """Mixed path on one element (a patch of one element, a node block of a
     few nodes): <= 1e-5 from the oracle after 40 steps for every layout."""
-    for layout in range(6):
+    for layout in range(7):
         for recompute in ((0, 1) if layout < 2 else (1,)):
             p = _one_element_pair(name, kind)
             p.gpu.set_launch(256, 256, 1, recompute, layout, 0)
@@ -99,10 +100,10 @@ def test_abi_rejects_bad_arguments_and_states():
         _check(lib.akx_solve_step(h, 1, akx.FP64), -2)        # no time step yet
         _check(lib.akx_set_time_step(h, -1.0), -1)
         _check(lib.akx_set_time_step(h, 1e-6), 0)
-        # sizes must match exactly; layouts are 0..5; tickets must exist
+        # sizes must match exactly; layouts are 0..6; tickets must exist
         _check(lib.akx_field_upload(h, 0, out.ctypes.data_as(ctypes.c_void_p), out.nbytes - 8), -1)
         _check(lib.akx_field_download(h, 0, out.ctypes.data_as(ctypes.c_void_p), out.nbytes + 8), -1)
-        _check(lib.akx_set_launch(h, 0, 0, -1, -1, 6, -1), -1)
+        _check(lib.akx_set_launch(h, 0, 0, -1, -1, 7, -1), -1)
         _check(lib.akx_download_wait(h, 12345), -1)
         done = ctypes.c_int()
         _check(lib.akx_download_query(h, 0, ctypes.byref(done)), -1)

@@ -48,10 +48,13 @@ def _oracle_specimen(en, k, dt, steps):
     return stable, cpu.nodal("u").copy()
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_ensemble_fp64_equals_each_specimen_alone(graph):
+@pytest.mark.parametrize("graph,layout", [(False, 0), (True, 0), (False, 6), (True, 6)])
+def test_ensemble_fp64_equals_each_specimen_alone(graph, layout):
+    """One specimen per material, one time step per node; layout 6 is the
+    fused node-block step."""
     en = _ensemble()
     model, dt_mat = _gpu_model(en)
+    model.set_launch(slot_layout=layout)
     steps = 200
     if graph:
         model.solve_step_graph(steps, akx.FP64)
@@ -78,8 +81,8 @@ def test_ensemble_mixed_error_per_specimen():
         assert err < 1e-5, (k, err)
 
 
-@pytest.mark.parametrize("precision,layout", [(akx.FP64, 0), (akx.MIXED, 0), (akx.MIXED, 4),
-                                              (akx.MIXED, 5)])
+@pytest.mark.parametrize("precision,layout", [(akx.FP64, 0), (akx.FP64, 6), (akx.MIXED, 0),
+                                              (akx.MIXED, 4), (akx.MIXED, 5)])
 def test_graph_replays_match_eager(precision, layout):
     """Replays of one captured graph, with host work and synchronisations in
     between, give what the eager launches give, every time; changing the time

@@ -74,7 +74,7 @@ def test_patch_linear_elastic_explicit_gpu(name, precision):
         assert abs(ekin) < 1e-12 * model.get_energy("potential")
 
 
-@pytest.mark.parametrize("layout,recompute", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("layout,recompute", [(0, 0), (1, 0), (0, 1), (1, 1), (6, 0), (6, 1)])
 def test_bar_c0_fp64_bitwise_1000_steps(layout, recompute):
     """c0 (4,000 triangle_3, 2,121 nodes), 1000 steps: FP64 GPU == oracle, with
     element-ordered (0) and node-ordered (1) force slots, B and det(J) w read
@@ -107,11 +107,16 @@ def test_bar_c0_mixed_1000_steps(recompute, layout):
 
 
 def test_cube_tet4_fp64_bitwise_and_mixed():
-    """reduced c1 (10^3 hexes -> 6,000 jittered tetrahedron_4), 200 steps."""
-    p = gh.cube_pair(10)
-    p.step(200, akx.FP64)
-    assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
-    assert _same(p.gpu.get_velocity(), p.cpu.nodal("v"))
+    """reduced c1 (10^3 hexes -> 6,000 jittered tetrahedron_4), 200 steps; FP64
+    with element slots and with the fused node-block step (layout 6)."""
+    for layout in (0, 6):
+        p = gh.cube_pair(10)
+        p.gpu.set_launch(slot_layout=layout)
+        p.step(200, akx.FP64)
+        assert _same(p.gpu.get_displacement(), p.cpu.nodal("u")), layout
+        assert _same(p.gpu.get_velocity(), p.cpu.nodal("v")), layout
+        assert _same(p.gpu.get_acceleration(), p.cpu.nodal("a")), layout
+        assert p.gpu.query("fused_available") == 1
     q = gh.cube_pair(10)
     q.step(200, akx.MIXED)
     err = gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u"))
@@ -132,14 +137,16 @@ def test_cube_tet4_mixed_1000_steps(recompute, layout):
 def test_hex8_neohookean_prescribed_velocity():
     """reduced c2 (6^3 hexahedron_8, 8 points each), neo-Hookean, top face
     pushed at 0.5 m/s: FP64 <= 1e-12, mixed <= 1e-5 after 300 steps."""
-    p = gh.hex_pair(6)
-    p.gpu.solve_step(300, akx.FP64)
-    gh.oracle_hex_step(p, 300)
-    err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
-    assert err <= 1e-12, err
-    assert gh.rel(p.gpu.get_velocity(), p.cpu.nodal("v")) <= 1e-12
-    epg, epc = p.gpu.get_energy("potential"), p.cpu.energy("potential")
-    assert abs(epg - epc) <= 1e-12 * abs(epc)
+    for layout in (1, 6):
+        p = gh.hex_pair(6)
+        p.gpu.set_launch(slot_layout=layout)
+        p.gpu.solve_step(300, akx.FP64)
+        gh.oracle_hex_step(p, 300)
+        err = gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u"))
+        assert err <= 1e-12, (layout, err)
+        assert gh.rel(p.gpu.get_velocity(), p.cpu.nodal("v")) <= 1e-12
+        epg, epc = p.gpu.get_energy("potential"), p.cpu.energy("potential")
+        assert abs(epg - epc) <= 1e-12 * abs(epc)
     for recompute, layout in ((0, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5)):
         q = gh.hex_pair(6)
         q.gpu.set_launch(mixed_recompute=recompute, slot_layout=layout)
@@ -281,6 +288,14 @@ def test_launch_shapes_do_not_change_results():
         p.gpu.set_launch(64, 64, 0, 0, 1, stage)
         p.step(50, akx.FP64)
         assert _same(p.gpu.get_displacement(), p.cpu.nodal("u"))
+    # the fused FP64 step: tables or rebuilt B, either block mapping, and an
+    # odd and an even number of steps (the displacement alternates buffers)
+    for recompute, x, steps in ((0, 0, 7), (1, 1, 8), (1, 0, 1)):
+        p = gh.cube_pair(6)
+        p.gpu.set_launch(0, 0, x, recompute, 6, 0)
+        p.step(steps, akx.FP64)
+        assert _same(p.gpu.get_displacement(), p.cpu.nodal("u")), (recompute, x, steps)
+        assert _same(p.gpu.get_internal_force(), p.cpu.nodal("f_int")), (recompute, x, steps)
 
 
 def test_autotune_restores_state_and_keeps_results():
