@@ -564,7 +564,9 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
 // and, per chunk of FUSED_CH elements, where each node's run starts.  `order`
 // is the assembly order (caller element ids), `off`/`slot` the FP64 CSR.
 // Leaves c->fused_ok false for laws with history (the step recomputes
-// elements) and for several laws in one mesh.
+// elements), for several laws in one mesh and for the 8-node hexahedron
+// (its FP64 element state alone takes most of the registers; fused, it
+// spills and runs one wave per SIMD).
 int build_fused(akx_ctx* c, const std::vector<uint32_t>& order, const std::vector<uint32_t>& off,
                 const std::vector<uint32_t>& slot) {
   c->fused_ok = false;
@@ -575,7 +577,7 @@ int build_fused(akx_ctx* c, const std::vector<uint32_t>& order, const std::vecto
   if (kind != AKX_MATERIAL_ELASTIC && kind != AKX_MATERIAL_NEOHOOKEAN) return AKX_OK;
   for (int64_t e = 1; e < E; ++e)
     if (c->mats[c->elem_mat[e]].kind != kind) return AKX_OK;
-  if (int64_t(FUSED_CH) * n > 0xFFFF) return AKX_OK;
+  if (n > 4 || int64_t(FUSED_CH) * n > 0xFFFF) return AKX_OK;
   std::vector<uint32_t> rank(E);  // assembly rank of each internal element
   for (int64_t r = 0; r < E; ++r) rank[c->elem_inv[order[r]]] = uint32_t(r);
   const int64_t nblk = (N + FUSED_NB - 1) / FUSED_NB;

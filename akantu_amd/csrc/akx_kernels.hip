@@ -1732,12 +1732,17 @@ hipError_t launch_fused(int et, int kind, bool recompute, bool has_fext, hipStre
   clear_error();
   if (F.nblk == 0) return hipSuccess;
   if (kind != ELASTIC && kind != NEOHOOKEAN) return hipErrorInvalidValue;
+  // elements of up to 4 nodes (the host builds no fused tables for others)
 #define F_(ET)                                                                       \
   return kind == ELASTIC ? launch_fused_et<ET, ELASTIC>(recompute, has_fext, s, EA, NA, F, lds) \
                          : launch_fused_et<ET, NEOHOOKEAN>(recompute, has_fext, s, EA, NA, F, lds);
-  AKX_ET_SWITCH(et, F_)
+  switch (et) {
+  case TRI3: F_(TRI3);
+  case QUAD4: F_(QUAD4);
+  case TET4: F_(TET4);
+  default: return hipErrorInvalidValue;
+  }
 #undef F_
-  return hipErrorInvalidValue;
 }
 
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,

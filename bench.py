@@ -105,6 +105,17 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     return elem, node
 
 
+def fused_bytes(E, N, n, d, evaluations, nblk, node_block=256, chunk=256):
+    """Compulsory HBM bytes of one fused FP64 step (slot layout 6): the blocks'
+    element lists (4 bytes per evaluation) and LDS slots (2 bytes per node of
+    an evaluation), the per-chunk run tables, connectivity and coordinates
+    once, u read and written (two buffers), v and a read and written, lumped
+    mass and blocked flags."""
+    chunks = evaluations / chunk + nblk
+    return (evaluations * (4 + 2 * n) + chunks * (node_block + 1) * 2 + (nblk + 1) * 8
+            + E * n * 4 + N * d * 8 + 2 * N * d * 8 + 4 * N * d * 8 + N * 8 + N * d)
+
+
 def make_workload(n):
     from akantu_amd import mesh as akm
     m = akm.cube_tet4(n, 0.01, 1)
@@ -350,6 +361,9 @@ def bench_cube(args, akx):
     eb64, nb64 = algorithmic_bytes(E, N, n, Q, d, 8,
                                    tables=not shapes["fp64"]["mixed_recompute"],
                                    node_ordered=bool(shapes["fp64"]["slot_layout"]))
+    if shapes["fp64"]["slot_layout"] == 6 and model.query("fused_available") == 1:
+        eb64 = fused_bytes(E, N, n, d, model.query("fused_elements"), (N + 255) // 256)
+        nb64 = 0
     copy_gbs = copy_bandwidth()
     log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
 
