@@ -281,14 +281,23 @@ void to_internal_points(const akx_ctx* c, const T* src, T* dst, int comp) {
     std::memcpy(dst + size_t(e) * per, src + size_t(c->elem_perm[e]) * per, per * sizeof(T));
 }
 
-// Mixed path with per-patch partial sums (slot layouts 3, 4 and 5; 6, the
-// fused FP64 step, runs as 5 on the mixed path): the element kernel reduces
+// Step fused over node blocks (slot layout 6): one launch per step, no force
+// slots; stateless laws of a single kind on elements of up to 4 nodes, FP64
+// or mixed.
+bool uses_fused(const akx_ctx* c, int /*precision*/) {
+  return c->node_ordered == 6 && c->fused_ok && c->groups.size() == 1 && !c->groups[0].filter &&
+         !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE);
+}
+
+// Mixed path with per-patch partial sums (slot layouts 3, 4 and 5; 6 runs as
+// 5 where the fused step does not apply): the element kernel reduces
 // each node's contributions inside its patch in FP64 and stores one FP64 (3)
 // or FP32 (4, 5) partial per (patch, node), in patch order (3, 4) or at the
 // node's place in a node-major list (5); the node kernel sums a node's
 // partials in FP64.  Needs the LDS patches and a single launch group.
 bool uses_partials(const akx_ctx* c, int precision) {
   return precision == AKX_MIXED && c->node_ordered >= 3 && c->node_ordered <= 6 &&
+         !uses_fused(c, precision) &&
          c->patch_ok && c->groups.size() == 1 &&
          !c->groups[0].filter &&
          !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
@@ -299,13 +308,6 @@ bool uses_partials(const akx_ctx* c, int precision) {
 // per-patch partials, FP32 otherwise
 int node_precision(const akx_ctx* c, int precision) {
   return uses_partials(c, precision) ? AKX_FP64 : precision;
-}
-
-// FP64 step fused over node blocks (slot layout 6): one launch per step, no
-// force slots; stateless laws of a single kind.
-bool uses_fused(const akx_ctx* c, int precision) {
-  return precision == AKX_FP64 && c->node_ordered == 6 && c->fused_ok && c->groups.size() == 1 &&
-         !c->groups[0].filter;
 }
 
 ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
@@ -1337,13 +1339,14 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
   }
   if (uses_fused(c, precision)) {
     // one fused launch per step; u alternates between c->u and c->u_alt
+    const int leap = precision == AKX_MIXED ? NODE_LEAP : 0;
     double* ucur = c->u;
     double* unext = c->u_alt;
     const KindGroup& g = c->groups[0];
     for (int s = 1; s <= nsteps; ++s) {
       const bool last = (s == nsteps);
       if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
-      int mode = NODE_CORRECT;
+      int mode = NODE_CORRECT | leap;
       if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
       else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
       NodeArgs NA = node_args(c, mode, precision);
@@ -1357,8 +1360,8 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       F.cbase = c->fcbase;
       F.u_out = unext;
       F.nblk = c->fused_blocks;
-      HIP_TRY(c, launch_fused(c->et, g.kind, c->mixed_recompute != 0, c->has_fext, c->stream, EA,
-                              NA, F, c->fused_lds));
+      HIP_TRY(c, launch_fused(c->et, g.kind, precision, c->mixed_recompute != 0, c->has_fext,
+                              c->stream, EA, NA, F, c->fused_lds));
       if (timing) {
         HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
         HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
@@ -1798,9 +1801,10 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
               cand.push_back(Shape{eb, nb, x, rcmp, lay, stage});
             }
           }
-  // the fused FP64 step (layout 6) has its own block shapes
-  if (precision == AKX_FP64 && c->fused_ok)
-    for (int rcmp = 0; rcmp < 2; ++rcmp)
+  // the fused step (layout 6) has its own block shapes; on the mixed path it
+  // always rebuilds B
+  if (c->fused_ok)
+    for (int rcmp = (precision == AKX_MIXED ? 1 : 0); rcmp < 2; ++rcmp)
       for (int x = 0; x < 2; ++x) cand.push_back(Shape{256, 256, x, rcmp, 6, 0});
   for (const Shape& sh : cand) {
           const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;

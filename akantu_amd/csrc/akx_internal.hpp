@@ -152,8 +152,9 @@ hipError_t launch_mass(int et, hipStream_t s, int64_t N, int64_t E, const uint32
                        const uint16_t* elem_mat, const MatParams* mats, double* mass);
 hipError_t launch_element(int et, int kind, int precision, hipStream_t s, const ElemArgs& A,
                           int block);
-hipError_t launch_fused(int et, int kind, bool recompute, bool has_fext, hipStream_t s,
-                        const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F, int lds);
+hipError_t launch_fused(int et, int kind, int precision, bool recompute, bool has_fext,
+                        hipStream_t s, const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F,
+                        int lds);
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
                        int block);
 hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,

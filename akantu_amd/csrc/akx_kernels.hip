@@ -1712,6 +1712,101 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
   return hipGetLastError();
 }
 
+// The same fused step on the mixed path: element forces in FP32 from node
+// differences formed in FP64 (mixed_element_forces, the patch kernel's
+// arithmetic), contributions in LDS in FP32, each node's run summed in FP64,
+// the node update with the half-step velocity form of the mixed path.  Its
+// sums follow the FP64 path's assembly order (any fixed order will do here).
+template <int ET, int KIND, bool HAS_FEXT>
+__global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs NA, FusedArgs F) {
+  static_assert(KIND != NH_DAMAGE, "the fused step recomputes elements: stateless laws only");
+  static_assert(FUSED_NB <= 256 && FUSED_CH == 256, "one node and one element per thread of 256");
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, dn = d * n;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* s_f = reinterpret_cast<float*>(smem);  // FUSED_CH * n contributions, d each
+  const int64_t b = logical_block(NA.xcd_remap);
+  if (b >= F.nblk) return;  // uniform over the block
+  const int t = threadIdx.x;
+  const int64_t n0 = b * FUSED_NB;
+  const int cnt = int(min<int64_t>(FUSED_NB, NA.N - n0));
+  const bool owner = t < cnt;
+  const int64_t node = n0 + t;
+  const int64_t E = EA.E;
+  const double* __restrict__ u_in = NA.u;
+  double f[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) f[i] = 0.;
+  const uint32_t j0 = F.el_off[b], j1 = F.el_off[b + 1];
+  const uint32_t c0 = F.ch_off[b];
+  uint32_t c = 0;
+  for (uint32_t jc = j0; jc < j1; jc += FUSED_CH, ++c) {
+    const uint32_t j = jc + uint32_t(t);
+    if (j < j1) {
+      const int64_t e = F.el[j];
+      const MatParams m = EA.mats[EA.elem_mat ? EA.elem_mat[e] : 0];
+      int64_t nd[n];
+#pragma unroll
+      for (int a = 0; a < n; ++a) nd[a] = EA.conn[a * E + e];
+      // differences to node 0 formed in FP64, then rounded
+      float du[dn], dX[dn];
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        const double u0 = u_in[nd[0] * d + i], x0 = EA.X[nd[0] * d + i];
+        du[i] = 0.f;
+        dX[i] = 0.f;
+#pragma unroll
+        for (int a = 1; a < n; ++a) {
+          du[i + a * d] = float(u_in[nd[a] * d + i] - u0);
+          dX[i + a * d] = float(EA.X[nd[a] * d + i] - x0);
+        }
+      }
+      float fe[dn];
+      mixed_element_forces<ET, KIND, true>(EA, e, m, du, dX, fe);
+#pragma unroll
+      for (int a = 0; a < n; ++a) {
+        const uint32_t ls = F.lslot[size_t(j) * n + a];
+        if (ls != 0xFFFFu)
+#pragma unroll
+          for (int i = 0; i < d; ++i) s_f[ls * d + i] = fe[i + a * d];
+      }
+    }
+    __syncthreads();
+    if (owner) {
+      const uint16_t* cb = F.cbase + size_t(c0 + c) * (FUSED_NB + 1);
+      const int q0 = cb[t], q1 = cb[t + 1];
+      for (int q = q0; q < q1; ++q)
+#pragma unroll
+        for (int i = 0; i < d; ++i) f[i] += -1. * double(s_f[q * d + i]);
+    }
+    __syncthreads();
+  }
+  if (!owner) return;
+  const int mode = NA.mode;
+  // between the steps of a run the half-step form neither reads nor writes a
+  const bool leap_mid = (mode & NODE_LEAP) && (mode & NODE_CORRECT) && (mode & NODE_PREDICT);
+  const double dt = NA.dt_node ? NA.dt_node[node] : NA.dt;
+  double u[d], v[d], acc[d], fx[d];
+  uint8_t bl[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    u[i] = u_in[node * d + i];
+    v[i] = NA.v[node * d + i];
+    acc[i] = leap_mid ? 0. : NA.a[node * d + i];
+    bl[i] = NA.blocked[node * d + i];
+    fx[i] = HAS_FEXT ? NA.fext[node * d + i] : 0.;
+  }
+  if (mode & (NODE_CORRECT | NODE_GATHER))
+    node_correct<d, HAS_FEXT>(NA, node, dt, f, fx, bl, u, v, acc);
+  if (mode & NODE_PREDICT) node_predict<d>(NA, node, dt, bl, u, v, acc);
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    F.u_out[node * d + i] = u[i];
+    NA.v[node * d + i] = v[i];
+    if (!leap_mid) NA.a[node * d + i] = acc[i];
+  }
+}
+
 template <int ET, int KIND>
 static hipError_t launch_fused_et(bool recompute, bool has_fext, hipStream_t s,
                                   const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F,
@@ -1727,11 +1822,34 @@ static hipError_t launch_fused_et(bool recompute, bool has_fext, hipStream_t s,
   return hipGetLastError();
 }
 
-hipError_t launch_fused(int et, int kind, bool recompute, bool has_fext, hipStream_t s,
-                        const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F, int lds) {
+template <int ET, int KIND>
+static hipError_t launch_fused_mixed_et(bool has_fext, hipStream_t s, const ElemArgs& EA,
+                                        const NodeArgs& NA, const FusedArgs& F, int lds) {
+  const unsigned g = unsigned(F.nblk);
+  if (has_fext) k_step_fused_mixed<ET, KIND, true><<<g, 256, lds, s>>>(EA, NA, F);
+  else k_step_fused_mixed<ET, KIND, false><<<g, 256, lds, s>>>(EA, NA, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(int et, int kind, int precision, bool recompute, bool has_fext,
+                        hipStream_t s, const ElemArgs& EA, const NodeArgs& NA, const FusedArgs& F,
+                        int lds) {
   clear_error();
   if (F.nblk == 0) return hipSuccess;
   if (kind != ELASTIC && kind != NEOHOOKEAN) return hipErrorInvalidValue;
+  if (precision == 1) {
+    // FP32 contributions: half the LDS of the FP64 step
+#define M_(ET)                                                                            \
+  return kind == ELASTIC ? launch_fused_mixed_et<ET, ELASTIC>(has_fext, s, EA, NA, F, lds / 2) \
+                         : launch_fused_mixed_et<ET, NEOHOOKEAN>(has_fext, s, EA, NA, F, lds / 2);
+    switch (et) {
+    case TRI3: M_(TRI3);
+    case QUAD4: M_(QUAD4);
+    case TET4: M_(TET4);
+    default: return hipErrorInvalidValue;
+    }
+#undef M_
+  }
   // elements of up to 4 nodes (the host builds no fused tables for others)
 #define F_(ET)                                                                       \
   return kind == ELASTIC ? launch_fused_et<ET, ELASTIC>(recompute, has_fext, s, EA, NA, F, lds) \

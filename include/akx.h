@@ -230,10 +230,10 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * patches (2, mixed path) or one partial sum per (patch, node), summed in
  * FP64 and stored in FP64 (3, mixed path) or FP32 (4, mixed path) in patch
  * order, or in FP32 at the node's place in a node-major list (5, mixed
- * path), or no slots at all (6, FP64 path: one fused launch per step over
- * node blocks that recompute the elements around their nodes; stateless
- * laws of one kind, elements of up to 4 nodes, else node-ordered slots; the
- * mixed path runs 6 as 5), and whether the node
+ * path), or no slots at all (6: one fused launch per step over node blocks
+ * that recompute the elements around their nodes, on either path; stateless
+ * laws of one kind on elements of up to 4 nodes -- elsewhere the FP64 path
+ * runs 6 as node-ordered slots and the mixed path as 5), and whether the node
  * kernel copies its block's contiguous slot range to LDS first (node_stage
  * 1/0) (-1 keeps).  On the FP64 path none of these changes a result bit; on
  * the mixed path layouts 0/1, 2, 3 and 4/5 differ in rounding only

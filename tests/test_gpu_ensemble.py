@@ -68,9 +68,11 @@ def test_ensemble_fp64_equals_each_specimen_alone(graph, layout):
         assert np.array_equal(u[n0:n1], ref), k
 
 
-def test_ensemble_mixed_error_per_specimen():
+@pytest.mark.parametrize("layout", [0, 6])
+def test_ensemble_mixed_error_per_specimen(layout):
     en = _ensemble(count=5, seed=11)
     model, dt_mat = _gpu_model(en)
+    model.set_launch(slot_layout=layout)
     steps = 1000
     model.solve_step_graph(steps, akx.MIXED)
     u = model.get_displacement()
@@ -82,7 +84,7 @@ def test_ensemble_mixed_error_per_specimen():
 
 
 @pytest.mark.parametrize("precision,layout", [(akx.FP64, 0), (akx.FP64, 6), (akx.MIXED, 0),
-                                              (akx.MIXED, 4), (akx.MIXED, 5)])
+                                              (akx.MIXED, 4), (akx.MIXED, 5), (akx.MIXED, 6)])
 def test_graph_replays_match_eager(precision, layout):
     """Replays of one captured graph, with host work and synchronisations in
     between, give what the eager launches give, every time; changing the time

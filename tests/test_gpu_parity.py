@@ -90,7 +90,8 @@ def test_bar_c0_fp64_bitwise_1000_steps(layout, recompute):
         assert abs(p.gpu.get_energy(kind) - p.cpu.energy(kind)) <= 1e-12 * abs(p.cpu.energy(kind))
 
 
-@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 0), (1, 2), (1, 3), (1, 4), (1, 5)])
+@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 0), (1, 2), (1, 3), (1, 4), (1, 5),
+                                              (1, 6)])
 def test_bar_c0_mixed_1000_steps(recompute, layout):
     """Mixed path on c0 after 1000 steps (B read from FP32 tables or rebuilt
     from the coordinates; gather, LDS-patch and per-patch partial-sum kernels
@@ -123,7 +124,8 @@ def test_cube_tet4_fp64_bitwise_and_mixed():
     assert err <= 1e-5, err
 
 
-@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5)])
+@pytest.mark.parametrize("recompute,layout", [(0, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5),
+                                              (1, 6)])
 def test_cube_tet4_mixed_1000_steps(recompute, layout):
     """The 1000-step bar of the mixed path on tetrahedron_4 (8^3 cube), with
     the gather, node-ordered, LDS-patch and per-patch partial-sum kernels."""
@@ -272,7 +274,8 @@ def test_launch_shapes_do_not_change_results():
             (1, ((256, 128, 1, 2, 1), (256, 64, 0, 2, 0), (256, 256, 1, 2, 1))),
             (1, ((256, 256, 1, 3, 0), (256, 64, 0, 3, 0), (256, 128, 1, 3, 1))),
             (1, ((256, 256, 1, 4, 0), (256, 64, 0, 4, 0), (256, 128, 1, 4, 1))),
-            (1, ((256, 256, 1, 5, 0), (256, 64, 0, 5, 1), (256, 128, 1, 5, 0)))):
+            (1, ((256, 256, 1, 5, 0), (256, 64, 0, 5, 1), (256, 128, 1, 5, 0))),
+            (1, ((256, 256, 1, 6, 0), (64, 64, 0, 6, 0), (128, 256, 1, 6, 1)))):
         ref = None
         for eb, nb, x, lay, stage in shapes:
             p = gh.cube_pair(6)
