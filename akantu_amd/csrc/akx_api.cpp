@@ -470,8 +470,15 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
         cnt[k]++;
         lconn[size_t(a) * E + e] = uint16_t(k);
       }
+    // patch-local numbering by decreasing run length (then node id): the
+    // threads summing the runs of one wavefront loop about equally long
+    std::vector<int> ord(m), newk(m);
+    for (int k = 0; k < m; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+    for (int j = 0; j < m; ++j) newk[ord[j]] = j;
     uint32_t acc = 0;
-    for (int k = 0; k < m; ++k) {
+    for (int j = 0; j < m; ++j) {
+      const int k = ord[j];
       if (hi[k] - lo[k] + 1 != cnt[k] || cnt[k] > 255) return AKX_OK;  // run not contiguous
       pids.push_back(nodes[k]);
       rstart.push_back(lo[k]);
@@ -483,6 +490,7 @@ int build_patches(akx_ctx* c, const std::vector<uint32_t>& slot2) {
       for (int a = 0; a < n; ++a) {
         const int k = lconn[size_t(a) * E + e];
         lrank[size_t(a) * E + e] = uint8_t(pos[size_t(a) * E + e] - lo[k]);
+        lconn[size_t(a) * E + e] = uint16_t(newk[k]);
       }
     poff[b + 1] = poff[b] + uint32_t(m);
   }

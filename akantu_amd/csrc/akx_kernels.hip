@@ -877,11 +877,24 @@ k_element_patch(ElemArgs A) {
       double f[d];
 #pragma unroll
       for (int i = 0; i < d; ++i) f[i] = 0.;
-      // chunks of RUNCHUNK entries: a chunk's LDS reads are issued together
-      // (the force area has RUNCHUNK - 1 entries of padding after its end);
-      // entries past the run add +0, so the sum is the sequential one
+      // chunks of RUNCHUNK entries: a chunk's LDS reads are issued together.
+      // Full chunks add unmasked; the last, partial chunk reads into the
+      // force area's RUNCHUNK - 1 entries of padding and its entries past
+      // the run add +0, so the sum is the sequential one.  Patch-local nodes
+      // are numbered by decreasing run length, so the lanes of a wavefront
+      // run about the same number of chunks.
       const int q1 = s_rb[k + 1];
-      for (int q = s_rb[k]; q < q1; q += RUNCHUNK) {
+      int q = s_rb[k];
+      for (; q + RUNCHUNK <= q1; q += RUNCHUNK) {
+        float v[RUNCHUNK * d];
+#pragma unroll
+        for (int j = 0; j < RUNCHUNK * d; ++j) v[j] = s_f[q * d + j];
+#pragma unroll
+        for (int j = 0; j < RUNCHUNK; ++j)
+#pragma unroll
+          for (int i = 0; i < d; ++i) f[i] += double(v[j * d + i]);
+      }
+      if (q < q1) {
         float v[RUNCHUNK * d];
 #pragma unroll
         for (int j = 0; j < RUNCHUNK * d; ++j) v[j] = s_f[q * d + j];
