@@ -289,6 +289,19 @@ __device__ __forceinline__ float x_minus_log1p(float x) {
   return x - log1pf(x);
 }
 
+// log(1 + x): for |x| < 1/16 as 2 atanh(s), s = x / (2 + x), |s| < 1/31, by
+// its series to s^7 (first dropped term below 1e-11 relative; s carries the
+// reciprocal's ~1 ulp); else log1pf.
+__device__ __forceinline__ float log1p_small_f32(float x) {
+#pragma clang fp contract(fast)
+  if (fabsf(x) < 0.0625f) {
+    const float s = x * __builtin_amdgcn_rcpf(2.f + x), s2 = s * s;
+    const float p = fmaf(fmaf(fmaf(s2, 1.f / 7.f, 1.f / 5.f), s2, 1.f / 3.f), s2, 1.f);
+    return 2.f * s * p;
+  }
+  return log1pf(x);
+}
+
 // neo-Hookean second Piola-Kirchhoff stress from G in small-quantity form:
 //   E = (G + G^T + G^T G)/2,  C = I + 2E,  x = J - 1 from the invariants of G,
 //   S = C^{-1} (2 mu E + lambda log(1 + x) I)
@@ -347,6 +360,62 @@ __device__ __forceinline__ float neohookean_stress_f32(float lambda, float mu, c
     } else {
       rest = 0.5f * g2 - det2(G);
     }
+    *W = 0.5f * lambda * lj * lj + mu * (rest + x_minus_log1p(x));
+  }
+  return x;
+}
+
+// The same law in 3D on symmetric storage, for the kernels where the stress
+// evaluation dominates: E, C^{-1} and S kept as their 6 distinct entries
+// (xx, yy, zz, yz, xz, xy), C^{-1} through the adjugate and the hardware
+// reciprocal.  S (C^{-1} and T = 2 mu E + lambda ln J I commute, so S is
+// symmetric) is returned multiplied by `scale`.  Returns x = J - 1; W as
+// neohookean_stress_f32.
+__device__ __forceinline__ float neohookean_stress_sym3_f32(float lambda, float mu, const float* G,
+                                                             float scale, float* S6, float* W) {
+#pragma clang fp contract(fast)
+  auto gtg = [&](int i, int j) {
+    return G[3 * i] * G[3 * j] + G[3 * i + 1] * G[3 * j + 1] + G[3 * i + 2] * G[3 * j + 2];
+  };
+  float E[6];
+  E[0] = G[0] + 0.5f * gtg(0, 0);
+  E[1] = G[4] + 0.5f * gtg(1, 1);
+  E[2] = G[8] + 0.5f * gtg(2, 2);
+  E[3] = 0.5f * (G[7] + G[5] + gtg(1, 2));
+  E[4] = 0.5f * (G[6] + G[2] + gtg(0, 2));
+  E[5] = 0.5f * (G[3] + G[1] + gtg(0, 1));
+  const float c00 = 1.f + 2.f * E[0], c11 = 1.f + 2.f * E[1], c22 = 1.f + 2.f * E[2];
+  const float c12 = 2.f * E[3], c02 = 2.f * E[4], c01 = 2.f * E[5];
+  float Ci[6];
+  Ci[0] = c11 * c22 - c12 * c12;
+  Ci[1] = c00 * c22 - c02 * c02;
+  Ci[2] = c00 * c11 - c01 * c01;
+  Ci[3] = c01 * c02 - c00 * c12;
+  Ci[4] = c01 * c12 - c11 * c02;
+  Ci[5] = c02 * c12 - c22 * c01;
+  const float detc = c00 * Ci[0] + c01 * Ci[5] + c02 * Ci[4];
+  const float rc = __builtin_amdgcn_rcpf(detc);
+  const float tr = G[0] + G[4] + G[8];
+  const float i2 = G[0] * G[4] + G[4] * G[8] + G[0] * G[8] - G[3] * G[1] - G[7] * G[5] - G[6] * G[2];
+  const float dg = det3(G);
+  const float x = tr + i2 + dg;
+  const float lj = log1p_small_f32(x);
+  const float tm = 2.f * mu, tl = lambda * lj;
+  const float T0 = tm * E[0] + tl, T1 = tm * E[1] + tl, T2 = tm * E[2] + tl;
+  const float T3 = tm * E[3], T4 = tm * E[4], T5 = tm * E[5];
+  // S = Ci T, rows of the symmetric matrices as (xx xy xz / xy yy yz / xz yz zz)
+  const float sc = scale * rc;
+  S6[0] = sc * (Ci[0] * T0 + Ci[5] * T5 + Ci[4] * T4);
+  S6[1] = sc * (Ci[5] * T5 + Ci[1] * T1 + Ci[3] * T3);
+  S6[2] = sc * (Ci[4] * T4 + Ci[3] * T3 + Ci[2] * T2);
+  S6[3] = sc * (Ci[5] * T4 + Ci[1] * T3 + Ci[3] * T2);
+  S6[4] = sc * (Ci[0] * T4 + Ci[5] * T3 + Ci[4] * T2);
+  S6[5] = sc * (Ci[0] * T5 + Ci[5] * T1 + Ci[4] * T3);
+  if (W) {
+    float g2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g2 += G[k] * G[k];
+    const float rest = 0.5f * g2 - i2 - dg;
     *W = 0.5f * lambda * lj * lj + mu * (rest + x_minus_log1p(x));
   }
   return x;
