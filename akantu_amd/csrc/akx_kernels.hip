@@ -599,6 +599,29 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
           if (i == j) s += m.lambda_f * tr;
           P[i + j * d] = s * jq;
         }
+    } else if constexpr (d == 3) {
+      float S6[6], W;
+      float scale = jq;
+      neohookean_stress_sym3_f32(m.lambda_f, m.mu_f, G, 1.f, S6, KIND == NH_DAMAGE ? &W : nullptr);
+      if constexpr (KIND == NH_DAMAGE) {
+        const int64_t eq = e * Q + q;
+        double dam = A.damage[eq];
+        const double Fd = double(W) - A.Yd[eq] - m.Sd * dam;
+        if (Fd > 0) dam = (double(W) - A.Yd[eq]) / m.Sd;
+        dam = fmin(dam, 1.);
+        A.damage[eq] = dam;
+        scale *= float(1. - dam);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) S6[k] *= scale;
+      // P = F S = S + G S, S from its 6 entries
+      const float S[9] = {S6[0], S6[5], S6[4], S6[5], S6[1], S6[3], S6[4], S6[3], S6[2]};
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          P[i + 3 * j] = S[i + 3 * j] + G[i] * S[3 * j] + G[i + 3] * S[3 * j + 1] +
+                         G[i + 6] * S[3 * j + 2];
     } else {
       float S[d * d];
       float W;
