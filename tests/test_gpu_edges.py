@@ -70,6 +70,47 @@ def test_single_element_mixed_every_layout(name, kind):
             assert err <= 1e-5, (layout, recompute, err)
 
 
+FINITE_STRAIN_MESHES = {
+    "triangle_3": lambda: akm.rectangle_tri3(4, 4, 1.0, 1.0),
+    "quadrangle_4": lambda: akm.rectangle_quad4(4, 4, 1.0, 1.0),
+    "tetrahedron_4": lambda: akm.jitter_interior(akm.box_tet4(3, 3, 3), 0.05, 3),
+    "hexahedron_8": lambda: akm.jitter_interior(akm.box_hex8(3, 3, 3), 0.05, 3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FINITE_STRAIN_MESHES))
+@pytest.mark.parametrize("stretch", [0.02, 0.15])
+def test_mixed_neohookean_forces_at_finite_strain(name, stretch):
+    """Mixed-path neo-Hookean internal forces under a rotated finite stretch
+    (30 degrees, principal stretches up to 1 + `stretch`, plus a random
+    non-uniform part) against the oracle, <= 1e-5, for the table and rebuilt-
+    geometry kernels and the patch layouts.  At stretch 0.15, |J - 1| > 1/16
+    takes the FP32 law's log1pf branch, at 0.02 its series."""
+    mesh = FINITE_STRAIN_MESHES[name]()
+    X = mesh.nodes
+    d = X.shape[1]
+    c, s = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    if d == 2:
+        R = np.array([[c, -s], [s, c]])
+        U = np.diag([1 + stretch, 1 - stretch / 2])
+    else:
+        k = np.ones(3) / np.sqrt(3.0)
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        R = np.eye(3) + s * K + (1 - c) * K @ K
+        U = np.diag([1 + stretch, 1 - stretch / 2, 1 + stretch / 3])
+    rng = np.random.default_rng(11)
+    u = X @ (R @ U - np.eye(d)).T + rng.normal(scale=0.02 * stretch, size=X.shape)
+    p = gh.Pair(mesh, [dict(kind=akx.NEOHOOKEAN, E=1e7, nu=0.3, rho=1000.0)])
+    p.set_nodal("displacement", u)
+    p.cpu.assemble_internal_forces()
+    ref = p.cpu.nodal("f_int").copy()
+    for layout, recompute in ((0, 0), (0, 1), (1, 1), (3, 1), (4, 1), (5, 1)):
+        p.gpu.set_launch(256, 256, 1, recompute, layout, 0)
+        p.gpu.assemble_internal_forces(akx.MIXED)
+        err = gh.rel(p.gpu.get_internal_force(), ref)
+        assert err <= 1e-5, (layout, recompute, err)
+
+
 def _check(rc, expected):
     assert rc == expected, (rc, expected)
 
