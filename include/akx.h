@@ -237,7 +237,9 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * kernel copies its block's contiguous slot range to LDS first (node_stage
  * 1/0) (-1 keeps).  On the FP64 path none of these changes a result bit; on
  * the mixed path layouts 0/1, 2, 3 and 4/5 differ in rounding only
- * (patch-relative FP32 node values, partial sums). */
+ * (patch-relative FP32 node values, partial sums), and so do the tables and
+ * the rebuilt geometry (for hexahedron_8 rebuilt in the Walsh basis of the
+ * trilinear shape functions, without B). */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 
