@@ -1285,9 +1285,22 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
 #pragma unroll
             for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
         }
-        for (; p < p1; ++p)
+        // tail in groups of 4, all loads of a group issued before the adds
+        // (entries past the run reload the group's first entry, unused):
+        // a node's few per-patch partials cost one round trip, not one each
+        for (; p < p1; p += 4) {
+          const int cnt4 = int(min<uint32_t>(4u, p1 - p));
+          TS x[4 * D];
 #pragma unroll
-          for (int i = 0; i < D; ++i) add(i, slots[int64_t(p) * D + i]);
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < D; ++i) x[j * D + i] = slots[int64_t(j < cnt4 ? p + j : p) * D + i];
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (j < cnt4)
+#pragma unroll
+              for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
+        }
       }
       for (; p + CHUNK <= p1; p += CHUNK) {
         uint32_t s[CHUNK];
