@@ -1222,6 +1222,29 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   const bool leap = mode & NODE_LEAP;
   // the half-step form between two steps touches neither a nor its loads
   const bool leap_mid = leap && correct && predict;
+  const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
+  const bool gather = mode & (NODE_CORRECT | NODE_GATHER);
+  const bool staged = A.node_ordered && A.stage_lds > 0 && gather;
+  const int ln = threadIdx.x;
+  const int64_t node = n0 + ln;
+  // Node-ordered contributions read from global memory: the node's range and
+  // its first group of 4 contributions are loaded before the block's state is
+  // staged, so that this dependent chain overlaps the staging loads.
+  const bool early = gather && A.node_ordered && !staged && ln < cnt;
+  uint32_t p0 = 0, p1 = 0;
+  int c0 = 0;
+  TS x0[4 * D];
+  if (early) {
+    p0 = A.csr_off[node];
+    p1 = A.csr_off[node + 1];
+    c0 = int(min<uint32_t>(4u, p1 - p0));
+    if (c0 > 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) x0[j * D + i] = slots[int64_t(j < c0 ? p0 + j : p0) * D + i];
+    }
+  }
   for (int t = threadIdx.x; t < nk; t += nb) {
     s_u[t] = A.u[k0 + t];
     s_v[t] = A.v[k0 + t];
@@ -1234,8 +1257,6 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   // loads, then every thread reads its own run from there.
   extern __shared__ __align__(16) unsigned char dyn[];
   TS* st = reinterpret_cast<TS*>(dyn);
-  const bool staged = A.node_ordered && A.stage_lds > 0 && (mode & (NODE_CORRECT | NODE_GATHER));
-  const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
   uint32_t base = 0;
   if (staged) {
     const uint32_t q0 = A.csr_off[n0], q1 = A.csr_off[n0 + cnt];
@@ -1244,16 +1265,17 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
     base = q0;
   }
   __syncthreads();
-  const int ln = threadIdx.x;
-  const int64_t node = n0 + ln;
   if (ln < cnt) {
     // one time step per node when independent bodies advance together
     const double dt = A.dt_node ? A.dt_node[node] : A.dt;
-    if (mode & (NODE_CORRECT | NODE_GATHER)) {
+    if (gather) {
       double f[D], c[D];
 #pragma unroll
       for (int i = 0; i < D; ++i) { f[i] = 0.; c[i] = 0.; }
-      const uint32_t p0 = A.csr_off[node], p1 = A.csr_off[node + 1];
+      if (!early) {
+        p0 = A.csr_off[node];
+        p1 = A.csr_off[node + 1];
+      }
       auto add = [&](int i, TS raw) {
         const double x = -1. * double(raw);
         if constexpr (sizeof(TS) == 8) {
@@ -1275,15 +1297,26 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
           for (int i = 0; i < D; ++i) add(i, st[q * D + i]);
         p = p1;
       } else if (A.node_ordered) {
-        // the node's contributions are contiguous: slots[p*D ...] for p in [p0, p1)
-        for (; p + CHUNK <= p1; p += CHUNK) {
-          TS x[CHUNK * D];
+        // the node's contributions are contiguous: slots[p*D ...] for p in
+        // [p0, p1), the first (up to) 4 already loaded, added first
 #pragma unroll
-          for (int j = 0; j < CHUNK * D; ++j) x[j] = slots[int64_t(p) * D + j];
+        for (int j = 0; j < 4; ++j)
+          if (j < c0)
 #pragma unroll
-          for (int j = 0; j < CHUNK; ++j)
+            for (int i = 0; i < D; ++i) add(i, x0[j * D + i]);
+        p += uint32_t(c0);
+        // long runs of element slots in batches of CHUNK; the per-patch
+        // partials (SRC 2, 3: a few per node) skip this loop and its registers
+        if constexpr (SRC < 2) {
+          for (; p + CHUNK <= p1; p += CHUNK) {
+            TS x[CHUNK * D];
 #pragma unroll
-            for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
+            for (int j = 0; j < CHUNK * D; ++j) x[j] = slots[int64_t(p) * D + j];
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j)
+#pragma unroll
+              for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
+          }
         }
         // tail in groups of 4, all loads of a group issued before the adds
         // (entries past the run reload the group's first entry, unused):
