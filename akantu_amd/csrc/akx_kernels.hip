@@ -1227,10 +1227,12 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   const bool staged = A.node_ordered && A.stage_lds > 0 && gather;
   const int ln = threadIdx.x;
   const int64_t node = n0 + ln;
-  // Node-ordered contributions read from global memory: the node's range and
-  // its first group of 4 contributions are loaded before the block's state is
-  // staged, so that this dependent chain overlaps the staging loads.
-  const bool early = gather && A.node_ordered && !staged && ln < cnt;
+  // Per-patch partial sums read from global memory (a few per node): the
+  // node's range and its first group of 4 partials (through the entry list
+  // when they are not node-ordered) are loaded before the block's state is
+  // staged, so that this dependent chain overlaps the staging loads.  Element
+  // slots (SRC 0, 1: long runs) are read after the staging in batches.
+  const bool early = SRC >= 2 && gather && !staged && ln < cnt;
   uint32_t p0 = 0, p1 = 0;
   int c0 = 0;
   TS x0[4 * D];
@@ -1239,10 +1241,16 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
     p1 = A.csr_off[node + 1];
     c0 = int(min<uint32_t>(4u, p1 - p0));
     if (c0 > 0) {
+      uint32_t sj[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t q = j < c0 ? p0 + j : p0;
+        sj[j] = A.node_ordered ? q : A.csr_slot[q];
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < D; ++i) x0[j * D + i] = slots[int64_t(j < c0 ? p0 + j : p0) * D + i];
+        for (int i = 0; i < D; ++i) x0[j * D + i] = slots[int64_t(sj[j]) * D + i];
     }
   }
   for (int t = threadIdx.x; t < nk; t += nb) {
@@ -1296,15 +1304,17 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
 #pragma unroll
           for (int i = 0; i < D; ++i) add(i, st[q * D + i]);
         p = p1;
-      } else if (A.node_ordered) {
-        // the node's contributions are contiguous: slots[p*D ...] for p in
-        // [p0, p1), the first (up to) 4 already loaded, added first
+      } else {
+        // the first (up to) 4 contributions, already loaded, are added first
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (j < c0)
 #pragma unroll
             for (int i = 0; i < D; ++i) add(i, x0[j * D + i]);
         p += uint32_t(c0);
+      }
+      if (!staged && A.node_ordered) {
+        // the node's contributions are contiguous: slots[p*D ...] for p in [p0, p1)
         // long runs of element slots in batches of CHUNK; the per-patch
         // partials (SRC 2, 3: a few per node) skip this loop and its registers
         if constexpr (SRC < 2) {
@@ -1335,19 +1345,21 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
               for (int i = 0; i < D; ++i) add(i, x[j * D + i]);
         }
       }
-      for (; p + CHUNK <= p1; p += CHUNK) {
-        uint32_t s[CHUNK];
+      if constexpr (SRC < 2) {
+        for (; p + CHUNK <= p1; p += CHUNK) {
+          uint32_t s[CHUNK];
 #pragma unroll
-        for (int j = 0; j < CHUNK; ++j) s[j] = A.csr_slot[p + j];
-        TS x[CHUNK][D];
+          for (int j = 0; j < CHUNK; ++j) s[j] = A.csr_slot[p + j];
+          TS x[CHUNK][D];
 #pragma unroll
-        for (int j = 0; j < CHUNK; ++j)
+          for (int j = 0; j < CHUNK; ++j)
 #pragma unroll
-          for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
+            for (int i = 0; i < D; ++i) x[j][i] = slots[int64_t(s[j]) * D + i];
 #pragma unroll
-        for (int j = 0; j < CHUNK; ++j)
+          for (int j = 0; j < CHUNK; ++j)
 #pragma unroll
-          for (int i = 0; i < D; ++i) add(i, x[j][i]);
+            for (int i = 0; i < D; ++i) add(i, x[j][i]);
+        }
       }
       // tail in groups of 4 with all loads of a group issued first (the short
       // per-patch partial lists end up here entirely)
