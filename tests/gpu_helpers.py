@@ -17,9 +17,9 @@ def rel(a, b):
 class Pair:
     """A GPU model and the oracle model, kept in lock-step."""
 
-    def __init__(self, mesh, materials, elem_mat=None):
+    def __init__(self, mesh, materials, elem_mat=None, keep_numbering=False):
         self.mesh = mesh
-        self.gpu = akx.SolidMechanicsModel(mesh)
+        self.gpu = akx.SolidMechanicsModel(mesh, keep_numbering=keep_numbering)
         for m in materials:
             self.gpu.add_material(kind=m.get("kind", akx.ELASTIC), E=m["E"], nu=m["nu"],
                                   rho=m["rho"], plane_stress=m.get("plane_stress", 0),

@@ -111,6 +111,37 @@ def test_mixed_neohookean_forces_at_finite_strain(name, stretch):
         assert err <= 1e-5, (layout, recompute, err)
 
 
+@pytest.mark.parametrize("keep_numbering", [True, False])
+def test_node_without_elements(keep_numbering):
+    """A node that no element uses (here the last node, and kept last when the
+    library keeps the caller's numbering) has an empty force range: FP64 ==
+    oracle bit for bit and mixed <= 1e-5 for every layout, the blocked orphan
+    stays at rest."""
+    base = akm.cube_tet4(3, 0.02, 5)
+    mesh = akm.Mesh(base.elem_type, np.vstack([base.nodes, [[2.0, 2.0, 2.0]]]), base.conn)
+    X = mesh.nodes
+    mask = np.zeros_like(X, dtype=bool)
+    mask[np.isclose(X[:, 2], 0.0)] = True
+    mask[-1] = True
+    v = np.zeros_like(X)
+    v[:, 0] = np.sin(3 * X[:, 1]) * X[:, 2]
+    v[-1] = 0.0
+    for precision in (akx.FP64, akx.MIXED):
+        for layout in range(7):
+            p = gh.Pair(mesh, [dict(E=210e9, nu=0.3, rho=7800.0)], keep_numbering=keep_numbering)
+            p.block(mask)
+            p.set_nodal("velocity", v)
+            p.set_time_step(0.8 * p.cpu.stable_time_step())
+            p.gpu.set_launch(256, 256, 1, 1, layout, 0)
+            p.step(30, precision)
+            u = p.gpu.get_displacement()
+            assert np.all(u[-1] == 0.0), (precision, layout)
+            if precision == akx.FP64:
+                assert np.array_equal(u, p.cpu.nodal("u")), layout
+            else:
+                assert gh.rel(u, p.cpu.nodal("u")) <= 1e-5, layout
+
+
 def _check(rc, expected):
     assert rc == expected, (rc, expected)
 
