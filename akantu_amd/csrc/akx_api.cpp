@@ -74,7 +74,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
   dfree(c->fel_off); dfree(c->fel); dfree(c->flslot); dfree(c->fch_off); dfree(c->fcbase);
-  dfree(c->u_alt);
+  dfree(c->u_alt); dfree(c->fgeo);
   c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
@@ -648,6 +648,13 @@ int build_fused(akx_ctx* c, const std::vector<uint32_t>& order, const std::vecto
   HIP_TRY(c, hipMemcpy(c->flslot, lslot.data(), lslot.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->fch_off, ch_off.data(), ch_off.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->fcbase, cbase.data(), cbase.size() * 2, hipMemcpyHostToDevice));
+  // on triangles the mixed fused step reads each element's coordinate
+  // differences from a record instead of gathering X for its nodes (X is
+  // fixed after akx_init)
+  if (c->et == AKX_TRIANGLE_3) {
+    HIP_TRY(c, dalloc(&c->fgeo, size_t(E) * fused_geo_stride(n, d)));
+    HIP_TRY(c, launch_fused_geometry(c->et, c->stream, E, c->X, c->conn_soa, c->fgeo));
+  }
   c->fused_blocks = nblk;
   c->fused_elements = int64_t(el.size());
   c->fused_lds = FUSED_CH * n * d * 8;
@@ -1367,6 +1374,7 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       F.ch_off = c->fch_off;
       F.cbase = c->fcbase;
       F.u_out = unext;
+      F.geo = c->fgeo;
       F.nblk = c->fused_blocks;
       HIP_TRY(c, launch_fused(c->et, g.kind, precision, c->mixed_recompute != 0, c->has_fext,
                               c->stream, EA, NA, F, c->fused_lds));

@@ -109,8 +109,13 @@ struct FusedArgs {
   const uint32_t* ch_off;  // [nblk + 1] first chunk table of each block
   const uint16_t* cbase;   // [chunk * (FUSED_NB + 1) + k] first slot of block node k in the chunk
   double* u_out;           // displacement written by the step (read from NodeArgs::u)
+  const float* geo;        // mixed path on triangles: per element x_a - x_0 (a > 0), FP32
   int64_t nblk;
 };
+
+// Floats per element of the fused mixed step's geometry record: (n - 1) d
+// coordinate differences, padded to whole 16-byte vectors.
+constexpr int fused_geo_stride(int n, int d) { return ((n - 1) * d + 3) & ~3; }
 
 enum NodeMode {
   NODE_CORRECT = 1,     // gather forces, residual, lumped solve, corrector
@@ -157,6 +162,8 @@ hipError_t launch_fused(int et, int kind, int precision, bool recompute, bool ha
                         int lds);
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
                        int block);
+hipError_t launch_fused_geometry(int et, hipStream_t s, int64_t E, const double* X,
+                                 const uint32_t* conn_soa, float* geo);
 hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
                             const uint32_t* conn_soa, const uint16_t* elem_mat,
                             const MatParams* mats, double* partial, int nblocks_max,
@@ -310,6 +317,7 @@ struct akx_ctx {
   uint32_t *fel_off = nullptr, *fel = nullptr, *fch_off = nullptr;
   uint16_t *flslot = nullptr, *fcbase = nullptr;
   double* u_alt = nullptr;  // second displacement buffer of the fused step
+  float* fgeo = nullptr;     // geometry records of the fused mixed step (triangles)
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal

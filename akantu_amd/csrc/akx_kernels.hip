@@ -1993,17 +1993,31 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
       int64_t nd[n];
 #pragma unroll
       for (int a = 0; a < n; ++a) nd[a] = EA.conn[a * E + e];
-      // differences to node 0 formed in FP64, then rounded
+      // differences to node 0 formed in FP64, then rounded.  For triangles
+      // the coordinate differences come from the element's record (the same
+      // values, formed by k_fused_geometry): one 16-byte load instead of
+      // three coordinate gathers, each element being evaluated by about 1.6
+      // node blocks.  Larger elements share their nodes among more elements
+      // and gather X (cached) faster than they would read a record.
+      constexpr bool RECORD = ET == TRI3;
+      constexpr int G = fused_geo_stride(n, d);
+      float gx[RECORD ? G : 1];
+      if constexpr (RECORD) {
+        const float4 v = reinterpret_cast<const float4*>(F.geo)[e];
+        gx[0] = v.x; gx[1] = v.y; gx[2] = v.z; gx[3] = v.w;
+      }
       float du[dn], dX[dn];
 #pragma unroll
       for (int i = 0; i < d; ++i) {
-        const double u0 = u_in[nd[0] * d + i], x0 = EA.X[nd[0] * d + i];
+        const double u0 = u_in[nd[0] * d + i];
+        const double x0 = RECORD ? 0. : EA.X[nd[0] * d + i];
         du[i] = 0.f;
         dX[i] = 0.f;
 #pragma unroll
         for (int a = 1; a < n; ++a) {
           du[i + a * d] = float(u_in[nd[a] * d + i] - u0);
-          dX[i + a * d] = float(EA.X[nd[a] * d + i] - x0);
+          if constexpr (RECORD) dX[i + a * d] = gx[(a - 1) * d + i];
+          else dX[i + a * d] = float(EA.X[nd[a] * d + i] - x0);
         }
       }
       float fe[dn];
@@ -2050,6 +2064,41 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
     NA.v[node * d + i] = v[i];
     if (!leap_mid) NA.a[node * d + i] = acc[i];
   }
+}
+
+// Geometry records of the mixed fused step on triangles: per element,
+// x_a - x_0 for a > 0 formed in FP64 and rounded to FP32 (the arithmetic the
+// element kernels use on the fly), zero-padded to fused_geo_stride floats.
+template <int ET>
+__global__ void __launch_bounds__(256) k_fused_geometry(int64_t E, const double* __restrict__ X,
+                                                        const uint32_t* __restrict__ conn,
+                                                        float* __restrict__ geo) {
+  using T = Elem<ET>;
+  constexpr int d = T::d, n = T::n, G = fused_geo_stride(n, d);
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int64_t n0 = conn[e];
+  float r[G];
+#pragma unroll
+  for (int k = 0; k < G; ++k) r[k] = 0.f;
+#pragma unroll
+  for (int a = 1; a < n; ++a) {
+    const int64_t na = conn[a * E + e];
+#pragma unroll
+    for (int i = 0; i < d; ++i) r[(a - 1) * d + i] = float(X[na * d + i] - X[n0 * d + i]);
+  }
+#pragma unroll
+  for (int k = 0; k < G; ++k) geo[e * G + k] = r[k];
+}
+
+hipError_t launch_fused_geometry(int et, hipStream_t s, int64_t E, const double* X,
+                                 const uint32_t* conn_soa, float* geo) {
+  clear_error();
+  if (E == 0) return hipSuccess;
+  const unsigned g = unsigned((E + 255) / 256);
+  if (et != TRI3) return hipErrorInvalidValue;
+  k_fused_geometry<TRI3><<<g, 256, 0, s>>>(E, X, conn_soa, geo);
+  return hipGetLastError();
 }
 
 template <int ET, int KIND>
