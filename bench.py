@@ -189,6 +189,17 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     return 2 * nbytes / t / 1e9
 
 
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(m, materials, blocked, v0, dt, warm=2, steps=10, threads=1):
     """The oracle on this host: `steps` timed steps after `warm` untimed ones."""
     from oracle import oracle as orc
@@ -368,6 +379,7 @@ def bench_cube(args, akx):
     log("device copy ceiling: %s GB/s" % (None if copy_gbs is None else round(copy_gbs, 1)))
 
     cpu = None
+    cpu_all = None
     bitwise = None
     if not args.no_cpu:
         threads = 1
@@ -379,6 +391,18 @@ def bench_cube(args, akx):
                          "-ffp-contract=off), full %d-element mesh, %d timed steps after 2 "
                          "warm-up steps, %.3f s/step" % (E, args.cpu_steps, sec)}
         log("CPU: %.3f s/step" % sec)
+        # the same on all the host cores this job may use (the oracle's element
+        # partition with per-thread force arrays reduced in a fixed order)
+        nthr = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+        _, sec_all = cpu_baseline(m, [mat], blocked, v0, dt, 2, args.cpu_steps, nthr)
+        cpu_model = host_cpu_model()
+        cpu_all = {"value": E * Q / sec_all, "unit": "QP-updates/s", "cores": nthr,
+                   "kind": "port",
+                   "sample": "oracle on %d threads, %s, full %d-element mesh, %d timed steps "
+                             "after 2 warm-up steps, %.3f s/step"
+                             % (nthr, cpu_model, E, args.cpu_steps, sec_all)}
+        cpu["sample"] += ", " + cpu_model
+        log("CPU, %d threads: %.3f s/step" % (nthr, sec_all))
         # the FP64 GPU path against the oracle on the full mesh, same step count
         reset(model, v0)
         model.set_launch(**shapes["fp64"])
@@ -412,6 +436,7 @@ def bench_cube(args, akx):
                           "node_gbs": nb / (res["mixed"]["node_ms"] * 1e-3) / 1e9},
         "copy_ceiling_gbs": copy_gbs,
         "cpu_baseline": cpu,
+        "cpu_all_cores": cpu_all,
     })
     return out
 
