@@ -870,6 +870,9 @@ int akx_init(akx_ctx* c) {
   for (int64_t e = 0; e < E; ++e)
     if (c->elem_mat[e] >= int(c->mats.size()))
       return fail(c, AKX_ERR_ARGUMENT, "element assigned to a missing material");
+  // a graph captured before holds the buffers freed here: the next graph
+  // step captures again
+  free_graph(c);
   free_derived(c);
 
   // assembly order: material by material (creation order), ascending element
