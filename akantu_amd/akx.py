@@ -321,11 +321,11 @@ class SolidMechanicsModel:
         return v.value
 
     # -- output (SolidMechanicsModel::setBaseName / addDumpField / dump) --
-    def set_base_name(self, name, directory="paraview", asynchronous=True):
+    def set_base_name(self, name, directory="paraview", asynchronous=True, binary=True):
         from .dumper import Dumper
         if getattr(self, "_dumper", None) is not None:
             self._dumper.close()
-        self._dumper = Dumper(self, name, directory, asynchronous)
+        self._dumper = Dumper(self, name, directory, asynchronous, binary=binary)
 
     def add_dump_field(self, name):
         if getattr(self, "_dumper", None) is None:

@@ -7,7 +7,11 @@ src/model/model.cc and src/io/dumper/), which writes nodal fields
 (displacement, velocity, ...) and per-element quadrature fields into
 `paraview/<base>_<step>.vtu`.  Nodal fields are point data; per-point fields
 of elements with several quadrature points are averaged per element as cell
-data.  Files are ASCII so that they can be read without any VTK library.
+data.  Arrays are written in VTK's inline binary format by default (per
+DataArray one base64 stream of a UInt64 byte count followed by the raw
+little-endian values, header_type="UInt64"), or as ASCII text with
+binary=False.  The mesh part of a file (points, connectivity, offsets, cell
+types) is encoded once per Dumper.
 
 Asynchronous by default: dump() takes a device snapshot of every field
 (akx_field_download_async), and one writer thread waits for the copies and
@@ -17,6 +21,7 @@ recycled once their file is written; at most `depth` dumps are in flight
 """
 from __future__ import annotations
 
+import base64
 import collections
 import concurrent.futures
 import os
@@ -35,36 +40,66 @@ NODAL = ("displacement", "velocity", "acceleration", "external_force", "internal
 POINTWISE = ("stress", "grad_u", "damage")
 
 
-def _array_xml(parent, name, values, ncomp):
+_NP = {"Float64": "<f8", "Int64": "<i8", "UInt8": "u1"}
+
+
+def _encode(values, kind, binary):
+    """The text of one DataArray: base64(UInt64 byte count + raw data) or ASCII."""
+    flat = np.ascontiguousarray(np.asarray(values).reshape(-1), dtype=_NP[kind])
+    if binary:
+        raw = flat.tobytes()
+        return base64.b64encode(np.array([len(raw)], dtype="<u8").tobytes() + raw).decode("ascii")
+    if kind == "Float64":
+        return " ".join(repr(float(v)) for v in flat)
+    return " ".join(str(int(v)) for v in flat)
+
+
+def _add_array(parent, name, kind, ncomp, text, binary):
+    da = ET.SubElement(parent, "DataArray", type=kind, Name=name,
+                       NumberOfComponents=str(ncomp), format="binary" if binary else "ascii")
+    da.text = text
+
+
+def _array_xml(parent, name, values, ncomp, binary):
     values = np.asarray(values)
     kind = "UInt8" if values.dtype == np.uint8 or values.dtype == bool else "Float64"
-    da = ET.SubElement(parent, "DataArray", type=kind, Name=name,
-                       NumberOfComponents=str(ncomp), format="ascii")
-    flat = values.reshape(-1)
-    if kind == "UInt8":
-        da.text = " ".join(str(int(v)) for v in flat)
-    else:
-        da.text = " ".join(repr(float(v)) for v in flat)
+    _add_array(parent, name, kind, ncomp, _encode(values, kind, binary), binary)
 
 
-def write_vtu(path, mesh, point_data=None, cell_data=None):
-    """Write one .vtu: coordinates padded to 3D, point and cell arrays."""
+def mesh_arrays(mesh, binary=True):
+    """The encoded mesh part of a .vtu (positions padded to 3D, connectivity,
+    offsets, cell types), reusable for every dump of one mesh."""
     N, d = mesh.nodes.shape
     E, n = mesh.conn.shape
-    root = ET.Element("VTKFile", type="UnstructuredGrid", version="0.1", byte_order="LittleEndian")
+    xyz = np.zeros((N, 3))
+    xyz[:, :d] = mesh.nodes
+    return {"positions": _encode(xyz, "Float64", binary),
+            "connectivity": _encode(mesh.conn, "Int64", binary),
+            "offsets": _encode(n * np.arange(1, E + 1, dtype=np.int64), "Int64", binary),
+            "types": _encode(np.full(E, VTK_CELL[mesh.elem_type], dtype=np.uint8), "UInt8",
+                             binary),
+            "binary": binary}
+
+
+def write_vtu(path, mesh, point_data=None, cell_data=None, binary=True, mesh_text=None):
+    """Write one .vtu: coordinates padded to 3D, point and cell arrays.
+    mesh_text: the result of mesh_arrays(mesh, binary), to skip re-encoding."""
+    N, d = mesh.nodes.shape
+    E, n = mesh.conn.shape
+    if mesh_text is None or mesh_text["binary"] != binary:
+        mesh_text = mesh_arrays(mesh, binary)
+    attrs = dict(type="UnstructuredGrid", version="0.1", byte_order="LittleEndian")
+    if binary:
+        attrs["header_type"] = "UInt64"
+    root = ET.Element("VTKFile", **attrs)
     grid = ET.SubElement(root, "UnstructuredGrid")
     piece = ET.SubElement(grid, "Piece", NumberOfPoints=str(N), NumberOfCells=str(E))
     pts = ET.SubElement(piece, "Points")
-    xyz = np.zeros((N, 3))
-    xyz[:, :d] = mesh.nodes
-    _array_xml(pts, "positions", xyz, 3)
+    _add_array(pts, "positions", "Float64", 3, mesh_text["positions"], binary)
     cells = ET.SubElement(piece, "Cells")
-    conn = ET.SubElement(cells, "DataArray", type="Int64", Name="connectivity", format="ascii")
-    conn.text = " ".join(str(int(v)) for v in mesh.conn.reshape(-1))
-    off = ET.SubElement(cells, "DataArray", type="Int64", Name="offsets", format="ascii")
-    off.text = " ".join(str(n * (k + 1)) for k in range(E))
-    typ = ET.SubElement(cells, "DataArray", type="UInt8", Name="types", format="ascii")
-    typ.text = " ".join([str(VTK_CELL[mesh.elem_type])] * E)
+    _add_array(cells, "connectivity", "Int64", 1, mesh_text["connectivity"], binary)
+    _add_array(cells, "offsets", "Int64", 1, mesh_text["offsets"], binary)
+    _add_array(cells, "types", "UInt8", 1, mesh_text["types"], binary)
     if point_data:
         pd = ET.SubElement(piece, "PointData")
         for name, arr in point_data.items():
@@ -73,12 +108,12 @@ def write_vtu(path, mesh, point_data=None, cell_data=None):
             if ncomp == 2:  # ParaView draws vectors with 3 components
                 arr = np.concatenate([arr, np.zeros((arr.shape[0], 1), arr.dtype)], axis=1)
                 ncomp = 3
-            _array_xml(pd, name, arr, ncomp)
+            _array_xml(pd, name, arr, ncomp, binary)
     if cell_data:
         cd = ET.SubElement(piece, "CellData")
         for name, arr in cell_data.items():
             arr = np.asarray(arr)
-            _array_xml(cd, name, arr, 1 if arr.ndim == 1 else arr.shape[1])
+            _array_xml(cd, name, arr, 1 if arr.ndim == 1 else arr.shape[1], binary)
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     ET.ElementTree(root).write(path, xml_declaration=True, encoding="utf-8")
 
@@ -95,8 +130,11 @@ def write_pvd(path, entries):
 class Dumper:
     """model.set_base_name / add_dump_field / dump, for akx.SolidMechanicsModel."""
 
-    def __init__(self, model, base_name, directory="paraview", asynchronous=True, depth=2):
+    def __init__(self, model, base_name, directory="paraview", asynchronous=True, depth=2,
+                 binary=True):
         self.model = model
+        self.binary = bool(binary)
+        self._mesh_text = None
         self.base = base_name
         self.dir = directory
         self.fields = []
@@ -128,7 +166,10 @@ class Dumper:
         return pd, cd
 
     def _write(self, name, arrays, entries):
-        write_vtu(os.path.join(self.dir, name), self.model.mesh, *self._data(arrays))
+        if self._mesh_text is None:
+            self._mesh_text = mesh_arrays(self.model.mesh, self.binary)
+        write_vtu(os.path.join(self.dir, name), self.model.mesh, *self._data(arrays),
+                  binary=self.binary, mesh_text=self._mesh_text)
         write_pvd(os.path.join(self.dir, self.base + ".pvd"), entries)
 
     def dump(self, time=None):

@@ -1,4 +1,5 @@
 """ParaView output (.vtu / .pvd) written by akantu_amd/dumper.py."""
+import base64
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -8,26 +9,39 @@ from akantu_amd import dumper
 from akantu_amd import mesh as akm
 
 
+_DT = {"Float64": "<f8", "Int64": "<i8", "UInt8": "u1"}
+
+
 def _arrays(path):
+    """Every DataArray of a .vtu, decoded from ASCII or from VTK's inline
+    binary form (base64 of a UInt64 byte count followed by the raw data)."""
     root = ET.parse(path).getroot()
     piece = root.find("UnstructuredGrid/Piece")
     out = {}
     for da in piece.iter("DataArray"):
-        vals = np.array([float(v) for v in da.text.split()])
+        if da.get("format") == "binary":
+            assert root.get("header_type") == "UInt64"
+            raw = base64.b64decode(da.text)
+            nbytes = int(np.frombuffer(raw[:8], dtype="<u8")[0])
+            assert len(raw) == 8 + nbytes
+            vals = np.frombuffer(raw[8:], dtype=_DT[da.get("type")]).astype(np.float64)
+        else:
+            vals = np.array([float(v) for v in da.text.split()])
         nc = int(da.get("NumberOfComponents", "1"))
         out[da.get("Name")] = vals.reshape(-1, nc) if nc > 1 else vals
     return piece, out
 
 
+@pytest.mark.parametrize("binary", [True, False])
 @pytest.mark.parametrize("make", [lambda: akm.rectangle_tri3(3, 2, 1.0, 0.5),
                                   lambda: akm.box_hex8(2, 2, 1)])
-def test_vtu_round_trip(tmp_path, make):
+def test_vtu_round_trip(tmp_path, make, binary):
     m = make()
     rng = np.random.default_rng(0)
     u = rng.normal(size=m.nodes.shape)
     s = rng.normal(size=(m.nb_elements, m.dim * m.dim))
     path = tmp_path / "out" / "a_0000.vtu"
-    dumper.write_vtu(str(path), m, {"displacement": u}, {"stress": s})
+    dumper.write_vtu(str(path), m, {"displacement": u}, {"stress": s}, binary=binary)
     piece, arr = _arrays(path)
     assert int(piece.get("NumberOfPoints")) == m.nb_nodes
     assert int(piece.get("NumberOfCells")) == m.nb_elements
@@ -37,6 +51,20 @@ def test_vtu_round_trip(tmp_path, make):
     assert np.array_equal(got, u)           # repr() keeps every bit
     assert np.array_equal(arr["stress"], s)
     assert set(arr["types"]) == {dumper.VTK_CELL[m.elem_type]}
+    assert np.array_equal(arr["offsets"], m.conn.shape[1] * np.arange(1, m.nb_elements + 1))
+
+
+def test_binary_vtu_reuses_the_mesh_encoding(tmp_path):
+    """Two dumps of one mesh written with the cached mesh encoding are the
+    files written without it."""
+    m = akm.box_tet4(3, 2, 2)
+    u = np.random.default_rng(1).normal(size=m.nodes.shape)
+    cache = dumper.mesh_arrays(m, True)
+    for k in range(2):
+        dumper.write_vtu(str(tmp_path / ("c%d.vtu" % k)), m, {"displacement": u * k},
+                         mesh_text=cache)
+        dumper.write_vtu(str(tmp_path / ("n%d.vtu" % k)), m, {"displacement": u * k})
+        assert (tmp_path / ("c%d.vtu" % k)).read_bytes() == (tmp_path / ("n%d.vtu" % k)).read_bytes()
 
 
 def test_pvd_lists_the_dumps(tmp_path):
