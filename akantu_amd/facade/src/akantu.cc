@@ -689,15 +689,46 @@ void SolidMechanicsModel::addDumpField(const std::string& field) {
 }
 
 namespace {
+// VTK inline binary data: base64 of a UInt64 byte count followed by the raw
+// little-endian values, one stream per DataArray (header_type="UInt64").
+std::string vtk_base64(const std::vector<unsigned char>& raw) {
+  static const char* T = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::vector<unsigned char> b(8 + raw.size());
+  const uint64_t n = raw.size();
+  std::memcpy(b.data(), &n, 8);
+  std::memcpy(b.data() + 8, raw.data(), raw.size());
+  std::string out;
+  out.reserve((b.size() + 2) / 3 * 4);
+  size_t i = 0;
+  for (; i + 2 < b.size(); i += 3) {
+    const uint32_t v = uint32_t(b[i]) << 16 | uint32_t(b[i + 1]) << 8 | b[i + 2];
+    out += T[v >> 18 & 63]; out += T[v >> 12 & 63]; out += T[v >> 6 & 63]; out += T[v & 63];
+  }
+  if (i < b.size()) {
+    const bool two = i + 1 < b.size();
+    const uint32_t v = uint32_t(b[i]) << 16 | (two ? uint32_t(b[i + 1]) << 8 : 0u);
+    out += T[v >> 18 & 63]; out += T[v >> 12 & 63];
+    out += two ? T[v >> 6 & 63] : '=';
+    out += '=';
+  }
+  return out;
+}
+
+template <typename V>
+void vtu_binary(std::ostream& o, const char* type, const std::string& name, UInt ncomp,
+                const std::vector<V>& v) {
+  std::vector<unsigned char> raw(v.size() * sizeof(V));
+  std::memcpy(raw.data(), v.data(), raw.size());
+  o << "<DataArray type=\"" << type << "\" Name=\"" << name << "\" NumberOfComponents=\""
+    << ncomp << "\" format=\"binary\">\n" << vtk_base64(raw) << "\n</DataArray>\n";
+}
+
 void vtu_array(std::ostream& o, const std::string& name, const Real* v, size_t count,
                UInt ncomp, UInt pad_to) {
-  o << "<DataArray type=\"Float64\" Name=\"" << name << "\" NumberOfComponents=\"" << pad_to
-    << "\" format=\"ascii\">\n";
-  for (size_t k = 0; k < count; ++k) {
-    for (UInt c = 0; c < pad_to; ++c) o << (c < ncomp ? v[k * ncomp + c] : 0.) << ' ';
-    o << '\n';
-  }
-  o << "</DataArray>\n";
+  std::vector<double> p(count * pad_to, 0.);
+  for (size_t k = 0; k < count; ++k)
+    for (UInt c = 0; c < ncomp; ++c) p[k * pad_to + c] = v[k * ncomp + c];
+  vtu_binary(o, "Float64", name, pad_to, p);
 }
 }  // namespace
 
@@ -772,20 +803,21 @@ void SolidMechanicsModel::dump() {
     if (!o) AKANTU_EXCEPTION("cannot write " << dir << "/" << file);
     o.precision(17);
     o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
-         "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\""
+         "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n<UnstructuredGrid>\n"
+         "<Piece NumberOfPoints=\""
       << N << "\" NumberOfCells=\"" << E << "\">\n<Points>\n";
     vtu_array(o, "positions", mesh->getNodes().storage(), N, d, 3);
-    o << "</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    o << "</Points>\n<Cells>\n";
     const auto& conn = mesh->getConnectivity(t);
+    std::vector<int64_t> cv(size_t(E) * n), ov(E);
     for (UInt e = 0; e < E; ++e) {
-      for (UInt a = 0; a < n; ++a) o << conn(e, a) << ' ';
-      o << '\n';
+      for (UInt a = 0; a < n; ++a) cv[size_t(e) * n + a] = int64_t(conn(e, a));
+      ov[e] = int64_t(e + 1) * n;
     }
-    o << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
-    for (UInt e = 0; e < E; ++e) o << size_t(e + 1) * n << '\n';
-    o << "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
-    for (UInt e = 0; e < E; ++e) o << cell << '\n';
-    o << "</DataArray>\n</Cells>\n<PointData>\n";
+    vtu_binary(o, "Int64", "connectivity", 1, cv);
+    vtu_binary(o, "Int64", "offsets", 1, ov);
+    vtu_binary(o, "UInt8", "types", 1, std::vector<uint8_t>(E, uint8_t(cell)));
+    o << "</Cells>\n<PointData>\n";
     for (const auto& it : items) {
       if (it.point) continue;
       if (it.bytes_u8) {

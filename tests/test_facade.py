@@ -109,11 +109,22 @@ def test_explicit_dynamic_driver_matches_oracle(precision, tmp_path):
     u0 = np.loadtxt(tmp_path / "u0.txt")
     u = np.loadtxt(tmp_path / "u.txt")
     # the last dump (written by the background writer) holds the final state
+    import base64
     import xml.etree.ElementTree as ET
-    piece = ET.parse(tmp_path / "paraview" / dumps[-1]).getroot().find("UnstructuredGrid/Piece")
-    arrays = {da.get("Name"): da.text for da in piece.iter("DataArray")}
-    assert np.array_equal(np.array(arrays["displacement"].split(), dtype=float).reshape(-1, 3), u)
-    assert len(arrays["stress"].split()) == 9 * m.nb_elements
+    root = ET.parse(tmp_path / "paraview" / dumps[-1]).getroot()
+    assert root.get("header_type") == "UInt64"
+    piece = root.find("UnstructuredGrid/Piece")
+    kinds = {"Float64": "<f8", "Int64": "<i8", "UInt8": "u1"}
+    arrays = {}
+    for da in piece.iter("DataArray"):
+        assert da.get("format") == "binary"
+        raw = base64.b64decode(da.text.strip())
+        assert int(np.frombuffer(raw[:8], dtype="<u8")[0]) == len(raw) - 8
+        arrays[da.get("Name")] = np.frombuffer(raw[8:], dtype=kinds[da.get("type")])
+    assert np.array_equal(arrays["displacement"].reshape(-1, 3), u)
+    assert arrays["stress"].size == 9 * m.nb_elements
+    assert np.array_equal(arrays["connectivity"].reshape(m.conn.shape), m.conn)
+    assert np.array_equal(arrays["offsets"], m.conn.shape[1] * np.arange(1, m.nb_elements + 1))
     dt = float((tmp_path / "dt.txt").read_text())
     en = np.loadtxt(tmp_path / "energy.csv", delimiter=",", skiprows=1, usecols=(0, 2, 3))
     assert en.shape == (steps, 3)
