@@ -78,6 +78,8 @@ void free_derived(akx_ctx* c) {
   c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
+  // one entry per material: sized by the material count at its first use
+  dfree(c->mat_dt);
   dfree(c->partial); dfree(c->scalar); dfree(c->gradu); dfree(c->stress); dfree(c->flag);
   for (auto& g : c->groups) dfree(g.filter);
   c->groups.clear();
@@ -95,7 +97,6 @@ void free_graph(akx_ctx* c) {
 void free_mesh_buffers(akx_ctx* c) {
   free_graph(c);
   dfree(c->dt_node);
-  dfree(c->mat_dt);
   free_derived(c);
   dfree(c->X); dfree(c->conn_soa);
   dfree(c->u); dfree(c->v); dfree(c->a); dfree(c->fext); dfree(c->fint); dfree(c->mass);
@@ -670,6 +671,25 @@ int read_blocked(akx_ctx* c, std::vector<uint8_t>& internal) {
   return AKX_OK;
 }
 
+// Whether the surrogate set up on this context can run the current mesh and
+// materials; sizes its per-point buffers for the current element count.
+int surrogate_check(akx_ctx* c) {
+  if (!c->sur.setup) return fail(c, AKX_ERR_STATE, "akx_surrogate_setup has not been called");
+  if (c->et != AKX_TETRAHEDRON_4)
+    return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate element kernel is for tetrahedron_4");
+  if (c->sur.mat < 0 || c->sur.mat >= int(c->mats.size()) ||
+      c->mats[c->sur.mat].kind != AKX_MATERIAL_NEOHOOKEAN_DAMAGE)
+    return fail(c, AKX_ERR_STATE, "the surrogate's material is not a neohookean_damage material");
+  for (const auto& m : c->mats)
+    if (m.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE &&
+        (m.lambda != c->sur.lambda || m.mu != c->sur.mu || m.Sd != c->sur.Sd))
+      return fail(c, AKX_ERR_UNSUPPORTED,
+                  "every neohookean_damage material must be the one the surrogate was set up for");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c, c->sur.ensure_points(c->E));
+  return AKX_OK;
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -755,6 +775,10 @@ int akx_mesh_set(akx_ctx* c, int et, int dim, int64_t nb_nodes, const double* no
   c->hconn.assign(conn, conn + nb_elements * n);
   c->elem_mat.assign(nb_elements, 0);
   c->mats.clear();
+  // the surrogate was set up for a material of the previous mesh
+  c->sur.enabled = false;
+  c->sur.setup = false;
+  c->sur.mat = -1;
   c->has_fext = false;
   c->has_bc_inc = false;
   for (double& x : c->bc_inc) x = 0;
@@ -870,6 +894,14 @@ int akx_init(akx_ctx* c) {
   for (int64_t e = 0; e < E; ++e)
     if (c->elem_mat[e] >= int(c->mats.size()))
       return fail(c, AKX_ERR_ARGUMENT, "element assigned to a missing material");
+  if (c->sur.enabled) {
+    // materials or element count may have changed since akx_surrogate_enable
+    const int src = surrogate_check(c);
+    if (src) {
+      c->sur.enabled = false;
+      return src;
+    }
+  }
   // a graph captured before holds the buffers freed here: the next graph
   // step captures again
   free_graph(c);
@@ -1639,6 +1671,7 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
 
 // ---------------------------------------------------------------------------
 // constitutive surrogate
+
 int akx_surrogate_setup(akx_ctx* c, int mat_id, double eps_max) {
   int rc = check_ready(c);
   if (rc) return rc;
@@ -1650,6 +1683,7 @@ int akx_surrogate_setup(akx_ctx* c, int mat_id, double eps_max) {
   if (c->d != 3) return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate is three-dimensional");
   if (!(eps_max > 0) || eps_max > 0.5) return fail(c, AKX_ERR_ARGUMENT, "eps_max in (0, 0.5]");
   (void)hipSetDevice(c->device);
+  free_graph(c);
   auto& S = c->sur;
   S.mat = mat_id;
   S.lambda = m.lambda;
@@ -1734,20 +1768,14 @@ int akx_surrogate_dataset(akx_ctx* c, int64_t first, int64_t count, float* x, fl
 int akx_surrogate_enable(akx_ctx* c, int on) {
   int rc = check_ready(c);
   if (rc) return rc;
+  // the captured steps hold the surrogate switch and its scales
+  free_graph(c);
   if (!on) {
     c->sur.enabled = false;
     return AKX_OK;
   }
-  if (!c->sur.setup) return fail(c, AKX_ERR_STATE, "akx_surrogate_setup has not been called");
-  if (c->et != AKX_TETRAHEDRON_4)
-    return fail(c, AKX_ERR_UNSUPPORTED, "the surrogate element kernel is for tetrahedron_4");
-  for (const auto& m : c->mats)
-    if (m.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE &&
-        (m.lambda != c->sur.lambda || m.mu != c->sur.mu || m.Sd != c->sur.Sd))
-      return fail(c, AKX_ERR_UNSUPPORTED,
-                  "every neohookean_damage material must be the one the surrogate was set up for");
-  (void)hipSetDevice(c->device);
-  HIP_TRY(c, c->sur.ensure_points(c->E));
+  rc = surrogate_check(c);
+  if (rc) return rc;
   c->sur.enabled = true;
   return AKX_OK;
 }

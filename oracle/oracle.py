@@ -27,41 +27,52 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def configure(L):
+    """Set the ctypes signatures of an oracle library (also used for copies
+    of the oracle built from deliberately altered sources, to show that the
+    tests catch a planted mistake)."""
+    L.orc_create.restype = _P
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _I64, _P, _I64, _P]
+    L.orc_destroy.argtypes = [_P]
+    L.orc_last_error.restype = ctypes.c_char_p
+    L.orc_last_error.argtypes = [_P]
+    L.orc_add_material.argtypes = [_P, ctypes.c_int, _D, _D, _D, ctypes.c_int]
+    L.orc_set_damage.argtypes = [_P, ctypes.c_int, _D, _P]
+    L.orc_assign_materials.argtypes = [_P, _P]
+    L.orc_init.argtypes = [_P]
+    L.orc_set_threads.argtypes = [_P, ctypes.c_int]
+    L.orc_field.restype = ctypes.POINTER(_D)
+    L.orc_field.argtypes = [_P, ctypes.c_int, ctypes.POINTER(_I64)]
+    L.orc_blocked.restype = ctypes.POINTER(ctypes.c_uint8)
+    L.orc_blocked.argtypes = [_P]
+    L.orc_set_time_step.argtypes = [_P, _D]
+    L.orc_assemble_internal_forces.argtypes = [_P]
+    L.orc_solve_step.argtypes = [_P, ctypes.c_int]
+    L.orc_stable_dt.restype = _D
+    L.orc_stable_dt.argtypes = [_P]
+    L.orc_energy.restype = _D
+    L.orc_energy.argtypes = [_P, ctypes.c_int]
+    L.orc_point_stress.restype = _D
+    L.orc_point_stress.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _D, _D,
+                                   _P, _P, _P]
+    L.orc_point_energy.restype = _D
+    L.orc_point_energy.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _P]
+    L.orc_point_finite_residual.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
+    L.orc_truss_dynamic_explicit.argtypes = [ctypes.c_char_p, _I64]
+    return L
+
+
+def load(path):
+    """An oracle library other than the default one, configured."""
+    return configure(ctypes.CDLL(path))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_create.restype = _P
-        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _I64, _P, _I64, _P]
-        L.orc_destroy.argtypes = [_P]
-        L.orc_last_error.restype = ctypes.c_char_p
-        L.orc_last_error.argtypes = [_P]
-        L.orc_add_material.argtypes = [_P, ctypes.c_int, _D, _D, _D, ctypes.c_int]
-        L.orc_set_damage.argtypes = [_P, ctypes.c_int, _D, _P]
-        L.orc_assign_materials.argtypes = [_P, _P]
-        L.orc_init.argtypes = [_P]
-        L.orc_set_threads.argtypes = [_P, ctypes.c_int]
-        L.orc_field.restype = ctypes.POINTER(_D)
-        L.orc_field.argtypes = [_P, ctypes.c_int, ctypes.POINTER(_I64)]
-        L.orc_blocked.restype = ctypes.POINTER(ctypes.c_uint8)
-        L.orc_blocked.argtypes = [_P]
-        L.orc_set_time_step.argtypes = [_P, _D]
-        L.orc_assemble_internal_forces.argtypes = [_P]
-        L.orc_solve_step.argtypes = [_P, ctypes.c_int]
-        L.orc_stable_dt.restype = _D
-        L.orc_stable_dt.argtypes = [_P]
-        L.orc_energy.restype = _D
-        L.orc_energy.argtypes = [_P, ctypes.c_int]
-        L.orc_point_stress.restype = _D
-        L.orc_point_stress.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _D, _D,
-                                       _P, _P, _P]
-        L.orc_point_energy.restype = _D
-        L.orc_point_energy.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _P]
-        L.orc_point_finite_residual.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
-        L.orc_truss_dynamic_explicit.argtypes = [ctypes.c_char_p, _I64]
-        _lib = L
+        _lib = configure(ctypes.CDLL(_LIB_PATH))
     return _lib
 
 
@@ -72,8 +83,9 @@ def _ptr(a):
 class OracleModel:
     """FP64 CPU restatement of SolidMechanicsModel's explicit lumped-mass path."""
 
-    def __init__(self, mesh, materials, elem_mat=None, threads=1):
-        L = lib()
+    def __init__(self, mesh, materials, elem_mat=None, threads=1, library=None):
+        L = library if library is not None else lib()
+        self._L = L
         self._X = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
         self._conn = np.ascontiguousarray(mesh.conn, dtype=np.uint32)
         self.dim = mesh.dim
@@ -100,12 +112,12 @@ class OracleModel:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     def field(self, name):
         n = _I64()
-        p = lib().orc_field(self.h, FIELDS[name], ctypes.byref(n))
+        p = self._L.orc_field(self.h, FIELDS[name], ctypes.byref(n))
         return np.ctypeslib.as_array(p, shape=(n.value,))
 
     def nodal(self, name):
@@ -113,28 +125,28 @@ class OracleModel:
 
     @property
     def blocked(self):
-        p = lib().orc_blocked(self.h)
+        p = self._L.orc_blocked(self.h)
         return np.ctypeslib.as_array(p, shape=(self.nb_nodes * self.dim,)).reshape(
             self.nb_nodes, self.dim)
 
     def set_time_step(self, dt):
-        lib().orc_set_time_step(self.h, dt)
+        self._L.orc_set_time_step(self.h, dt)
 
     def set_threads(self, n):
-        lib().orc_set_threads(self.h, n)
+        self._L.orc_set_threads(self.h, n)
 
     def solve_step(self, nsteps=1):
-        if lib().orc_solve_step(self.h, nsteps) != 0:
+        if self._L.orc_solve_step(self.h, nsteps) != 0:
             raise RuntimeError("oracle: not initialised")
 
     def assemble_internal_forces(self):
-        lib().orc_assemble_internal_forces(self.h)
+        self._L.orc_assemble_internal_forces(self.h)
 
     def stable_time_step(self):
-        return lib().orc_stable_dt(self.h)
+        return self._L.orc_stable_dt(self.h)
 
     def energy(self, kind):
-        return lib().orc_energy(self.h, {"kinetic": 0, "potential": 1, "external work": 2}[kind])
+        return self._L.orc_energy(self.h, {"kinetic": 0, "potential": 1, "external work": 2}[kind])
 
 
 def point_stress(kind, dim, E, nu, G, plane_stress=0, Sd=0.0, Yd=0.0, damage=0.0):
