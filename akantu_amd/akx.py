@@ -75,6 +75,19 @@ SIGNATURES = {
     "akx_surrogate_dataset": (_I, [_P, _I64, _I64, _P, _P]),
     "akx_surrogate_enable": (_I, [_P, _I]),
     "akx_surrogate_fallbacks": (_I, [_P, ctypes.POINTER(ctypes.c_uint64), _I]),
+    "akx_fe_gradient_on_integration_points": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_interpolate_on_integration_points": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_compute_btd": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_compute_ntb": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_integrate": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_integrate_scalar": (_I, [_P, _P, _P, _I64, _PD]),
+    "akx_fe_integrate_on_integration_points": (_I, [_P, _P, _I, _P, _I64, _P]),
+    "akx_fe_assemble_field_lumped": (_I, [_P, _P, _I, _P]),
+    "akx_material_elements": (_I, [_P, _I, ctypes.POINTER(_I64), _P]),
+    "akx_material_compute_stress": (_I, [_P, _I, _P, _P]),
+    "akx_material_compute_all_stresses": (_I, [_P, _I, _P, _P]),
+    "akx_material_assemble_internal_forces": (_I, [_P, _I]),
+    "akx_material_potential_energy": (_I, [_P, _I, _PD]),
 }
 
 
@@ -150,6 +163,98 @@ class Context:
 
     def __del__(self):
         self.close()
+
+
+def _elements(elements):
+    if elements is None:
+        return None, 0, None
+    ids = np.ascontiguousarray(elements, dtype=np.uint32)
+    return _ptr(ids), ids.size, ids
+
+
+class FEEngine:
+    """FEEngine operators of a model's mesh (src/fe_engine/fe_engine.hh:95-244),
+    FP64 on the GPU in the reference's operation order.  `elements` is the
+    reference's filter_elements (caller element ids; None = all); per-point
+    arrays have one row per point of the listed elements (element-major)."""
+
+    def __init__(self, model):
+        self.model = model
+        self.ctx = model.ctx
+        m = model.mesh
+        self.dim, self.nb_nodes_per_element, self.nb_qp = m.dim, m.conn.shape[1], m.nb_qp
+
+    def _count(self, elements):
+        return self.model.mesh.nb_elements if elements is None else len(elements)
+
+    def get_nb_integration_points(self):
+        return self.nb_qp
+
+    def get_shapes_derivatives(self):
+        return self.model.get("shapes_derivatives")
+
+    def get_jacobians(self):
+        return self.model.get("jacobians")
+
+    def _op(self, fn, arr, nb, out_cols, elements):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        p, n, keep = _elements(elements)
+        out = np.zeros((self._count(elements) * self.nb_qp, out_cols))
+        self.ctx.check(fn(self.ctx.h, _ptr(arr), int(nb), p, n, _ptr(out)))
+        return out
+
+    def gradient_on_integration_points(self, u, nb_degree_of_freedom=None, elements=None):
+        u = np.asarray(u, dtype=np.float64)
+        nb = nb_degree_of_freedom or (u.shape[1] if u.ndim == 2 else 1)
+        return self._op(lib.akx_fe_gradient_on_integration_points, u, nb, nb * self.dim, elements)
+
+    def interpolate_on_integration_points(self, u, nb_degree_of_freedom=None, elements=None):
+        u = np.asarray(u, dtype=np.float64)
+        nb = nb_degree_of_freedom or (u.shape[1] if u.ndim == 2 else 1)
+        return self._op(lib.akx_fe_interpolate_on_integration_points, u, nb, nb, elements)
+
+    def compute_btd(self, Ds, elements=None):
+        Ds = np.asarray(Ds, dtype=np.float64)
+        rows = Ds.shape[1] // self.dim
+        return self._op(lib.akx_fe_compute_btd, Ds, rows, rows * self.nb_nodes_per_element,
+                        elements)
+
+    def compute_ntb(self, bs, elements=None):
+        bs = np.asarray(bs, dtype=np.float64)
+        nb = bs.shape[1] if bs.ndim == 2 else 1
+        return self._op(lib.akx_fe_compute_ntb, bs, nb, nb * self.nb_nodes_per_element, elements)
+
+    def integrate(self, f, nb_degree_of_freedom=None, elements=None):
+        """Per-element integrals (n_el, nb_dof) of a per-point field."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        nb = nb_degree_of_freedom or (f.shape[1] if f.ndim == 2 else 1)
+        p, n, keep = _elements(elements)
+        out = np.zeros((self._count(elements), nb))
+        self.ctx.check(lib.akx_fe_integrate(self.ctx.h, _ptr(f), int(nb), p, n, _ptr(out)))
+        return out
+
+    def integrate_scalar(self, f, elements=None):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        p, n, keep = _elements(elements)
+        v = ctypes.c_double()
+        self.ctx.check(lib.akx_fe_integrate_scalar(self.ctx.h, _ptr(f), p, n, ctypes.byref(v)))
+        return v.value
+
+    def integrate_on_integration_points(self, f, nb_degree_of_freedom=None, elements=None):
+        f = np.asarray(f, dtype=np.float64)
+        nb = nb_degree_of_freedom or (f.shape[1] if f.ndim == 2 else 1)
+        return self._op(lib.akx_fe_integrate_on_integration_points, f, nb, nb, elements)
+
+    def assemble_field_lumped(self, field, lumped=None):
+        """lumped (N, nb_dof) += row-sum lumping of a per-point field."""
+        field = np.ascontiguousarray(field, dtype=np.float64)
+        nb = field.shape[1] if field.ndim == 2 else 1
+        if lumped is None:
+            lumped = np.zeros((self.model.mesh.nb_nodes, nb))
+        out = np.ascontiguousarray(lumped, dtype=np.float64)
+        self.ctx.check(lib.akx_fe_assemble_field_lumped(self.ctx.h, _ptr(field), int(nb),
+                                                        _ptr(out)))
+        return out
 
 
 class SolidMechanicsModel:
@@ -263,6 +368,45 @@ class SolidMechanicsModel:
                                         ctypes.byref(best)))
         shape = {k: int(v) for k, v in zip(self.LAUNCH_KEYS, chosen)}
         return shape, best.value
+
+    # -- FEEngine and Material operators (fe_engine.hh, material.hh:110-195) --
+    def get_fe_engine(self):
+        return FEEngine(self)
+
+    def material_elements(self, mat_id):
+        n = ctypes.c_int64()
+        self.ctx.check(lib.akx_material_elements(self.ctx.h, int(mat_id), ctypes.byref(n), None))
+        ids = np.zeros(n.value, dtype=np.uint32)
+        self.ctx.check(lib.akx_material_elements(self.ctx.h, int(mat_id), ctypes.byref(n),
+                                                 _ptr(ids)))
+        return ids
+
+    def material_compute_stress(self, mat_id, grad_u):
+        """Material::computeStress on the material's points from given gradients."""
+        g = np.ascontiguousarray(grad_u, dtype=np.float64)
+        out = np.zeros_like(g)
+        self.ctx.check(lib.akx_material_compute_stress(self.ctx.h, int(mat_id), _ptr(g),
+                                                       _ptr(out)))
+        return out
+
+    def material_compute_all_stresses(self, mat_id):
+        """Material::computeAllStresses; returns (grad_u, stress) of its points."""
+        rows = len(self.material_elements(mat_id)) * self.mesh.nb_qp
+        g = np.zeros((rows, self.dim * self.dim))
+        s = np.zeros_like(g)
+        self.ctx.check(lib.akx_material_compute_all_stresses(self.ctx.h, int(mat_id), _ptr(g),
+                                                             _ptr(s)))
+        return g, s
+
+    def material_assemble_internal_forces(self, mat_id):
+        """Material::assembleInternalForces: adds into the internal force."""
+        self.ctx.check(lib.akx_material_assemble_internal_forces(self.ctx.h, int(mat_id)))
+
+    def material_potential_energy(self, mat_id):
+        v = ctypes.c_double()
+        self.ctx.check(lib.akx_material_potential_energy(self.ctx.h, int(mat_id),
+                                                         ctypes.byref(v)))
+        return v.value
 
     # -- constitutive surrogate (neohookean_damage on tetrahedron_4) --
     def surrogate_setup(self, mat_id, eps_max):

@@ -1090,8 +1090,10 @@ int akx_field_upload(akx_ctx* c, int field, const void* host, size_t bytes) {
     return AKX_OK;
   }
   double* p = nodal_ptr(c, field);
-  if (!p || field == AKX_INTERNAL_FORCE || field == AKX_DISPLACEMENT_INCREMENT)
+  if (!p || field == AKX_DISPLACEMENT_INCREMENT)
     return fail(c, AKX_ERR_ARGUMENT, "field cannot be uploaded");
+  // the internal force is writable as the reference's getInternalForce() is:
+  // Material::assembleInternalForces adds into it
   if (bytes != nd * 8) return fail(c, AKX_ERR_ARGUMENT, "size mismatch");
   std::vector<double> tmp(nd);
   to_internal_nodal(c, static_cast<const double*>(host), tmp.data());
@@ -1900,6 +1902,355 @@ int akx_set_launch(akx_ctx* c, int element_block, int node_block, int xcd_remap,
   if (mixed_recompute >= 0) c->mixed_recompute = mixed_recompute ? 1 : 0;
   if (slot_layout >= 0) c->node_ordered = slot_layout;
   if (node_stage >= 0) c->node_stage = node_stage ? 1 : 0;
+  return AKX_OK;
+}
+
+}  // extern "C"
+
+/* ======================================================================== */
+/* FEEngine and Material operators on caller fields (akx_fe.hip)            */
+/* ======================================================================== */
+namespace {
+
+// A device buffer released at the end of an operator call (the calls
+// synchronise before they return).
+struct TempBuf {
+  void* p = nullptr;
+  ~TempBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* get() { return static_cast<T*>(p); }
+};
+
+hipError_t temp_alloc(TempBuf& b, size_t bytes) {
+  return hipMalloc(&b.p, std::max<size_t>(bytes, 8));
+}
+
+// The caller's element list (nullptr: every element, caller order) as
+// internal ids on the device; returns the row count through *count.
+int fe_element_list(akx_ctx* c, const uint32_t* ids, int64_t nb, TempBuf& dev, int64_t* count) {
+  if (ids && nb < 0) return fail(c, AKX_ERR_ARGUMENT, "negative element count");
+  const int64_t cnt = ids ? nb : c->E;
+  std::vector<uint32_t> internal(size_t(std::max<int64_t>(cnt, 0)));
+  for (int64_t t = 0; t < cnt; ++t) {
+    const uint32_t e = ids ? ids[t] : uint32_t(t);
+    if (int64_t(e) >= c->E) return fail(c, AKX_ERR_ARGUMENT, "element id out of range");
+    internal[size_t(t)] = c->elem_inv[e];
+  }
+  HIP_TRY(c, temp_alloc(dev, internal.size() * 4));
+  HIP_TRY(c, hipMemcpy(dev.p, internal.data(), internal.size() * 4, hipMemcpyHostToDevice));
+  *count = cnt;
+  return AKX_OK;
+}
+
+FeArgs fe_args(akx_ctx* c, const uint32_t* filt, int64_t count, int nb) {
+  FeArgs A;
+  std::memset(&A, 0, sizeof A);
+  A.count = count;
+  A.filt = filt;
+  A.E = c->E;
+  A.conn = c->conn_soa;
+  A.B = c->B64;
+  A.J = c->J64;
+  A.tab = make_tables(c->et);
+  A.nb = nb;
+  return A;
+}
+
+// A nodal caller array (N, nb) in the internal node numbering, on the device.
+int upload_nodal(akx_ctx* c, const double* host, int nb, TempBuf& dev) {
+  const size_t N = size_t(c->N);
+  std::vector<double> h(N * nb);
+  for (size_t k = 0; k < N; ++k)
+    for (int i = 0; i < nb; ++i) h[k * nb + i] = host[size_t(c->node_perm[k]) * nb + i];
+  HIP_TRY(c, temp_alloc(dev, h.size() * 8));
+  HIP_TRY(c, hipMemcpy(dev.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  return AKX_OK;
+}
+
+int upload_rows(akx_ctx* c, const double* host, size_t n, TempBuf& dev) {
+  HIP_TRY(c, temp_alloc(dev, n * 8));
+  if (n) HIP_TRY(c, hipMemcpy(dev.p, host, n * 8, hipMemcpyHostToDevice));
+  return AKX_OK;
+}
+
+int download_rows(akx_ctx* c, const TempBuf& dev, size_t n, double* host) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (n) HIP_TRY(c, hipMemcpy(host, dev.p, n * 8, hipMemcpyDeviceToHost));
+  return AKX_OK;
+}
+
+// One FEEngine operator: rows_in / rows_out values per listed element.
+int fe_op(akx_ctx* c, int op, const double* in, int nb, bool nodal_in, size_t in_per_elem,
+          size_t out_per_elem, const uint32_t* ids, int64_t nb_ids, double* out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!in || !out) return fail(c, AKX_ERR_ARGUMENT, "null array");
+  (void)hipSetDevice(c->device);
+  TempBuf filt, din, dout;
+  int64_t cnt = 0;
+  if ((rc = fe_element_list(c, ids, nb_ids, filt, &cnt))) return rc;
+  if (nodal_in) rc = upload_nodal(c, in, nb, din);
+  else rc = upload_rows(c, in, size_t(cnt) * in_per_elem, din);
+  if (rc) return rc;
+  HIP_TRY(c, temp_alloc(dout, size_t(cnt) * out_per_elem * 8));
+  FeArgs A = fe_args(c, filt.get<uint32_t>(), cnt, nb);
+  A.in = din.get<double>();
+  A.out = dout.get<double>();
+  HIP_TRY(c, launch_fe(op, c->et, c->stream, A));
+  return download_rows(c, dout, size_t(cnt) * out_per_elem, out);
+}
+
+MatParams mat_params(const HostMaterial& m) {
+  MatParams p;
+  std::memset(&p, 0, sizeof p);
+  p.lambda = m.lambda; p.mu = m.mu; p.rho = m.rho; p.Sd = m.Sd;
+  p.lambda_f = float(m.lambda); p.mu_f = float(m.mu); p.Sd_f = float(m.Sd);
+  p.inv_Sd_f = m.Sd > 0 ? float(1. / m.Sd) : 0.f;
+  p.kind = m.kind;
+  return p;
+}
+
+// Caller ids of a material's elements, ascending (its element filter).
+std::vector<uint32_t> material_filter(const akx_ctx* c, int mat) {
+  std::vector<uint32_t> ids;
+  for (int64_t e = 0; e < c->E; ++e)
+    if (c->elem_mat[e] == mat) ids.push_back(uint32_t(e));
+  return ids;
+}
+
+int material_setup(akx_ctx* c, int mat, TempBuf& filt, int64_t* cnt, MatArgs* M) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (mat < 0 || mat >= int(c->mats.size())) return fail(c, AKX_ERR_ARGUMENT, "no such material");
+  (void)hipSetDevice(c->device);
+  const std::vector<uint32_t> ids = material_filter(c, mat);
+  if ((rc = fe_element_list(c, ids.data(), int64_t(ids.size()), filt, cnt))) return rc;
+  const size_t ne = size_t(c->E) * c->Q * c->d * c->d;
+  if (!c->gradu) {
+    HIP_TRY(c, dalloc(&c->gradu, ne));
+    HIP_TRY(c, dalloc(&c->stress, ne));
+  }
+  std::memset(M, 0, sizeof *M);
+  M->mat = mat_params(c->mats[mat]);
+  M->gradu = c->gradu;
+  M->stress = c->stress;
+  M->damage = c->damage;
+  M->Yd = c->Yd;
+  return AKX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int akx_fe_gradient_on_integration_points(akx_ctx* c, const double* u, int nb_dof,
+                                          const uint32_t* elements, int64_t nb_elements,
+                                          double* nablauq) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (nb_dof < 1 || nb_dof > FE_MAX_DOF) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 9");
+  return fe_op(c, FE_GRADIENT, u, nb_dof, true, 0, size_t(c->Q) * nb_dof * c->d, elements,
+               nb_elements, nablauq);
+}
+
+int akx_fe_interpolate_on_integration_points(akx_ctx* c, const double* u, int nb_dof,
+                                             const uint32_t* elements, int64_t nb_elements,
+                                             double* uq) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (nb_dof < 1 || nb_dof > FE_MAX_DOF) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 9");
+  return fe_op(c, FE_INTERPOLATE, u, nb_dof, true, 0, size_t(c->Q) * nb_dof, elements,
+               nb_elements, uq);
+}
+
+int akx_fe_compute_btd(akx_ctx* c, const double* Ds, int rows, const uint32_t* elements,
+                       int64_t nb_elements, double* BtDs) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (rows < 1 || rows > FE_MAX_DOF) return fail(c, AKX_ERR_ARGUMENT, "1 <= rows <= 9");
+  return fe_op(c, FE_BTD, Ds, rows, false, size_t(c->Q) * rows * c->d,
+               size_t(c->Q) * rows * c->n, elements, nb_elements, BtDs);
+}
+
+int akx_fe_compute_ntb(akx_ctx* c, const double* bs, int nb_dof, const uint32_t* elements,
+                       int64_t nb_elements, double* Ntbs) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (nb_dof < 1 || nb_dof > 24) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 24");
+  return fe_op(c, FE_NTB, bs, nb_dof, false, size_t(c->Q) * nb_dof, size_t(c->Q) * nb_dof * c->n,
+               elements, nb_elements, Ntbs);
+}
+
+int akx_fe_integrate(akx_ctx* c, const double* f, int nb_dof, const uint32_t* elements,
+                     int64_t nb_elements, double* intf) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (nb_dof < 1 || nb_dof > 192) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 192");
+  return fe_op(c, FE_INTEGRATE, f, nb_dof, false, size_t(c->Q) * nb_dof, size_t(nb_dof),
+               elements, nb_elements, intf);
+}
+
+int akx_fe_integrate_on_integration_points(akx_ctx* c, const double* f, int nb_dof,
+                                           const uint32_t* elements, int64_t nb_elements,
+                                           double* out) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (nb_dof < 1 || nb_dof > 192) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 192");
+  return fe_op(c, FE_INTEGRATE_POINTS, f, nb_dof, false, size_t(c->Q) * nb_dof,
+               size_t(c->Q) * nb_dof, elements, nb_elements, out);
+}
+
+int akx_fe_integrate_scalar(akx_ctx* c, const double* f, const uint32_t* elements,
+                            int64_t nb_elements, double* value) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!f || !value) return fail(c, AKX_ERR_ARGUMENT, "null array");
+  (void)hipSetDevice(c->device);
+  TempBuf filt, din, per, tmp, sort, res;
+  int64_t cnt = 0;
+  if ((rc = fe_element_list(c, elements, nb_elements, filt, &cnt))) return rc;
+  if (cnt == 0) {
+    *value = 0.;
+    return AKX_OK;
+  }
+  if (cnt >= int64_t(1) << 31) return fail(c, AKX_ERR_ARGUMENT, "too many elements");
+  if ((rc = upload_rows(c, f, size_t(cnt) * c->Q, din))) return rc;
+  HIP_TRY(c, temp_alloc(per, size_t(cnt) * 8));
+  HIP_TRY(c, temp_alloc(tmp, size_t(cnt) * 8));
+  HIP_TRY(c, temp_alloc(res, 8));
+  FeArgs A = fe_args(c, filt.get<uint32_t>(), cnt, 1);
+  A.in = din.get<double>();
+  A.out = per.get<double>();
+  HIP_TRY(c, launch_fe(FE_INTEGRATE, c->et, c->stream, A));
+  size_t sort_bytes = 0;
+  HIP_TRY(c, launch_reduce_sorted(c->stream, cnt, per.get<double>(), tmp.get<double>(),
+                                  res.get<double>(), nullptr, &sort_bytes));
+  HIP_TRY(c, temp_alloc(sort, sort_bytes));
+  HIP_TRY(c, launch_reduce_sorted(c->stream, cnt, per.get<double>(), tmp.get<double>(),
+                                  res.get<double>(), sort.p, &sort_bytes));
+  return download_rows(c, res, 1, value);
+}
+
+int akx_fe_assemble_field_lumped(akx_ctx* c, const double* field, int nb_dof, double* lumped) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!field || !lumped) return fail(c, AKX_ERR_ARGUMENT, "null array");
+  if (nb_dof < 1 || nb_dof > 24) return fail(c, AKX_ERR_ARGUMENT, "1 <= nb_dof <= 24");
+  (void)hipSetDevice(c->device);
+  const int64_t E = c->E, N = c->N;
+  const int Q = c->Q, n = c->n;
+  // the field in internal element order, the lumped vector in internal node order
+  std::vector<double> fi(size_t(E) * Q * nb_dof);
+  for (int64_t e = 0; e < E; ++e)
+    std::memcpy(&fi[size_t(e) * Q * nb_dof], field + size_t(c->elem_perm[e]) * Q * nb_dof,
+                sizeof(double) * Q * nb_dof);
+  TempBuf din, integ, dl;
+  if ((rc = upload_rows(c, fi.data(), fi.size(), din))) return rc;
+  if ((rc = upload_nodal(c, lumped, nb_dof, dl))) return rc;
+  HIP_TRY(c, temp_alloc(integ, size_t(E) * n * nb_dof * 8));
+  FeArgs A = fe_args(c, nullptr, E, nb_dof);
+  A.in = din.get<double>();
+  A.out = integ.get<double>();
+  HIP_TRY(c, launch_fe(FE_LUMPED_ELEMENTS, c->et, c->stream, A));
+  HIP_TRY(c, launch_fe_lumped_nodes(c->stream, N, nb_dof, n, c->mass_off, c->mass_slot,
+                                    integ.get<double>(), dl.get<double>()));
+  std::vector<double> h(size_t(N) * nb_dof);
+  if ((rc = download_rows(c, dl, h.size(), h.data()))) return rc;
+  for (int64_t k = 0; k < N; ++k)
+    for (int i = 0; i < nb_dof; ++i)
+      lumped[size_t(c->node_perm[k]) * nb_dof + i] = h[size_t(k) * nb_dof + i];
+  return AKX_OK;
+}
+
+int akx_material_elements(akx_ctx* c, int mat, int64_t* count, uint32_t* ids) {
+  if (!c || !count) return AKX_ERR_ARGUMENT;
+  if (mat < 0 || mat >= int(c->mats.size())) return fail(c, AKX_ERR_ARGUMENT, "no such material");
+  const std::vector<uint32_t> f = material_filter(c, mat);
+  *count = int64_t(f.size());
+  if (ids) std::copy(f.begin(), f.end(), ids);
+  return AKX_OK;
+}
+
+int akx_material_compute_stress(akx_ctx* c, int mat, const double* grad_u, double* stress) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (!grad_u) return fail(c, AKX_ERR_ARGUMENT, "null gradient");
+  TempBuf filt, din, dout;
+  int64_t cnt = 0;
+  MatArgs M;
+  int rc = material_setup(c, mat, filt, &cnt, &M);
+  if (rc) return rc;
+  const size_t rows = size_t(cnt) * c->Q * c->d * c->d;
+  if ((rc = upload_rows(c, grad_u, rows, din))) return rc;
+  if (stress) HIP_TRY(c, temp_alloc(dout, rows * 8));
+  FeArgs A = fe_args(c, filt.get<uint32_t>(), cnt, c->d * c->d);
+  A.in = din.get<double>();
+  A.out = stress ? dout.get<double>() : nullptr;
+  HIP_TRY(c, launch_material(MAT_STRESS, c->et, c->mats[mat].kind, c->stream, A, M));
+  if (stress) return download_rows(c, dout, rows, stress);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AKX_OK;
+}
+
+int akx_material_compute_all_stresses(akx_ctx* c, int mat, double* grad_u, double* stress) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  TempBuf filt;
+  int64_t cnt = 0;
+  MatArgs M;
+  int rc = material_setup(c, mat, filt, &cnt, &M);
+  if (rc) return rc;
+  FeArgs A = fe_args(c, filt.get<uint32_t>(), cnt, c->d);
+  A.in = c->u;
+  HIP_TRY(c, launch_material(MAT_ALL_STRESSES, c->et, c->mats[mat].kind, c->stream, A, M));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (grad_u || stress) {
+    // the material's rows, gathered from the internal point fields
+    const int Q = c->Q, dd = c->d * c->d;
+    const size_t ne = size_t(c->E) * Q * dd;
+    std::vector<double> g(ne), s(ne);
+    HIP_TRY(c, hipMemcpy(g.data(), c->gradu, ne * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(s.data(), c->stress, ne * 8, hipMemcpyDeviceToHost));
+    const std::vector<uint32_t> ids = material_filter(c, mat);
+    for (size_t t = 0; t < ids.size(); ++t) {
+      const size_t e = c->elem_inv[ids[t]];
+      if (grad_u) std::memcpy(grad_u + t * Q * dd, &g[e * Q * dd], sizeof(double) * Q * dd);
+      if (stress) std::memcpy(stress + t * Q * dd, &s[e * Q * dd], sizeof(double) * Q * dd);
+    }
+  }
+  return AKX_OK;
+}
+
+int akx_material_assemble_internal_forces(akx_ctx* c, int mat) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  TempBuf filt;
+  int64_t cnt = 0;
+  MatArgs M;
+  int rc = material_setup(c, mat, filt, &cnt, &M);
+  if (rc) return rc;
+  FeArgs A = fe_args(c, filt.get<uint32_t>(), cnt, c->d);
+  A.out = static_cast<double*>(c->slots);
+  HIP_TRY(c, launch_material(MAT_FORCES, c->et, c->mats[mat].kind, c->stream, A, M));
+  HIP_TRY(c, launch_material_gather(c->stream, c->N, c->d, c->n, mat, c->csr_off, c->csr_slot,
+                                    c->elem_mat_d, static_cast<double*>(c->slots), c->fint));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->fint_valid = true;
+  return AKX_OK;
+}
+
+int akx_material_potential_energy(akx_ctx* c, int mat, double* value) {
+  if (!c || !value) return AKX_ERR_ARGUMENT;
+  TempBuf filt;
+  int64_t cnt = 0;
+  MatArgs M;
+  int rc = material_setup(c, mat, filt, &cnt, &M);
+  if (rc) return rc;
+  if (cnt == 0) {
+    *value = 0.;
+    return AKX_OK;
+  }
+  HIP_TRY(c, launch_energy(AKX_ENERGY_POTENTIAL, c->et, c->mats[mat].kind, c->stream, c->N, c->E,
+                           c->u, c->v, c->mass, c->fext, c->fint, c->blocked, c->conn_soa, c->B64,
+                           c->J64, c->elem_mat_d, c->mats_d, c->damage, filt.get<uint32_t>(), cnt,
+                           c->partial, c->partial_blocks, c->scalar + 1));
+  double r = 0;
+  HIP_TRY(c, hipMemcpyAsync(&r, c->scalar + 1, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *value = r;
   return AKX_OK;
 }
 

@@ -185,6 +185,44 @@ hipError_t launch_add(hipStream_t s, int64_t n, const double* x, const double* y
 hipError_t launch_snapshot(hipStream_t s, int64_t rows, int dw, int sw, const uint32_t* inv,
                            const void* src, void* dst, int elem_bytes);
 
+// FEEngine / Material operators on caller fields (akx_fe.hip), FP64 in the
+// reference's operation order.
+constexpr int FE_MAX_DOF = 9;  // components of a nodal field (gradient, interpolation)
+enum FeOp {
+  FE_GRADIENT, FE_INTERPOLATE, FE_BTD, FE_NTB, FE_INTEGRATE, FE_INTEGRATE_POINTS,
+  FE_LUMPED_ELEMENTS
+};
+enum MatOp { MAT_STRESS, MAT_ALL_STRESSES, MAT_FORCES };
+struct FeArgs {
+  int64_t count;          // listed elements (rows of the per-element arrays)
+  const uint32_t* filt;   // their internal ids, nullptr = 0 .. count-1
+  int64_t E;              // total elements (stride of the SoA tables)
+  const uint32_t* conn;   // [a][E]
+  const double* B;        // [(q*dn + k)][E]
+  const double* J;        // [q][E]
+  ElemTables tab;         // N of the quadrature points
+  const double* in;       // operand: nodal (internal numbering) or per-element rows
+  double* out;            // result rows
+  int nb;                 // components of the operand (rows of D for computeBtD)
+};
+struct MatArgs {
+  MatParams mat;
+  double* gradu;          // [(e*Q + q)][d*d], internal element ids
+  double* stress;
+  double* damage;         // [e*Q + q]
+  const double* Yd;
+};
+hipError_t launch_fe(int op, int et, hipStream_t s, const FeArgs& A);
+hipError_t launch_fe_lumped_nodes(hipStream_t s, int64_t N, int nb, int n, const uint32_t* off,
+                                  const uint32_t* slot, const double* integ, double* lumped);
+hipError_t launch_reduce_sorted(hipStream_t s, int64_t m, double* a, double* tmp, double* out,
+                                void* sort_tmp, size_t* sort_bytes);
+hipError_t launch_material(int op, int et, int kind, hipStream_t s, const FeArgs& A,
+                           const MatArgs& M);
+hipError_t launch_material_gather(hipStream_t s, int64_t N, int d, int n, int mat,
+                                  const uint32_t* off, const uint32_t* slot,
+                                  const uint16_t* elem_mat, const double* fe, double* fint);
+
 // The constitutive surrogate of one context: training set, FP32 master
 // weights with Adam moments, BF16 working copies, batch buffers
 // (akx_surrogate.hip).

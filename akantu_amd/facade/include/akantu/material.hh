@@ -1,8 +1,10 @@
 /**
  * material.hh -- constitutive laws of the explicit path as the façade sees
- * them: parameters (registerParam / getParam / setParam, material.cc:104-121)
- * and the per-point fields a driver reads (getStress, getGradU,
- * getPotentialEnergy).  The arithmetic runs in libakx.so.
+ * them: parameters (registerParam / getParam / setParam, material.cc:104-121),
+ * the per-point fields a driver reads (getStress, getGradU, getDamage), and
+ * the operators of material.hh:110-195 (computeStress, computeAllStresses,
+ * assembleInternalForces, getPotentialEnergy).  The arithmetic runs in
+ * libakx.so, in FP64 with the reference's operation order.
  *
  * Laws (MaterialFactory ids, `material <id> [ ... ]` input-file sections):
  *   elastic           MaterialElastic<dim>       (material_elastic.cc)
@@ -20,6 +22,7 @@
 
 #include "aka_array.hh"
 #include "aka_common.hh"
+#include "fe_engine.hh"
 
 namespace akantu {
 
@@ -33,6 +36,8 @@ class Material {
   const ID& getID() const { return id_; }
   const std::string& getName() const { return name_; }
   int getKind() const { return kind_; }
+  /// index of the material in the model (creation order)
+  UInt getIndex() const;
 
   /// parameters: "name", "rho", "E", "nu", "Plane_Stress", "Sd", "Yd"
   Real getParam(const std::string& key) const;
@@ -51,12 +56,26 @@ class Material {
   Real getPushWaveSpeed() const;
   Real getCelerity() const { return getPushWaveSpeed(); }
 
-  /// per-point fields of this material's elements (E_m*Q, d*d), column-major
+  /// per-point fields of this material's elements (E_m*Q, d*d), column-major:
+  /// the internals of the last computeStress / computeAllStresses, or, when
+  /// the model has stepped since, the fields of the current displacement
   const Array<Real>& getStress(ElementType type);
-  const Array<Real>& getGradU(ElementType type);
+  /// writable, as the "grad_u" internal is: computeStress reads it
+  Array<Real>& getGradU(ElementType type);
   const Array<Real>& getDamage(ElementType type);
   Real getPotentialEnergy();
   Real getEnergy(const std::string& id);
+
+  /// Material::computeStress (material.hh:110-113): stress of this material's
+  /// points from its gradient internal (getGradU); updates the damage history
+  void computeStress(ElementType type, GhostType ghost_type = _not_ghost);
+  /// Material::computeAllStresses (material.hh:195, material.cc:254-279):
+  /// gradient of the model's displacement on the points, then computeStress
+  void computeAllStresses(GhostType ghost_type = _not_ghost);
+  /// Material::assembleInternalForces (material.hh:186, material.cc:178-246,
+  /// 637-722): adds -int B^T sigma of this material's elements, from its
+  /// stress internal, to the model's internal force, in element order
+  void assembleInternalForces(GhostType ghost_type = _not_ghost);
 
   /// element filter of this material (ascending element ids)
   const Array<UInt>& getElementFilter() const { return filter_; }
@@ -75,6 +94,10 @@ class Material {
   std::map<std::string, RandomSpec> random_;
   Array<UInt> filter_;
   Array<Real> stress_, gradu_, damage_;
+  // model state each host field was taken at: gradu_ / stress_ are the
+  // internals of the last computeStress / computeAllStresses (or a download)
+  // while the model has not moved since
+  UInt gradu_version_ = ~0u, stress_version_ = ~0u;
 };
 
 /// MaterialFactory::registerAllocator / INSTANTIATE_MATERIAL (material.hh:59-60,687-691)

@@ -29,6 +29,7 @@
 #include "aka_array.hh"
 #include "aka_common.hh"
 #include "boundary_condition_functor.hh"
+#include "fe_engine.hh"
 #include "material.hh"
 #include "mesh.hh"
 
@@ -61,7 +62,8 @@ class SolidMechanicsModel {
   Array<Real>& getAcceleration();
   Array<Real>& getExternalForce();
   Array<Real>& getForce() { return getExternalForce(); }
-  const Array<Real>& getInternalForce();
+  /// writable as in the reference: Material::assembleInternalForces adds into it
+  Array<Real>& getInternalForce();
   Array<bool>& getBlockedDOFs();
   const Array<Real>& getMass();
   const Array<Real>& getCurrentPosition();
@@ -91,6 +93,9 @@ class SolidMechanicsModel {
   /// add a material in code (besides the input-file sections)
   Material& registerNewMaterial(const std::string& type, const std::string& name);
 
+  /// the model's FEEngine (SolidMechanicsModel::getFEEngine, model.hh)
+  FEEngine& getFEEngine(const ID& name = "");
+
   Mesh& getMesh() { return mesh_; }
   UInt getSpatialDimension() const { return dim_; }
 
@@ -113,9 +118,16 @@ class SolidMechanicsModel {
   void autotune(UInt nsteps = 10);
   void getLastStepTiming(Real& element_ms, Real& node_ms);
 
-  // used by Material
+  // used by Material and FEEngine
   akx_ctx* handle() { return ctx_; }
   void downloadPointField(int field, Array<Real>& out, const Material& m);
+  /// sends the host arrays a caller may have written, before a device operation
+  void synchronizeHost() { pushDirty(); }
+  /// changes whenever the displacement or the material state may have changed
+  UInt stateVersion() const { return state_version_; }
+  void internalForceChanged() { host_stale_[kFint] = true; }
+  UInt materialIndex(const Material& m) const;
+  bool isInitialized() const { return initialized_; }
 
  private:
   enum NodalId { kU, kV, kA, kFext, kFint, kBlocked, kMass, kPos, kInc, kPrev, kCount };
@@ -147,6 +159,8 @@ class SolidMechanicsModel {
   std::vector<DumpBuffer> dump_bufs_;  // one per dump field, reused
   std::future<void> dump_job_;         // the file writer of the last dump
   UInt steps_done_ = 0;
+  UInt state_version_ = 0;
+  std::unique_ptr<FEEngine> fe_engine_;
   bool host_stale_[kCount] = {};
   bool host_dirty_[kCount] = {};
 };

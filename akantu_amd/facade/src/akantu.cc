@@ -307,12 +307,22 @@ Real Material::getPushWaveSpeed() const {
   return std::sqrt((getLambda() + 2 * getMu()) / getParam("rho"));
 }
 
+UInt Material::getIndex() const { return model_.materialIndex(*this); }
+
 const Array<Real>& Material::getStress(ElementType) {
-  model_.downloadPointField(AKX_STRESS, stress_, *this);
+  model_.synchronizeHost();
+  if (stress_version_ != model_.stateVersion()) {
+    model_.downloadPointField(AKX_STRESS, stress_, *this);
+    stress_version_ = model_.stateVersion();
+  }
   return stress_;
 }
-const Array<Real>& Material::getGradU(ElementType) {
-  model_.downloadPointField(AKX_GRAD_U, gradu_, *this);
+Array<Real>& Material::getGradU(ElementType) {
+  model_.synchronizeHost();
+  if (gradu_version_ != model_.stateVersion()) {
+    model_.downloadPointField(AKX_GRAD_U, gradu_, *this);
+    gradu_version_ = model_.stateVersion();
+  }
   return gradu_;
 }
 const Array<Real>& Material::getDamage(ElementType) {
@@ -320,9 +330,51 @@ const Array<Real>& Material::getDamage(ElementType) {
   return damage_;
 }
 Real Material::getPotentialEnergy() {
-  if (model_.getNbMaterials() != 1)
-    AKANTU_EXCEPTION("per-material potential energy needs a single material");
-  return model_.getEnergy("potential");
+  if (!model_.isInitialized()) AKANTU_EXCEPTION("initFull first");
+  model_.synchronizeHost();
+  Real e = 0.;
+  int rc = akx_material_potential_energy(model_.handle(), int(getIndex()), &e);
+  if (rc) AKANTU_EXCEPTION("akx_material_potential_energy: " << akx_last_error(model_.handle()));
+  return e;
+}
+
+void Material::computeStress(ElementType type, GhostType ghost_type) {
+  if (ghost_type != _not_ghost) return;  // no ghost elements on one GPU
+  if (!model_.isInitialized()) AKANTU_EXCEPTION("initFull first");
+  const UInt dd = model_.getSpatialDimension() * model_.getSpatialDimension();
+  const UInt rows = filter_.size() * getNbIntegrationPoints(type);
+  if (gradu_.size() != rows || gradu_.getNbComponent() != dd) getGradU(type);
+  stress_.resize(rows, dd);
+  model_.synchronizeHost();
+  int rc = akx_material_compute_stress(model_.handle(), int(getIndex()), gradu_.storage(),
+                                       stress_.storage());
+  if (rc) AKANTU_EXCEPTION("akx_material_compute_stress: " << akx_last_error(model_.handle()));
+  stress_version_ = model_.stateVersion();
+}
+
+void Material::computeAllStresses(GhostType ghost_type) {
+  if (ghost_type != _not_ghost) return;
+  if (!model_.isInitialized()) AKANTU_EXCEPTION("initFull first");
+  const UInt d = model_.getSpatialDimension();
+  const UInt rows = filter_.size() * getNbIntegrationPoints(model_.getMesh().getElementType());
+  gradu_.resize(rows, d * d);
+  stress_.resize(rows, d * d);
+  model_.synchronizeHost();
+  int rc = akx_material_compute_all_stresses(model_.handle(), int(getIndex()), gradu_.storage(),
+                                             stress_.storage());
+  if (rc)
+    AKANTU_EXCEPTION("akx_material_compute_all_stresses: " << akx_last_error(model_.handle()));
+  gradu_version_ = stress_version_ = model_.stateVersion();
+}
+
+void Material::assembleInternalForces(GhostType ghost_type) {
+  if (ghost_type != _not_ghost) return;
+  if (!model_.isInitialized()) AKANTU_EXCEPTION("initFull first");
+  model_.synchronizeHost();
+  int rc = akx_material_assemble_internal_forces(model_.handle(), int(getIndex()));
+  if (rc)
+    AKANTU_EXCEPTION("akx_material_assemble_internal_forces: " << akx_last_error(model_.handle()));
+  model_.internalForceChanged();
 }
 Real Material::getEnergy(const std::string& id) {
   if (id == "potential") return getPotentialEnergy();
@@ -491,6 +543,7 @@ void SolidMechanicsModel::initFull(const AnalysisMethodArg& method) {
   check(akx_assign_materials(ctx_, em.data()), "akx_assign_materials");
   check(akx_init(ctx_), "akx_init");
   initialized_ = true;
+  ++state_version_;
   for (bool& s : host_stale_) s = true;
   host_stale_[kFext] = host_stale_[kBlocked] = false;
   for (bool& d : host_dirty_) d = false;
@@ -519,11 +572,12 @@ void SolidMechanicsModel::pull(NodalId id) {
 
 void SolidMechanicsModel::pushDirty() {
   const size_t nd = size_t(mesh_.getNbNodes()) * dim_;
-  Array<Real>* arr[] = {&u_, &v_, &a_, &fext_};
-  for (int id : {kU, kV, kA, kFext}) {
+  Array<Real>* arr[] = {&u_, &v_, &a_, &fext_, &fint_};
+  for (int id : {kU, kV, kA, kFext, kFint}) {
     if (!host_dirty_[id]) continue;
     check(akx_field_upload(ctx_, field_of(id), arr[id]->storage(), nd * 8), "akx_field_upload");
     host_dirty_[id] = false;
+    if (id == kU) ++state_version_;
   }
   if (host_dirty_[kBlocked]) {
     std::vector<uint8_t> b(nd);
@@ -548,7 +602,11 @@ Array<bool>& SolidMechanicsModel::getBlockedDOFs() {
   host_dirty_[kBlocked] = true;
   return blocked_;
 }
-const Array<Real>& SolidMechanicsModel::getInternalForce() { pull(kFint); return fint_; }
+Array<Real>& SolidMechanicsModel::getInternalForce() {
+  pull(kFint);
+  host_dirty_[kFint] = true;
+  return fint_;
+}
 const Array<Real>& SolidMechanicsModel::getMass() {
   if (!initialized_) AKANTU_EXCEPTION("initFull first");
   pull(kMass);
@@ -583,6 +641,7 @@ void SolidMechanicsModel::solveStep(UInt nsteps) {
   pushDirty();
   check(akx_solve_step(ctx_, int(nsteps), int(precision_)), "akx_solve_step");
   steps_done_ += nsteps;
+  ++state_version_;
   for (int id : {kU, kV, kA, kFint, kPos, kInc, kPrev}) host_stale_[id] = true;
 }
 
@@ -591,6 +650,7 @@ void SolidMechanicsModel::assembleInternalForces() {
   pushDirty();
   check(akx_assemble_internal_forces(ctx_, int(precision_)), "akx_assemble_internal_forces");
   host_stale_[kFint] = true;
+  ++state_version_;
 }
 
 Real SolidMechanicsModel::getEnergy(const std::string& id) {
@@ -644,6 +704,17 @@ void SolidMechanicsModel::setDirichletIncrement(const std::string& group, Spatia
   check(akx_set_dirichlet_increment(ctx_, int64_t(nodes.size()), nodes.data(), int(axis), increment),
         "akx_set_dirichlet_increment");
   host_stale_[kBlocked] = true;
+}
+
+UInt SolidMechanicsModel::materialIndex(const Material& m) const {
+  for (size_t k = 0; k < materials_.size(); ++k)
+    if (materials_[k].get() == &m) return UInt(k);
+  AKANTU_EXCEPTION("material " << m.getName() << " does not belong to this model");
+}
+
+FEEngine& SolidMechanicsModel::getFEEngine(const ID&) {
+  if (!fe_engine_) fe_engine_.reset(new FEEngine(*this));
+  return *fe_engine_;
 }
 
 Material& SolidMechanicsModel::getMaterial(UInt i) {
@@ -865,5 +936,160 @@ void SolidMechanicsModel::downloadPointField(int field, Array<Real>& out, const 
     std::memcpy(out.storage() + size_t(k) * Q * comp, all.data() + size_t(f(k)) * Q * comp,
                 sizeof(Real) * Q * comp);
 }
+
+/* ------------------------------------------------------------------------ */
+/* FEEngine (fe_engine.hh:75-251) over the akx_fe_* operators                */
+/* ------------------------------------------------------------------------ */
+const Array<UInt> empty_filter;
+
+namespace {
+const uint32_t* filter_ptr(const Array<UInt>& f) {
+  static_assert(sizeof(UInt) == sizeof(uint32_t), "element ids are 32-bit");
+  return &f == &empty_filter ? nullptr : reinterpret_cast<const uint32_t*>(f.storage());
+}
+int64_t filter_size(const Array<UInt>& f) { return &f == &empty_filter ? 0 : int64_t(f.size()); }
+}  // namespace
+
+void FEEngine::checkType(const ElementType& type, const GhostType& ghost_type) const {
+  if (!model_.isInitialized()) AKANTU_EXCEPTION("FEEngine: initFull the model first");
+  if (type != model_.getMesh().getElementType())
+    AKANTU_EXCEPTION("FEEngine: the mesh has no elements of type " << int(type));
+  if (ghost_type != _not_ghost) AKANTU_EXCEPTION("FEEngine: no ghost elements on one GPU");
+}
+
+void FEEngine::initShapeFunctions(const GhostType& ghost_type) {
+  checkType(model_.getMesh().getElementType(), ghost_type);
+}
+
+UInt FEEngine::getNbIntegrationPoints(const ElementType& type, const GhostType&) const {
+  return akantu::getNbIntegrationPoints(type);
+}
+
+const Array<Real>& FEEngine::getShapesDerivatives(const ElementType& type,
+                                                  const GhostType& ghost_type, UInt) {
+  checkType(type, ghost_type);
+  const UInt rows = model_.getMesh().getNbElement() * getNbIntegrationPoints(type);
+  const UInt comp = getSpatialDimension(type) * getNbNodesPerElement(type);
+  shapes_derivatives_.resize(rows, comp);
+  int rc = akx_field_download(model_.handle(), AKX_SHAPES_DERIVATIVES, shapes_derivatives_.storage(),
+                              size_t(rows) * comp * 8);
+  if (rc) AKANTU_EXCEPTION("akx_field_download: " << akx_last_error(model_.handle()));
+  return shapes_derivatives_;
+}
+
+const Array<Real>& FEEngine::getShapes(const ElementType& type, const GhostType& ghost_type,
+                                       UInt) {
+  checkType(type, ghost_type);
+  // N_a at the points: the interpolation of each node's indicator field
+  const UInt E = model_.getMesh().getNbElement(), Q = getNbIntegrationPoints(type);
+  const UInt n = getNbNodesPerElement(type), N = model_.getMesh().getNbNodes();
+  const auto& conn = model_.getMesh().getConnectivity(type);
+  shapes_.resize(E * Q, n);
+  Array<Real> ind(N, 1), nq;
+  Array<UInt> one(1, 1);
+  for (UInt e = 0; e < E; ++e) {
+    one(0) = e;
+    for (UInt a = 0; a < n; ++a) {
+      ind.clear();
+      ind(conn(e, a)) = 1.;
+      interpolateOnIntegrationPoints(ind, nq, 1, type, ghost_type, one);
+      for (UInt q = 0; q < Q; ++q) shapes_(e * Q + q, a) = nq(q);
+    }
+  }
+  return shapes_;
+}
+
+#define AKX_FE_CHECK(call)                                                           \
+  do {                                                                               \
+    int _rc = (call);                                                                \
+    if (_rc) AKANTU_EXCEPTION(#call << ": " << akx_last_error(model_.handle()));    \
+  } while (0)
+
+void FEEngine::gradientOnIntegrationPoints(const Array<Real>& u, Array<Real>& nablauq,
+                                           UInt nb_dof, const ElementType& type,
+                                           const GhostType& ghost_type,
+                                           const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  const UInt ne = filter_size(filter) ? filter.size() : model_.getMesh().getNbElement();
+  nablauq.resize(ne * getNbIntegrationPoints(type), nb_dof * getSpatialDimension(type));
+  AKX_FE_CHECK(akx_fe_gradient_on_integration_points(model_.handle(), u.storage(), int(nb_dof),
+                                                     filter_ptr(filter), filter_size(filter),
+                                                     nablauq.storage()));
+}
+
+void FEEngine::interpolateOnIntegrationPoints(const Array<Real>& u, Array<Real>& uq, UInt nb_dof,
+                                              const ElementType& type,
+                                              const GhostType& ghost_type,
+                                              const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  const UInt ne = filter_size(filter) ? filter.size() : model_.getMesh().getNbElement();
+  uq.resize(ne * getNbIntegrationPoints(type), nb_dof);
+  AKX_FE_CHECK(akx_fe_interpolate_on_integration_points(model_.handle(), u.storage(), int(nb_dof),
+                                                        filter_ptr(filter), filter_size(filter),
+                                                        uq.storage()));
+}
+
+void FEEngine::computeBtD(const Array<Real>& Ds, Array<Real>& BtDs, const ElementType& type,
+                          const GhostType& ghost_type, const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  const UInt d = getSpatialDimension(type), rows = Ds.getNbComponent() / d;
+  BtDs.resize(Ds.size(), rows * getNbNodesPerElement(type));
+  AKX_FE_CHECK(akx_fe_compute_btd(model_.handle(), Ds.storage(), int(rows), filter_ptr(filter),
+                                  filter_size(filter), BtDs.storage()));
+}
+
+void FEEngine::computeNtb(const Array<Real>& bs, Array<Real>& Ntbs, const ElementType& type,
+                          const GhostType& ghost_type, const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  Ntbs.resize(bs.size(), bs.getNbComponent() * getNbNodesPerElement(type));
+  AKX_FE_CHECK(akx_fe_compute_ntb(model_.handle(), bs.storage(), int(bs.getNbComponent()),
+                                  filter_ptr(filter), filter_size(filter), Ntbs.storage()));
+}
+
+void FEEngine::integrate(const Array<Real>& f, Array<Real>& intf, UInt nb_dof,
+                         const ElementType& type, const GhostType& ghost_type,
+                         const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  intf.resize(f.size() / getNbIntegrationPoints(type), nb_dof);
+  AKX_FE_CHECK(akx_fe_integrate(model_.handle(), f.storage(), int(nb_dof), filter_ptr(filter),
+                                filter_size(filter), intf.storage()));
+}
+
+Real FEEngine::integrate(const Array<Real>& f, const ElementType& type,
+                         const GhostType& ghost_type, const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  Real v = 0.;
+  AKX_FE_CHECK(akx_fe_integrate_scalar(model_.handle(), f.storage(), filter_ptr(filter),
+                                       filter_size(filter), &v));
+  return v;
+}
+
+void FEEngine::integrateOnIntegrationPoints(const Array<Real>& f, Array<Real>& intf, UInt nb_dof,
+                                            const ElementType& type, const GhostType& ghost_type,
+                                            const Array<UInt>& filter) const {
+  checkType(type, ghost_type);
+  intf.resize(f.size(), nb_dof);
+  AKX_FE_CHECK(akx_fe_integrate_on_integration_points(model_.handle(), f.storage(), int(nb_dof),
+                                                      filter_ptr(filter), filter_size(filter),
+                                                      intf.storage()));
+}
+
+void FEEngine::assembleFieldLumped(
+    const std::function<void(Matrix<Real>&, const Element&)>& field_funct, UInt nb_dof,
+    Array<Real>& lumped, ElementType type, const GhostType& ghost_type) const {
+  checkType(type, ghost_type);
+  const UInt E = model_.getMesh().getNbElement(), Q = getNbIntegrationPoints(type);
+  // fillField (fe_engine_tmpl.hh): one nb_dof x Q matrix per element
+  Array<Real> field(E * Q, nb_dof);
+  for (UInt e = 0; e < E; ++e) {
+    Matrix<Real> m(field.storage() + size_t(e) * Q * nb_dof, nb_dof, Q);
+    field_funct(m, Element{type, e, ghost_type});
+  }
+  if (lumped.size() != model_.getMesh().getNbNodes() || lumped.getNbComponent() != nb_dof)
+    lumped.resize(model_.getMesh().getNbNodes(), nb_dof);
+  AKX_FE_CHECK(akx_fe_assemble_field_lumped(model_.handle(), field.storage(), int(nb_dof),
+                                            lumped.storage()));
+}
+#undef AKX_FE_CHECK
 
 }  // namespace akantu

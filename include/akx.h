@@ -68,7 +68,7 @@ enum akx_field {
   AKX_VELOCITY = 1,              /* (N,d) f64 */
   AKX_ACCELERATION = 2,          /* (N,d) f64 */
   AKX_EXTERNAL_FORCE = 3,        /* (N,d) f64 */
-  AKX_INTERNAL_FORCE = 4,        /* (N,d) f64, of the last step / assembly */
+  AKX_INTERNAL_FORCE = 4,        /* (N,d) f64, of the last step / assembly (writable) */
   AKX_MASS = 5,                  /* (N,d) f64, lumped (download only) */
   AKX_BLOCKED_DOFS = 6,          /* (N,d) u8 */
   AKX_PREVIOUS_DISPLACEMENT = 7, /* (N,d) f64, u before the last step's predictor */
@@ -305,6 +305,85 @@ int akx_surrogate_dataset(akx_ctx* ctx, int64_t first, int64_t count, float* x, 
  * the plain law and are counted (akx_surrogate_fallbacks). */
 int akx_surrogate_enable(akx_ctx* ctx, int on);
 int akx_surrogate_fallbacks(akx_ctx* ctx, uint64_t* count, int reset);
+
+/* ---------------------------------------------------------------------------
+ * FEEngine operators on a caller's field (FEEngine, src/fe_engine/fe_engine.hh;
+ * FEEngineTemplate<IntegratorGauss, ShapeLagrange> for the element type of the
+ * mesh), FP64 in the reference's operation order (bit-identical results).
+ * `elements` lists caller element ids (the reference's filter_elements),
+ * nullptr / 0 = every element in order; per-element and per-point arrays
+ * follow that list (row = listed element * Q + q).  Nodal arrays are
+ * (N, nb_dof) in the caller's node numbering.  Synchronous.
+ * ------------------------------------------------------------------------- */
+/* FEEngine::gradientOnIntegrationPoints (fe_engine.hh:149-153;
+ * shape_lagrange_inline_impl.hh:353-379, shape_functions_inline_impl.hh:352-395):
+ * u (N, nb_dof) -> nablauq (n_el*Q, nb_dof*d), per point nb_dof x d
+ * column-major, [j*nb_dof + i] = du_i/dx_j.  1 <= nb_dof <= 9. */
+int akx_fe_gradient_on_integration_points(akx_ctx* ctx, const double* u, int nb_dof,
+                                          const uint32_t* elements, int64_t nb_elements,
+                                          double* nablauq);
+/* FEEngine::interpolateOnIntegrationPoints (fe_engine.hh:157-161;
+ * shape_functions_inline_impl.hh:300-335): u (N, nb_dof) -> uq (n_el*Q, nb_dof). */
+int akx_fe_interpolate_on_integration_points(akx_ctx* ctx, const double* u, int nb_dof,
+                                             const uint32_t* elements, int64_t nb_elements,
+                                             double* uq);
+/* FEEngine::computeBtD (fe_engine.hh:170-175; shape_lagrange_inline_impl.hh:382-423):
+ * Ds (n_el*Q, rows*d), per point rows x d column-major -> BtDs (n_el*Q, rows*n),
+ * per point D * B, rows x n column-major ([a*rows + i]).  1 <= rows <= 9. */
+int akx_fe_compute_btd(akx_ctx* ctx, const double* Ds, int rows, const uint32_t* elements,
+                       int64_t nb_elements, double* BtDs);
+/* FEEngine::computeNtb (fe_engine.hh:185-189; shape_lagrange_inline_impl.hh:495-533):
+ * bs (n_el*Q, nb_dof) -> Ntbs (n_el*Q, nb_dof*n), [a*nb_dof + i] = b_i N_a. */
+int akx_fe_compute_ntb(akx_ctx* ctx, const double* bs, int nb_dof, const uint32_t* elements,
+                       int64_t nb_elements, double* Ntbs);
+/* FEEngine::integrate (fe_engine.hh:95-100; integrator_gauss_inline_impl.hh:426-483):
+ * f (n_el*Q, nb_dof) -> intf (n_el, nb_dof) = sum_q f det(J) w.  nb_dof <= 192. */
+int akx_fe_integrate(akx_ctx* ctx, const double* f, int nb_dof, const uint32_t* elements,
+                     int64_t nb_elements, double* intf);
+/* FEEngine::integrate of a scalar field (fe_engine.hh:103-106;
+ * integrator_gauss_inline_impl.hh:525-541): the per-element integrals reduced
+ * with Math::reduce (aka_math.cc:225-250: sorted, then summed pairwise). */
+int akx_fe_integrate_scalar(akx_ctx* ctx, const double* f, const uint32_t* elements,
+                            int64_t nb_elements, double* value);
+/* FEEngine::integrateOnIntegrationPoints (fe_engine.hh:110-114;
+ * integrator_gauss_inline_impl.hh:545-597): out = f det(J) w per point. */
+int akx_fe_integrate_on_integration_points(akx_ctx* ctx, const double* f, int nb_dof,
+                                           const uint32_t* elements, int64_t nb_elements,
+                                           double* out);
+/* FEEngine::assembleFieldLumped, row sum (fe_engine.hh:240-244;
+ * fe_engine_template_tmpl_field.hh:126-199, DOFManager element-order assembly,
+ * dof_manager.cc:83-137): field (E*Q, nb_dof) for every element ->
+ * lumped (N, nb_dof) += sum over elements of integral N_a field.  nb_dof <= 24. */
+int akx_fe_assemble_field_lumped(akx_ctx* ctx, const double* field, int nb_dof, double* lumped);
+
+/* ---------------------------------------------------------------------------
+ * Material operators on the elements of one material (Material,
+ * src/model/solid_mechanics/material.hh), FP64, reference order.  A
+ * material's points are its elements in ascending caller id (its element
+ * filter), Q points each.  The material's gradient, stress and damage
+ * internals live on the device (akx_field_download of AKX_DAMAGE reads the
+ * damage).
+ * ------------------------------------------------------------------------- */
+/* Material::getElementFilter: number of elements (and, when ids is not null,
+ * their caller ids, ascending). */
+int akx_material_elements(akx_ctx* ctx, int mat_id, int64_t* count, uint32_t* ids);
+/* Material::computeStress (material.hh:110-113): stress of the material's
+ * points from the given gradients (n_mat*Q, d*d) -> stress (same shape; may be
+ * null).  Updates the damage history of those points (MaterialMarigo) and
+ * keeps gradient and stress as the material's internals. */
+int akx_material_compute_stress(akx_ctx* ctx, int mat_id, const double* grad_u, double* stress);
+/* Material::computeAllStresses (material.hh:195; material.cc:254-279): gradient
+ * of the current displacement on the material's points, then computeStress.
+ * grad_u / stress (n_mat*Q, d*d, optional) receive the result. */
+int akx_material_compute_all_stresses(akx_ctx* ctx, int mat_id, double* grad_u, double* stress);
+/* Material::assembleInternalForces (material.hh:186; material.cc:178-246,
+ * 637-722): -integral B^T sigma (or the finite-strain residual) from the
+ * material's internals, added to the context's internal force
+ * (AKX_INTERNAL_FORCE) node by node in element order. */
+int akx_material_assemble_internal_forces(akx_ctx* ctx, int mat_id);
+/* Material::getPotentialEnergy (material.cc:743-757) of one material, on the
+ * current displacement. */
+int akx_material_potential_energy(akx_ctx* ctx, int mat_id, double* value);
 
 #ifdef __cplusplus
 }

@@ -964,4 +964,314 @@ int orc_truss_dynamic_explicit(char* out, int64_t cap) {
   return int(s.size());
 }
 
+/* ------------------------------------------------------------------------ */
+/* FEEngine operators on a caller's field (fe_engine.hh:95-189,240-244).     */
+/* filter: element ids (nullptr = every element, in order); per-point        */
+/* inputs and outputs are in filter order.                                   */
+/* ------------------------------------------------------------------------ */
+namespace {
+size_t filtered(const Model& M, const uint32_t* filter, int64_t nf, int64_t t) {
+  return filter ? size_t(filter[t]) : size_t(t);
+}
+int64_t filter_count(const Model& M, const uint32_t* filter, int64_t nf) {
+  return filter ? nf : int64_t(M.E);
+}
+
+// Math::reduce (aka_math.cc:225-250): sort ascending, then pairwise sums.
+double math_reduce(std::vector<double> a) {
+  size_t nb_values = a.size();
+  if (nb_values == 0) return 0.;
+  size_t half = nb_values >> 1;
+  std::sort(a.begin(), a.end());
+  while (half) {
+    size_t remaining = nb_values - 2 * half;
+    if (remaining) a[nb_values - 2] += a[nb_values - 1];
+    for (size_t i = 0; i < half; ++i) a[i] = a[2 * i] + a[2 * i + 1];
+    nb_values = half;
+    half >>= 1;
+  }
+  return a[0];
+}
+}  // namespace
+
+// ShapeLagrange::gradientOnIntegrationPoints (shape_lagrange_inline_impl.hh:353-379):
+// extractNodalToElementField (fe_engine_inline_impl.hh:104-143) then, per point,
+// nabla_u = u_e * B^T (shape_functions_inline_impl.hh:352-395): out (nf*Q, nb_dof*d).
+int orc_fe_gradient(void* h, const double* u, int nb_dof, const uint32_t* filter, int64_t nf,
+                    double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int d = M->d, n = M->n, Q = M->nq, dn = d * n;
+  if (nb_dof < 1 || nb_dof > 9) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  double ue[9 * 8];
+  for (int64_t t = 0; t < cnt; ++t) {
+    const size_t e = filtered(*M, filter, nf, t);
+    for (int a = 0; a < n; ++a)
+      for (int i = 0; i < nb_dof; ++i)
+        ue[i + a * nb_dof] = u[size_t(M->conn[e * n + a]) * nb_dof + i];
+    for (int q = 0; q < Q; ++q)
+      matrix_matrixt(nb_dof, d, n, ue, &M->B[(e * Q + q) * dn],
+                     out + (size_t(t) * Q + q) * nb_dof * d);
+  }
+  return 0;
+}
+
+// ShapeFunctions::interpolateElementalFieldOnIntegrationPoints
+// (shape_functions_inline_impl.hh:300-335): uq = u_e * N, out (nf*Q, nb_dof).
+int orc_fe_interpolate(void* h, const double* u, int nb_dof, const uint32_t* filter, int64_t nf,
+                       double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int n = M->n, Q = M->nq;
+  if (nb_dof < 1 || nb_dof > 9) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  double ue[9 * 8], Nm[8 * 8], uq[9 * 8];
+  for (int q = 0; q < Q; ++q)
+    for (int a = 0; a < n; ++a) Nm[a + q * n] = M->Nq[size_t(q) * n + a];
+  for (int64_t t = 0; t < cnt; ++t) {
+    const size_t e = filtered(*M, filter, nf, t);
+    for (int a = 0; a < n; ++a)
+      for (int i = 0; i < nb_dof; ++i)
+        ue[i + a * nb_dof] = u[size_t(M->conn[e * n + a]) * nb_dof + i];
+    matrix_matrix(nb_dof, Q, n, ue, Nm, uq);
+    std::memcpy(out + size_t(t) * Q * nb_dof, uq, sizeof(double) * Q * nb_dof);
+  }
+  return 0;
+}
+
+// ShapeLagrange::computeBtD (shape_lagrange_inline_impl.hh:382-423):
+// Bt_D = D (rows x d) * B (d x n), out (nf*Q, rows*n).
+int orc_fe_btd(void* h, const double* D, int rows, const uint32_t* filter, int64_t nf,
+               double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int d = M->d, n = M->n, Q = M->nq, dn = d * n;
+  if (rows < 1 || rows > 9) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  for (int64_t t = 0; t < cnt; ++t) {
+    const size_t e = filtered(*M, filter, nf, t);
+    for (int q = 0; q < Q; ++q) {
+      const size_t r = size_t(t) * Q + q;
+      matrix_matrix(rows, n, d, D + r * rows * d, &M->B[(e * Q + q) * dn], out + r * rows * n);
+    }
+  }
+  return 0;
+}
+
+// ShapeLagrange::computeNtb (shape_lagrange_inline_impl.hh:495-533):
+// Ntb = b (nb_dof x 1) * N (1 x n), out (nf*Q, nb_dof*n).
+int orc_fe_ntb(void* h, const double* b, int nb_dof, const uint32_t* filter, int64_t nf,
+               double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int n = M->n, Q = M->nq;
+  if (nb_dof < 1 || nb_dof > 24) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  for (int64_t t = 0; t < cnt; ++t)
+    for (int q = 0; q < Q; ++q) {
+      const size_t r = size_t(t) * Q + q;
+      matrix_matrix(nb_dof, n, 1, b + r * nb_dof, &M->Nq[size_t(q) * n], out + r * nb_dof * n);
+    }
+  return 0;
+}
+
+// IntegratorGauss::integrate (integrator_gauss_inline_impl.hh:426-483):
+// intf = f (nb_dof x Q) * J (Q x 1), out (nf, nb_dof).
+int orc_fe_integrate(void* h, const double* f, int nb_dof, const uint32_t* filter, int64_t nf,
+                     double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int Q = M->nq;
+  if (nb_dof < 1 || nb_dof > 192) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  double Jq[8];
+  for (int64_t t = 0; t < cnt; ++t) {
+    const size_t e = filtered(*M, filter, nf, t);
+    for (int q = 0; q < Q; ++q) Jq[q] = M->JxW[e * Q + q];
+    matrix_matrix(nb_dof, 1, Q, f + size_t(t) * Q * nb_dof, Jq, out + size_t(t) * nb_dof);
+  }
+  return 0;
+}
+
+// IntegratorGauss::integrateOnIntegrationPoints (integrator_gauss_inline_impl.hh:545-597):
+// out = f * J per point.
+int orc_fe_integrate_on_points(void* h, const double* f, int nb_dof, const uint32_t* filter,
+                               int64_t nf, double* out) {
+  auto* M = static_cast<Model*>(h);
+  const int Q = M->nq;
+  if (nb_dof < 1 || nb_dof > 192) return -1;
+  const int64_t cnt = filter_count(*M, filter, nf);
+  for (int64_t t = 0; t < cnt; ++t) {
+    const size_t e = filtered(*M, filter, nf, t);
+    for (int q = 0; q < Q; ++q) {
+      const size_t r = size_t(t) * Q + q;
+      for (int k = 0; k < nb_dof; ++k) {
+        double v = f[r * nb_dof + k];
+        v *= M->JxW[e * Q + q];
+        out[r * nb_dof + k] = v;
+      }
+    }
+  }
+  return 0;
+}
+
+// IntegratorGauss::integrate of a scalar (integrator_gauss_inline_impl.hh:525-541):
+// per-element integrals reduced by Math::reduce.
+double orc_fe_integrate_scalar(void* h, const double* f, const uint32_t* filter, int64_t nf) {
+  auto* M = static_cast<Model*>(h);
+  const int64_t cnt = filter_count(*M, filter, nf);
+  std::vector<double> per(size_t(cnt), 0.);
+  orc_fe_integrate(h, f, 1, filter, nf, per.data());
+  return math_reduce(per);
+}
+
+// FEEngineTemplate::assembleFieldLumped -> assembleLumpedRowSum
+// (fe_engine_template_tmpl_field.hh:126-199): computeNtb, integrate, then
+// DOFManager::assembleElementalArrayToLumpedMatrix: element-order scatter into a
+// zeroed local array (dof_manager.cc:83-137), added once into the lumped vector
+// (dof_manager_default.cc:227-251).  field (E*Q, nb_dof) -> lumped (N, nb_dof),
+// which is added to (not overwritten).
+int orc_fe_assemble_field_lumped(void* h, const double* field, int nb_dof, double* lumped) {
+  auto* M = static_cast<Model*>(h);
+  const int n = M->n, Q = M->nq;
+  if (nb_dof < 1 || nb_dof > 24) return -1;
+  const size_t E = M->E;
+  std::vector<double> ntb(E * Q * nb_dof * n), integ(E * nb_dof * n);
+  orc_fe_ntb(h, field, nb_dof, nullptr, 0, ntb.data());
+  orc_fe_integrate(h, ntb.data(), nb_dof * n, nullptr, 0, integ.data());
+  std::vector<double> local(M->N * nb_dof, 0.);
+  for (size_t e = 0; e < E; ++e)
+    for (int a = 0; a < n; ++a)
+      for (int k = 0; k < nb_dof; ++k)
+        local[size_t(M->conn[e * n + a]) * nb_dof + k] += 1. * integ[e * nb_dof * n + a * nb_dof + k];
+  for (size_t k = 0; k < local.size(); ++k) lumped[k] += 1. * local[k];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Material operators (material.hh:110-195) on one material's elements.     */
+/* ------------------------------------------------------------------------ */
+namespace {
+void point_law(Model& M, const Material& mat, size_t eq, const double* G, double* S) {
+  const int d = M.d;
+  if (mat.kind == ELASTIC) elastic_stress(d, mat, G, S);
+  else if (mat.kind == NEOHOOKEAN) neohookean_stress(d, mat, G, S);
+  else neohookean_damage_stress(d, mat, G, S, M.damage[eq], M.Yd[eq]);
+}
+}  // namespace
+
+// Material::computeStress (material.hh:110-113; material_elastic.cc:110-145,
+// material_neohookean.cc:126-160, Marigo update): from a given gradient field of
+// the material's points (in its element-filter order), the stress; the damage
+// history of those points is updated, and both fields are kept as the
+// material's internals.
+int orc_material_compute_stress(void* h, int mat, const double* gradu, double* stress) {
+  auto* M = static_cast<Model*>(h);
+  if (mat < 0 || mat >= int(M->mats.size())) return -1;
+  Material& m = M->mats[mat];
+  const int d = M->d, Q = M->nq, dd = d * d;
+  size_t t = 0;
+  for (uint32_t e : m.filter)
+    for (int q = 0; q < Q; ++q, ++t) {
+      const size_t eq = size_t(e) * Q + q;
+      std::memcpy(&M->gradu[eq * dd], gradu + t * dd, sizeof(double) * dd);
+      point_law(*M, m, eq, &M->gradu[eq * dd], &M->stress[eq * dd]);
+      std::memcpy(stress + t * dd, &M->stress[eq * dd], sizeof(double) * dd);
+    }
+  return 0;
+}
+
+// Material::computeAllStresses (material.cc:254-279): gradient of the current
+// displacement on the material's points, minus the (zero) eigengradient, then
+// computeStress.
+int orc_material_compute_all_stresses(void* h, int mat) {
+  auto* M = static_cast<Model*>(h);
+  if (mat < 0 || mat >= int(M->mats.size())) return -1;
+  Material& m = M->mats[mat];
+  const int d = M->d, n = M->n, Q = M->nq, dn = d * n, dd = d * d;
+  double ue[24];
+  for (uint32_t e : m.filter) {
+    for (int a = 0; a < n; ++a)
+      for (int i = 0; i < d; ++i) ue[i + a * d] = M->u[size_t(M->conn[size_t(e) * n + a]) * d + i];
+    for (int q = 0; q < Q; ++q) {
+      const size_t eq = size_t(e) * Q + q;
+      double* G = &M->gradu[eq * dd];
+      matrix_matrixt(d, d, n, ue, &M->B[eq * dn], G);
+      for (int k = 0; k < dd; ++k) G[k] -= 0.;
+      point_law(*M, m, eq, G, &M->stress[eq * dd]);
+    }
+  }
+  return 0;
+}
+
+// Material::assembleInternalForces (material.cc:178-246 small strain,
+// :637-722 finite strain): B^T sigma (or the finite-strain residual) from the
+// material's stored gradient and stress, integrated, and scattered with scale
+// -1 into the model's internal force in element order (added to what is there).
+int orc_material_assemble_internal_forces(void* h, int mat) {
+  auto* M = static_cast<Model*>(h);
+  if (mat < 0 || mat >= int(M->mats.size())) return -1;
+  const Material& m = M->mats[mat];
+  const int d = M->d, n = M->n, Q = M->nq, dn = d * n, dd = d * d;
+  double Rq[8][24], fe[24];
+  for (uint32_t e : m.filter) {
+    for (int q = 0; q < Q; ++q) {
+      const size_t eq = size_t(e) * Q + q;
+      const double* Bq = &M->B[eq * dn];
+      if (m.kind == ELASTIC) matrix_matrix(d, n, d, &M->stress[eq * dd], Bq, Rq[q]);
+      else finite_strain_residual(d, n, Bq, &M->gradu[eq * dd], &M->stress[eq * dd], Rq[q]);
+    }
+    for (int k = 0; k < dn; ++k) {
+      double s = 0.;
+      for (int q = 0; q < Q; ++q) s += Rq[q][k] * M->JxW[size_t(e) * Q + q];
+      fe[k] = s * 1.;
+    }
+    for (int a = 0; a < n; ++a) {
+      double* dst = &M->fint[size_t(M->conn[size_t(e) * n + a]) * d];
+      for (int i = 0; i < d; ++i) dst[i] += -1. * fe[i + a * d];
+    }
+  }
+  return 0;
+}
+
+// Material::getPotentialEnergy (material.cc:743-757) of one material, from
+// the current displacement (as orc_energy kind 1 computes it per material).
+double orc_material_potential_energy(void* h, int mat) {
+  auto* M = static_cast<Model*>(h);
+  if (mat < 0 || mat >= int(M->mats.size())) return 0.;
+  Material& m = M->mats[mat];
+  const int d = M->d, Q = M->nq;
+  double fe[24];
+  std::vector<double> dam_save;
+  if (m.kind == NEOHOOKEAN_DAMAGE) dam_save = M->damage;
+  double part = 0.;
+  for (uint32_t e : m.filter) {
+    element_force(*M, m, e, fe);
+    double s_e = 0.;
+    for (int q = 0; q < Q; ++q) {
+      size_t eq = size_t(e) * Q + q;
+      const double* G = &M->gradu[eq * d * d];
+      const double* S = &M->stress[eq * d * d];
+      double w;
+      if (m.kind == ELASTIC) {
+        double dd = 0.;
+        for (int k = 0; k < d * d; ++k) dd += S[k] * G[k];
+        w = .5 * dd;
+      } else {
+        w = neohookean_energy(d, m, G);
+        if (m.kind == NEOHOOKEAN_DAMAGE) w *= (1 - M->damage[eq]);
+      }
+      s_e += w * M->JxW[eq];
+    }
+    part += s_e;
+  }
+  if (m.kind == NEOHOOKEAN_DAMAGE) M->damage = dam_save;
+  return part;
+}
+
+// element filter of a material (ascending element ids)
+int64_t orc_material_elements(void* h, int mat, uint32_t* ids) {
+  auto* M = static_cast<Model*>(h);
+  if (mat < 0 || mat >= int(M->mats.size())) return -1;
+  const auto& f = M->mats[mat].filter;
+  if (ids) std::copy(f.begin(), f.end(), ids);
+  return int64_t(f.size());
+}
+
 } // extern "C"

@@ -59,6 +59,19 @@ def configure(L):
     L.orc_point_energy.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _P]
     L.orc_point_finite_residual.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
     L.orc_truss_dynamic_explicit.argtypes = [ctypes.c_char_p, _I64]
+    for name in ("orc_fe_gradient", "orc_fe_interpolate", "orc_fe_btd", "orc_fe_ntb",
+                 "orc_fe_integrate", "orc_fe_integrate_on_points"):
+        getattr(L, name).argtypes = [_P, _P, ctypes.c_int, _P, _I64, _P]
+    L.orc_fe_integrate_scalar.restype = _D
+    L.orc_fe_integrate_scalar.argtypes = [_P, _P, _P, _I64]
+    L.orc_fe_assemble_field_lumped.argtypes = [_P, _P, ctypes.c_int, _P]
+    L.orc_material_compute_stress.argtypes = [_P, ctypes.c_int, _P, _P]
+    L.orc_material_compute_all_stresses.argtypes = [_P, ctypes.c_int]
+    L.orc_material_assemble_internal_forces.argtypes = [_P, ctypes.c_int]
+    L.orc_material_potential_energy.restype = _D
+    L.orc_material_potential_energy.argtypes = [_P, ctypes.c_int]
+    L.orc_material_elements.restype = _I64
+    L.orc_material_elements.argtypes = [_P, ctypes.c_int, _P]
     return L
 
 
@@ -90,6 +103,9 @@ class OracleModel:
         self._conn = np.ascontiguousarray(mesh.conn, dtype=np.uint32)
         self.dim = mesh.dim
         self.nb_nodes = mesh.nb_nodes
+        self.nb_elements = mesh.nb_elements
+        self.nb_nodes_per_element = mesh.conn.shape[1]
+        self.nb_qp = mesh.nb_qp
         self.h = L.orc_create(mesh.elem_type, mesh.dim, mesh.nb_nodes, _ptr(self._X),
                               mesh.nb_elements, _ptr(self._conn))
         if not self.h:
@@ -147,6 +163,74 @@ class OracleModel:
 
     def energy(self, kind):
         return self._L.orc_energy(self.h, {"kinetic": 0, "potential": 1, "external work": 2}[kind])
+
+    # -- FEEngine operators (fe_engine.hh:95-244), rows in `elements` order --
+    def _count(self, elements):
+        return self.nb_elements if elements is None else len(elements)
+
+    def _fe(self, name, arr, nb, out_cols, per_point, elements):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        ids = None if elements is None else np.ascontiguousarray(elements, dtype=np.uint32)
+        rows = self._count(elements) * (self.nb_qp if per_point else 1)
+        out = np.zeros((rows, out_cols))
+        rc = getattr(self._L, name)(self.h, _ptr(arr), int(nb),
+                                    None if ids is None else _ptr(ids),
+                                    0 if ids is None else ids.size, _ptr(out))
+        if rc != 0:
+            raise ValueError("oracle: %s rejected its arguments" % name)
+        return out
+
+    def fe_gradient(self, u, nb, elements=None):
+        return self._fe("orc_fe_gradient", u, nb, nb * self.dim, True, elements)
+
+    def fe_interpolate(self, u, nb, elements=None):
+        return self._fe("orc_fe_interpolate", u, nb, nb, True, elements)
+
+    def fe_btd(self, Ds, rows, elements=None):
+        return self._fe("orc_fe_btd", Ds, rows, rows * self.nb_nodes_per_element, True, elements)
+
+    def fe_ntb(self, bs, nb, elements=None):
+        return self._fe("orc_fe_ntb", bs, nb, nb * self.nb_nodes_per_element, True, elements)
+
+    def fe_integrate(self, f, nb, elements=None):
+        return self._fe("orc_fe_integrate", f, nb, nb, False, elements)
+
+    def fe_integrate_on_points(self, f, nb, elements=None):
+        return self._fe("orc_fe_integrate_on_points", f, nb, nb, True, elements)
+
+    def fe_integrate_scalar(self, f, elements=None):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        ids = None if elements is None else np.ascontiguousarray(elements, dtype=np.uint32)
+        return self._L.orc_fe_integrate_scalar(self.h, _ptr(f), None if ids is None else _ptr(ids),
+                                               0 if ids is None else ids.size)
+
+    def fe_assemble_field_lumped(self, field, nb, lumped):
+        field = np.ascontiguousarray(field, dtype=np.float64)
+        out = np.ascontiguousarray(lumped, dtype=np.float64).copy()
+        self._L.orc_fe_assemble_field_lumped(self.h, _ptr(field), int(nb), _ptr(out))
+        return out
+
+    # -- Material operators (material.hh:110-195) --
+    def material_elements(self, mat):
+        n = self._L.orc_material_elements(self.h, mat, None)
+        ids = np.zeros(n, dtype=np.uint32)
+        self._L.orc_material_elements(self.h, mat, _ptr(ids))
+        return ids
+
+    def material_compute_stress(self, mat, grad_u):
+        g = np.ascontiguousarray(grad_u, dtype=np.float64)
+        out = np.zeros_like(g)
+        self._L.orc_material_compute_stress(self.h, mat, _ptr(g), _ptr(out))
+        return out
+
+    def material_compute_all_stresses(self, mat):
+        self._L.orc_material_compute_all_stresses(self.h, mat)
+
+    def material_assemble_internal_forces(self, mat):
+        self._L.orc_material_assemble_internal_forces(self.h, mat)
+
+    def material_potential_energy(self, mat):
+        return self._L.orc_material_potential_energy(self.h, mat)
 
 
 def point_stress(kind, dim, E, nu, G, plane_stress=0, Sd=0.0, Yd=0.0, damage=0.0):

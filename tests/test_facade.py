@@ -6,7 +6,12 @@ The drivers follow the reference's own C++ programs for this path:
   * facade/examples/explicit_dynamic.cc:
       examples/explicit/explicit_dynamic.cc:38-106 on the bar of
       examples/explicit/bar.geo (10 x 1 x 1, x from -5) with
-      examples/explicit/material.dat, built here as a structured tet4 box.
+      examples/explicit/material.dat, built here as a structured tet4 box;
+  * facade/examples/fe_engine_operators.cc: the FEEngine checks of
+      test/test_fe_engine/test_gradient.cc:46-102 and test_integrate.cc:41-72,
+      the lumped mass, and SolidMechanicsModel::assembleInternalForces
+      restated on the Material API (computeAllStresses, assembleInternalForces
+      material by material).
 The GPU tests run them as child processes and check what they write against
 the FP64 oracle on the same mesh and initial state.
 """
@@ -46,7 +51,8 @@ BAR_STEEL = """material elastic [
 
 def _built():
     return os.path.exists(LIB) and all(
-        os.path.exists(os.path.join(BIN, b)) for b in ("explicit_dynamic", "patch_test_explicit"))
+        os.path.exists(os.path.join(BIN, b))
+        for b in ("explicit_dynamic", "patch_test_explicit", "fe_engine_operators"))
 
 
 needs_build = pytest.mark.skipif(not _built(), reason="C++ API not built (akantu_amd/build.py)")
@@ -58,7 +64,7 @@ def test_facade_library_loads():
 
 
 @needs_build
-@pytest.mark.parametrize("prog", ["explicit_dynamic", "patch_test_explicit"])
+@pytest.mark.parametrize("prog", ["explicit_dynamic", "patch_test_explicit", "fe_engine_operators"])
 def test_driver_usage(prog):
     r = subprocess.run([os.path.join(BIN, prog)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
@@ -148,3 +154,51 @@ def test_explicit_dynamic_driver_matches_oracle(precision, tmp_path):
     # the energy of the free bar is conserved by the central-difference scheme
     tot = en[:, 1] + en[:, 2]
     assert np.abs(tot - tot[0]).max() < 1e-2 * tot[0]
+
+
+TWO_MATERIALS = """material elastic [
+  name = steel
+  rho = 7800
+  E = 2.1e11
+  nu = 0.3
+]
+material neohookean [
+  name = rubber
+  rho = 1000
+  E = 1e7
+  nu = 0.45
+]
+"""
+
+
+@pytest.mark.gpu
+@needs_build
+@pytest.mark.parametrize("name", ["triangle_3", "quadrangle_4", "tetrahedron_4", "hexahedron_8"])
+def test_fe_engine_operators_driver(name, tmp_path):
+    """The FEEngine / Material driver on the reference's FEEngine test meshes,
+    two materials (elastic, neo-Hookean) on alternating elements; what it
+    writes against the oracle."""
+    from oracle import oracle as orc
+
+    msh = os.path.join(ph.GOLDEN, "fe_%s.msh" % name)
+    m = akm.read_msh(msh)
+    (tmp_path / "material.dat").write_text(TWO_MATERIALS)
+    r = _run([os.path.join(BIN, "fe_engine_operators"), msh, str(tmp_path / "material.dat"),
+              str(m.dim), str(tmp_path / "out")], tmp_path)
+    assert r.returncode == 0 and "OK" in r.stdout
+    d, Q, E = m.dim, m.nb_qp, m.nb_elements
+    em = (np.arange(E) % 3 == 1).astype(np.int32)
+    mats = [dict(kind=orc.ELASTIC, E=2.1e11, nu=0.3, rho=7800.0),
+            dict(kind=orc.NEOHOOKEAN, E=1e7, nu=0.45, rho=1000.0)]
+    cpu = orc.OracleModel(m, mats, em)
+    u = np.fromfile(str(tmp_path / "out_u.bin")).reshape(-1, d)
+    assert np.abs(u).max() > 0
+    grad = np.fromfile(str(tmp_path / "out_gradient.bin")).reshape(E * Q, d * d)
+    assert np.array_equal(grad, cpu.fe_gradient(u, d))
+    lumped = np.fromfile(str(tmp_path / "out_lumped.bin"))
+    assert np.array_equal(lumped, cpu.nodal("mass")[:, 0])
+    cpu.nodal("u")[:] = u
+    cpu.assemble_internal_forces()
+    f = np.fromfile(str(tmp_path / "out_fint.bin")).reshape(-1, d)
+    ref = cpu.nodal("f_int")
+    assert np.abs(f - ref).max() <= 1e-12 * np.abs(ref).max()

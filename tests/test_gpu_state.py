@@ -1,0 +1,103 @@
+"""Context state across re-initialisation: buffers sized by the material
+count or the element count follow a second akx_init, and the surrogate is
+re-checked against the materials and mesh it runs on."""
+import numpy as np
+import pytest
+
+from akantu_amd import akx
+from akantu_amd import mesh as akm
+from tests import gpu_helpers as gh
+
+pytestmark = pytest.mark.gpu
+
+
+def test_per_material_time_step_after_adding_a_material():
+    """The per-material stable time step buffer is sized by the material
+    count when it is first used; a material added afterwards (and a second
+    akx_init) gets its own entry, equal to the oracle's time step of a model
+    made of that material alone."""
+    m = akm.bar_tri3(20, 4)
+    g = akx.SolidMechanicsModel(m)
+    g.add_material(E=210e9, nu=0.3, rho=7800.0)
+    g.init_full()
+    one = g.get_stable_time_step_per_material()
+    assert one.shape == (1,) and one[0] == g.get_stable_time_step()
+    g.add_material(E=70e9, nu=0.33, rho=2700.0)
+    em = (np.arange(m.nb_elements) % 2).astype(np.int32)
+    g.assign_materials(em)
+    g.init_full()
+    two = g.get_stable_time_step_per_material()
+    assert two.shape == (2,)
+    from oracle import oracle as orc
+    for k, mat in enumerate([dict(E=210e9, nu=0.3, rho=7800.0),
+                             dict(E=70e9, nu=0.33, rho=2700.0)]):
+        sub = akm.Mesh(m.elem_type, m.nodes, m.conn[em == k])
+        sub = akm.unused_nodes_removed(sub)
+        assert two[k] == orc.OracleModel(sub, [mat]).stable_time_step()
+    assert g.get_stable_time_step() == two.min()
+
+
+def _plate(nx=6, ny=6, nz=2):
+    m = akm.notched_plate_tet4(nx, ny, nz)
+    yd = 2e5 * np.random.default_rng(1).weibull(5.0, size=m.nb_elements)
+    return m, yd
+
+
+def test_surrogate_rechecked_on_init():
+    """After akx_surrogate_enable, a neohookean_damage material with other
+    parameters added before a second akx_init makes that init fail with the
+    surrogate switched off; the next init runs the plain law."""
+    m, yd = _plate()
+    g = akx.SolidMechanicsModel(m)
+    g.add_material(kind=akx.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1200.0, Sd=2e7, Yd=yd)
+    g.init_full()
+    g.surrogate_setup(0, 0.05)
+    g.surrogate_enable(True)
+    assert g.query("surrogate_enabled") == 1.0
+    g.add_material(kind=akx.NEOHOOKEAN_DAMAGE, E=2e9, nu=0.3, rho=1200.0, Sd=2e7, Yd=yd)
+    g.assign_materials((np.arange(m.nb_elements) % 2).astype(np.int32))
+    with pytest.raises(akx.AkxError, match="surrogate"):
+        g.init_full()
+    assert g.query("surrogate_enabled") == 0.0
+    g.init_full()
+    g.set_time_step(0.8 * g.get_stable_time_step())
+    g.solve_step(5, akx.MIXED)
+    assert np.isfinite(g.get_displacement()).all()
+
+
+def test_surrogate_follows_a_larger_mesh():
+    """Enabled on a small plate, then the same context given a larger mesh:
+    the surrogate is reset with the mesh (its material is gone), and set up
+    again on the larger mesh it sizes its per-point buffers for it; the
+    surrogate step with an envelope nothing fits equals the plain step of
+    the oracle within 1e-5."""
+    small, yd_small = _plate(4, 4, 2)
+    g = akx.SolidMechanicsModel(small)
+    g.add_material(kind=akx.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1200.0, Sd=2e7, Yd=yd_small)
+    g.init_full()
+    g.surrogate_setup(0, 1e-9)
+    g.surrogate_enable(True)
+    big, yd = _plate(10, 10, 3)
+    mat = dict(kind=akx.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1200.0, Sd=2e7, Yd=yd)
+    p = gh.Pair(big, [mat])
+    # the context of `p` is fresh; reuse the old one through the C ABI
+    X = np.ascontiguousarray(big.nodes)
+    conn = np.ascontiguousarray(big.conn, dtype=np.uint32)
+    g.ctx.check(akx.lib.akx_mesh_set(g.ctx.h, big.elem_type, big.dim, big.nb_nodes,
+                                     akx._ptr(X), big.nb_elements, akx._ptr(conn)))
+    g.mesh = big
+    assert g.query("surrogate_enabled") == 0.0
+    g.materials = []
+    g.add_material(**mat)
+    g.init_full()
+    g.surrogate_setup(0, 1e-9)
+    g.surrogate_enable(True)
+    u = 1e-3 * np.sin(30.0 * big.nodes)
+    g.set("displacement", u)
+    p.set_nodal("displacement", u)
+    g.set_time_step(1e-7)
+    p.set_time_step(1e-7)
+    g.solve_step(3, akx.MIXED)
+    p.cpu.solve_step(3)
+    assert g.surrogate_fallbacks() > 0
+    assert gh.rel(g.get_displacement(), p.cpu.nodal("u")) <= 1e-5
