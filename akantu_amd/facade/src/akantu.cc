@@ -13,6 +13,7 @@
 #include <set>
 #include <sstream>
 
+#include "akantu/aka_random_generator.hh"
 #include "akantu/solid_mechanics_model.hh"
 #include "akx.h"
 
@@ -29,6 +30,19 @@ UInt getNbNodesPerElement(ElementType t) {
 }
 UInt getNbIntegrationPoints(ElementType t) {
   return t == _quadrangle_4 ? 4 : t == _hexahedron_8 ? 8 : 1;
+}
+
+void drawRandomValues(std::default_random_engine& engine, const std::string& dist, Real base,
+                      Real p0, Real p1, Real* out, size_t n) {
+  if (dist == "weibull") {
+    std::weibull_distribution<Real> w(p1, p0);  // [scale, shape] -> (shape, scale)
+    for (size_t k = 0; k < n; ++k) out[k] = base + w(engine);
+  } else if (dist == "uniform") {
+    std::uniform_real_distribution<Real> u(p0, p1);
+    for (size_t k = 0; k < n; ++k) out[k] = base + u(engine);
+  } else {
+    AKANTU_EXCEPTION("unsupported random distribution " << dist);
+  }
 }
 
 Parser& getStaticParser() {
@@ -526,12 +540,7 @@ void SolidMechanicsModel::initFull(const AnalysisMethodArg& method) {
           Real v = m.getParam("Yd");
           if (it != m.random_.end()) {
             const auto& r = it->second;
-            if (r.dist == "weibull")
-              v = r.base + std::weibull_distribution<Real>(r.p1, r.p0)(gen);
-            else if (r.dist == "uniform")
-              v = r.base + std::uniform_real_distribution<Real>(r.p0, r.p1)(gen);
-            else
-              AKANTU_EXCEPTION("unsupported random distribution " << r.dist);
+            drawRandomValues(gen, r.dist, r.base, r.p0, r.p1, &v, 1);
           }
           yd[size_t(e) * Q + q] = v;
         }
@@ -1093,3 +1102,14 @@ void FEEngine::assembleFieldLumped(
 #undef AKX_FE_CHECK
 
 }  // namespace akantu
+
+extern "C" int akantu_random_values(long seed, const char* dist, double base, double p0,
+                                    double p1, int64_t n, double* out) {
+  std::default_random_engine engine(static_cast<unsigned>(seed));
+  try {
+    akantu::drawRandomValues(engine, dist ? dist : "", base, p0, p1, out, size_t(n));
+  } catch (akantu::debug::Exception&) {
+    return -1;
+  }
+  return 0;
+}

@@ -349,6 +349,7 @@ struct Model {
   std::vector<double> damage, Yd;    // E*Q
   double dt = 0.;
   int nthreads = 1;
+  std::vector<std::vector<double>> part;  // per-thread force arrays (CPU baseline)
   int initialized = 0;
   std::string err;
 };
@@ -528,33 +529,42 @@ void assemble_internal_forces(Model& M) {
     return;
   }
   // Multi-threaded variant (CPU baseline only): contiguous element chunks per
-  // thread, private force arrays, reduction in fixed thread order.  This
-  // mirrors an MPI domain decomposition; it is not bit-identical to the
-  // single-thread order.
+  // thread into private force arrays (allocated once per model), then every
+  // thread adds the T partial arrays over its own range of DOFs, in thread
+  // order.  This mirrors an MPI domain decomposition; it is not
+  // bit-identical to the single-thread order.
   const int T = M.nthreads;
-  std::vector<std::vector<double>> part(T, std::vector<double>(M.fint.size(), 0.));
-  for (auto& mat : M.mats) {
-    size_t ne = mat.filter.size();
+  const size_t nd = M.fint.size();
+  if (M.part.size() != size_t(T) || (T && M.part[0].size() != nd))
+    M.part.assign(T, std::vector<double>(nd, 0.));
+  auto run = [&](auto&& body) {
     std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) {
-      th.emplace_back([&, t]() {
-        size_t b = ne * t / T, en = ne * (t + 1) / T;
-        double fe[24];
-        for (size_t k = b; k < en; ++k) {
-          uint32_t e = mat.filter[k];
-          element_force(M, mat, e, fe);
-          for (int a = 0; a < n; ++a) {
-            double* dst = &part[t][size_t(M.conn[size_t(e) * n + a]) * d];
-            for (int i = 0; i < d; ++i) dst[i] += -1. * fe[i + a * d];
-          }
-        }
-      });
-    }
+    for (int t = 0; t < T; ++t) th.emplace_back(body, t);
     for (auto& x : th) x.join();
-  }
+  };
+  run([&](int t) {
+    std::fill(M.part[t].begin(), M.part[t].end(), 0.);
+    double fe[24];
+    for (auto& mat : M.mats) {
+      const size_t ne = mat.filter.size(), b = ne * t / T, en = ne * (t + 1) / T;
+      for (size_t k = b; k < en; ++k) {
+        uint32_t e = mat.filter[k];
+        element_force(M, mat, e, fe);
+        for (int a = 0; a < n; ++a) {
+          double* dst = &M.part[t][size_t(M.conn[size_t(e) * n + a]) * d];
+          for (int i = 0; i < d; ++i) dst[i] += -1. * fe[i + a * d];
+        }
+      }
+    }
+  });
+  run([&](int t) {
+    const size_t b = nd * t / T, en = nd * (t + 1) / T;
+    for (int s = 0; s < T; ++s) {
+      const double* src = M.part[s].data();
+      for (size_t k = b; k < en; ++k) M.fint[k] += src[k];
+    }
+  });
   (void)dn;
-  for (int t = 0; t < T; ++t)
-    for (size_t k = 0; k < M.fint.size(); ++k) M.fint[k] += part[t][k];
 }
 
 int precompute(Model& M) {

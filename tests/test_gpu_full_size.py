@@ -24,6 +24,7 @@ import pytest
 
 from akantu_amd import akx
 from akantu_amd import mesh as akm
+from akantu_amd import rng as akrng
 from tests import gpu_helpers as gh
 
 pytestmark = pytest.mark.gpu
@@ -149,7 +150,7 @@ def test_plate_damage_full_size():
     E = mesh.nb_elements
     assert E > 4_000_000
     X = mesh.nodes
-    yd = 2e5 * np.random.default_rng(4).weibull(5.0, size=E)
+    yd = akrng.weibull_thresholds(E, 2e5, 5.0, seed=4)
     shift = np.tile([3e-3, -2e-3, 1e-3], (X.shape[0], 1))
     u = 2e-3 * np.stack([np.sin(20.0 * X[:, 1] + 30.0 * X[:, 2]),
                          np.cos(15.0 * X[:, 0]) * X[:, 1] * 10.0,

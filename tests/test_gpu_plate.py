@@ -4,7 +4,7 @@ tetrahedron_4 (the learned-surrogate workload, reduced), against the oracle.
 Law: S = (1 - d) S_NH(F) with the Marigo update on the driving energy
 Y = W_NH (material_neohookean_inline_impl.hh:46-65,145-158,
 material_marigo_inline_impl.hh:67-76); per-element thresholds Yd from a
-Weibull(m=5, scale 0.2 MPa) draw, Sd = 20 MPa.  The top face is pulled at a
+Weibull(m=5, scale 0.2 MPa) draw of the reference's engine (seed 4), Sd = 20 MPa.  The top face is pulled at a
 prescribed velocity (applyBC(IncrementValue) before every solveStep), the
 bottom face is held: after 300 steps about 3/4 of the points are damaged.
 
@@ -22,6 +22,7 @@ import pytest
 
 from akantu_amd import akx
 from akantu_amd import mesh as akm
+from akantu_amd import rng as akrng
 from tests import gpu_helpers as gh
 
 pytestmark = pytest.mark.gpu
@@ -34,8 +35,7 @@ STEPS = 300
 
 def plate_pair(nx=12, ny=12, nz=3, seed=4, yd_scale=2e5):
     m = akm.notched_plate_tet4(nx, ny, nz)
-    rng = np.random.default_rng(seed)
-    yd = yd_scale * rng.weibull(5.0, size=m.nb_elements)
+    yd = akrng.weibull_thresholds(m.nb_elements, yd_scale, 5.0, seed=seed)
     p = gh.Pair(m, [dict(PLATE, Sd=SD, Yd=yd)])
     X = m.nodes
     mask = np.zeros_like(X, dtype=bool)

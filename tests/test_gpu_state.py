@@ -6,6 +6,7 @@ import pytest
 
 from akantu_amd import akx
 from akantu_amd import mesh as akm
+from akantu_amd import rng as akrng
 from tests import gpu_helpers as gh
 
 pytestmark = pytest.mark.gpu
@@ -39,7 +40,7 @@ def test_per_material_time_step_after_adding_a_material():
 
 def _plate(nx=6, ny=6, nz=2):
     m = akm.notched_plate_tet4(nx, ny, nz)
-    yd = 2e5 * np.random.default_rng(1).weibull(5.0, size=m.nb_elements)
+    yd = akrng.weibull_thresholds(m.nb_elements, 2e5, 5.0, seed=1)
     return m, yd
 
 

@@ -7,6 +7,7 @@ import pytest
 
 from akantu_amd import akx
 from akantu_amd import mesh as akm
+from akantu_amd import rng as akrng
 from tests import mlp_reference as ref
 
 pytestmark = pytest.mark.gpu
@@ -21,8 +22,7 @@ def _lame(E, nu):
 
 def _plate_model(nx=12, ny=12, nz=3, seed=4, eps=0.05):
     m = akm.notched_plate_tet4(nx, ny, nz)
-    rng = np.random.default_rng(seed)
-    yd = 1e6 * rng.weibull(5.0, size=m.nb_elements)
+    yd = akrng.weibull_thresholds(m.nb_elements, 1e6, 5.0, seed=seed)
     model = akx.SolidMechanicsModel(m)
     model.add_material(kind=akx.NEOHOOKEAN_DAMAGE, Sd=SD, Yd=yd, **PLATE)
     model.init_full()
