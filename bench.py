@@ -33,6 +33,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -478,6 +479,24 @@ def bench_hex8(args, akx):
     u_fp64 = model.get_displacement()
     rel_u = float(np.linalg.norm(u_mixed - u_fp64) / np.linalg.norm(u_fp64))
     log("mixed vs FP64 after %d steps: rel u error %.3e" % (steps, rel_u))
+    strain = strain_run(model, shapes, akx, v0, top, dt) if not args.no_strain_run else None
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+        cpu_steps = max(1, args.cpu_steps // 5)
+        log("CPU baseline: oracle, %d threads, %d steps (+1 warm-up) on the full mesh"
+            % (threads, cpu_steps))
+        vt = v0.copy()
+        _, sec = cpu_baseline(m, [dict(kind=akx.NEOHOOKEAN, E=1e7, nu=0.45, rho=1000.0)],
+                              blocked, vt, dt, 1, cpu_steps, threads)
+        cpu = {"value": E * Q / sec, "unit": "QP-updates/s", "cores": threads,
+               "host_cpus": os.cpu_count() or 0, "kind": "port",
+               "sample": "oracle (C++ FP64 restatement of the reference loops), %d threads, "
+                         "%s, full %d-element mesh (%d points), %d timed steps after 1 warm-up "
+                         "step, %.3f s/step" % (threads, host_cpu_model(), E, E * Q, cpu_steps,
+                                                sec)}
+        log("CPU: %.3f s/step" % sec)
 
     def elem_kernel(patch, rc):
         return ("void akx::k_element_patch<3, 1>(akx::ElemArgs)" if patch else
@@ -494,14 +513,51 @@ def bench_hex8(args, akx):
     out.update({
         "fp64": fp64_record(res, shapes),
         "launch_mixed": shapes["mixed"],
-        "accuracy": {"steps": steps, "rel_u_error_mixed_vs_fp64": rel_u},
+        "accuracy": {"steps": steps, "rel_u_error_mixed_vs_fp64": rel_u,
+                     "to_10_percent_strain": strain},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": kbytes, "avg_ms": kms},
         "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     })
+    return out
+
+
+def strain_run(model, shapes, akx, v0, top, dt, strain=0.10, speed=0.5):
+    """The hexahedra compressed to `strain` (top face at `speed`), mixed and
+    FP64: displacement error, and each path's energy balance: the change of
+    E_kin + E_pot against the work of the moving face (getExternalWork summed
+    every step; the face's nodes carry their prescribed velocity)."""
+    nsteps = int(math.ceil(strain / (speed * dt)))
+    log("strain run: %d steps to %.0f %% strain, mixed and FP64, energies every step"
+        % (nsteps, 100 * strain))
+    vt = v0.copy()
+    vt[top, 2] = -speed
+    out = {"steps": nsteps}
+    u = {}
+    for name, prec in (("mixed", akx.MIXED), ("fp64", akx.FP64)):
+        reset(model, vt)
+        model.set_launch(**shapes[name])
+        e0 = model.get_energy("kinetic") + model.get_energy("potential")
+        work = 0.0
+        for _ in range(nsteps):
+            model.solve_step(1, prec)
+            work += model.get_energy("external work")
+        e1 = model.get_energy("kinetic") + model.get_energy("potential")
+        u[name] = model.get_displacement()
+        out[name] = {"energy_change": e1 - e0, "boundary_work": work,
+                     "balance_rel": (e1 - e0 - work) / work, "e_kin_plus_e_pot": e1}
+    out["top_displacement"] = float(-u["fp64"][top, 2].mean())
+    out["rel_u_error_mixed_vs_fp64"] = float(np.linalg.norm(u["mixed"] - u["fp64"]) /
+                                             np.linalg.norm(u["fp64"]))
+    out["rel_energy_error_mixed_vs_fp64"] = float(
+        abs(out["mixed"]["e_kin_plus_e_pot"] - out["fp64"]["e_kin_plus_e_pot"]) /
+        out["fp64"]["e_kin_plus_e_pot"])
+    log("strain run: top face moved %.4f, rel u error %.3e, energy balance mixed %.2e fp64 %.2e"
+        % (out["top_displacement"], out["rel_u_error_mixed_vs_fp64"],
+           out["mixed"]["balance_rel"], out["fp64"]["balance_rel"]))
     return out
 
 
@@ -742,6 +798,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-strain-run", action="store_true",
+                    help="hex8-neohookean: skip the run to 10 %% strain")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 1 GiB device copy (calibration of the traffic counters)")
     ap.add_argument("--launch-mixed", default=None,

@@ -176,11 +176,59 @@ def test_model_dump_writes_the_device_state(tmp_path):
     assert np.array_equal(arr["displacement"][:, :2], model.get_displacement())
     assert np.array_equal(arr["velocity"][:, :2], model.get_velocity())
     assert np.array_equal(arr["stress"], model.get("stress"))
+    # what the files hold is the reference's state: the oracle after the same
+    # 20 steps (its stresses are those of the last step's assembly, on the
+    # displacement the corrector leaves unchanged)
+    from oracle import oracle as orc
+    cpu = orc.OracleModel(m, [dict(E=210e9, nu=0.3, rho=7800.0)])
+    cpu.nodal("v")[:] = v0
+    cpu.set_time_step(model.time_step)
+    cpu.solve_step(20)
+    assert np.array_equal(arr["displacement"][:, :2], cpu.nodal("u"))
+    assert np.array_equal(arr["velocity"][:, :2], cpu.nodal("v"))
+    assert np.array_equal(arr["stress"], cpu.field("stress").reshape(m.nb_elements, 4))
     model.solve_step(20, akx.FP64)
     model.dump()
     model.flush_dumps()
     ds = ET.parse(tmp_path / "paraview" / "bar.pvd").getroot().findall("Collection/DataSet")
     assert len(ds) == 2
+    model.close()
+
+
+@pytest.mark.gpu
+def test_hex_damage_dump_equals_oracle(tmp_path):
+    """Hexahedra with the neo-Hookean + damage law, 30 FP64 steps: the dumped
+    nodal fields equal the oracle's, the per-element stress and damage (the
+    average of the 8 points) are within 1e-12 (the law's log)."""
+    from akantu_amd import akx
+    from oracle import oracle as orc
+    model = _hex_damage_model()
+    m = model.mesh
+    rng = np.random.default_rng(1)
+    v0 = rng.normal(scale=0.5, size=m.nodes.shape)
+    blocked = np.zeros(m.nodes.shape, dtype=bool)
+    blocked[m.nodes[:, 2] == 0.0] = True
+    model.set_base_name("hex", str(tmp_path / "paraview"))
+    for f in ("displacement", "velocity", "stress", "damage"):
+        model.add_dump_field(f)
+    model.solve_step(30, akx.FP64)
+    path = model.dump()
+    model.flush_dumps()
+    _, arr = _arrays(path)
+    cpu = orc.OracleModel(m, [dict(kind=orc.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1000.0,
+                                   Sd=1e4, Yd=np.full(m.nb_elements * m.nb_qp, 50.0))])
+    cpu.nodal("v")[:] = v0
+    cpu.blocked[:] = blocked
+    cpu.set_time_step(model.time_step)
+    cpu.solve_step(30)
+    u_ref = cpu.nodal("u")
+    assert np.abs(arr["displacement"] - u_ref).max() <= 1e-12 * np.abs(u_ref).max()
+    assert np.abs(arr["velocity"] - cpu.nodal("v")).max() <= 1e-12 * np.abs(cpu.nodal("v")).max()
+    dmg = cpu.field("damage").reshape(m.nb_elements, m.nb_qp).mean(axis=1)
+    assert (dmg > 0).any()
+    assert np.abs(arr["damage"] - dmg).max() <= 1e-12
+    sig = cpu.field("stress").reshape(m.nb_elements, m.nb_qp, 9).mean(axis=1)
+    assert np.abs(arr["stress"] - sig).max() <= 1e-12 * np.abs(sig).max()
     model.close()
 
 
