@@ -88,6 +88,7 @@ SIGNATURES = {
     "akx_material_compute_all_stresses": (_I, [_P, _I, _P, _P]),
     "akx_material_assemble_internal_forces": (_I, [_P, _I]),
     "akx_material_potential_energy": (_I, [_P, _I, _PD]),
+    "akx_energy_noalloc": (_I, [_I, _I, _I64, _P, _P, _P, _P, _P, _D, _P, _P]),
 }
 
 

@@ -21,7 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libakx.so")
 ARCH = os.environ.get("AKX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["akx_kernels.hip", "akx_surrogate.hip", "akx_fe.hip", "akx_api.cpp"]
+SOURCES = ["akx_kernels.hip", "akx_surrogate.hip", "akx_fe.hip", "akx_noalloc.hip",
+           "akx_api.cpp"]
 HEADERS = ["akx_device.hpp", "akx_internal.hpp", "akx_mlp.hpp"]
 FACADE = os.path.join(HERE, "facade")
 FACADE_LIB = os.path.join(HERE, "libakantu_amd.so")

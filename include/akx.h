@@ -385,6 +385,25 @@ int akx_material_assemble_internal_forces(akx_ctx* ctx, int mat_id);
  * current displacement. */
 int akx_material_potential_energy(akx_ctx* ctx, int mat_id, double* value);
 
+/* ---------------------------------------------------------------------------
+ * Allocation-free variant.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).
+ * ------------------------------------------------------------------------- */
+/* akx_energy for AKX_ENERGY_KINETIC (velocity (N, dim), mass one value per
+ * node) or AKX_ENERGY_EXTERNAL_WORK (velocity, external_force, internal_force
+ * (N, dim), blocked (N, dim) u8, time_step), on caller device arrays; the
+ * energy is written to the device address `energy`, on `hip_stream` (null:
+ * the default stream), asynchronously.  Allocates nothing: its reduction
+ * scratch is a fixed device array of 4096 + 1 doubles per GPU, held until the
+ * process ends; any nb_nodes fits it.  Calls that overlap on one GPU race on
+ * that array.  Bit-identical to akx_energy on the same arrays (same kernels,
+ * same grid); AKX_ENERGY_POTENTIAL is not offered (AKX_ERR_UNSUPPORTED). */
+int akx_energy_noalloc(int kind, int dim, int64_t nb_nodes, const double* velocity,
+                       const double* mass, const double* external_force,
+                       const double* internal_force, const uint8_t* blocked, double time_step,
+                       double* energy, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif
