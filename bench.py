@@ -117,6 +117,19 @@ def fused_bytes(E, N, n, d, evaluations, nblk, node_block=256, chunk=256):
             + E * n * 4 + N * d * 8 + 2 * N * d * 8 + 4 * N * d * 8 + N * 8 + N * d)
 
 
+def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_block=256,
+                      chunk=256):
+    """Compulsory HBM bytes of one fused mixed step (slot layout 6 on the mixed
+    path): as the FP64 one, but triangles read a 16-byte FP32 record of their
+    coordinate differences instead of gathering X, and within a run the node
+    update carries the half-step velocity (v read and written, no a)."""
+    chunks = evaluations / chunk + nblk
+    coords = E * 16 if n == 3 else N * d * 8
+    return (evaluations * (4 + 2 * n) + chunks * (node_block + 1) * 2 + (nblk + 1) * 8
+            + E * n * 4 + coords + 2 * N * d * 8 + 2 * N * d * 8 + N * 8 + N * d
+            + (N * 8 if dt_per_node else 0))
+
+
 def make_workload(n):
     from akantu_amd import mesh as akm
     m = akm.cube_tet4(n, 0.01, 1)
@@ -578,7 +591,7 @@ def bench_ensemble(args, akx):
     dt_mat = model.get_stable_time_step_per_material()
     model.set_node_time_steps(0.8 * dt_mat[en.node_specimen()])
     log("E=%d N=%d, specimen dt in [%.3e, %.3e]" % (E, N, 0.8 * dt_mat.min(), 0.8 * dt_mat.max()))
-    shapes = tune(model, args, akx)
+    shapes = tune(model, args, akx, "ensemble_tri3_traffic.json")
     res = measure(model, shapes, args, akx, v0, E * Q, graph=True)
     steps = args.error_steps if args.error_steps != 1000 else 2000
     chunks = 10
@@ -610,9 +623,26 @@ def bench_ensemble(args, akx):
     def elem_kernel(patch, rc):
         return ("void akx::k_element_patch<0, 0>(akx::ElemArgs)" if patch else
                 "void akx::k_element_mixed<0, 0, %s>(akx::ElemArgs)" % ("true" if rc else "false"))
-    kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
-        res, shapes, model, elem_kernel, "void akx::k_node<2, 1, false>(akx::NodeArgs)",
-        E, N, n, Q, d, dt_per_node=True)
+    fused = model.query("fused_available") == 1
+    nblk = (N + 255) // 256
+    step64 = None
+    if fused and shapes["fp64"]["slot_layout"] == 6:
+        step64 = fused_bytes(E, N, n, d, model.query("fused_elements"), nblk) + N * 8
+    if fused and shapes["mixed"]["slot_layout"] == 6:
+        # one kernel per step: its roofline is the step's
+        kname = ("void akx::k_step_fused_mixed<0, 0, false>"
+                 "(akx::ElemArgs, akx::NodeArgs, akx::FusedArgs)")
+        kbytes = fused_mixed_bytes(E, N, n, d, model.query("fused_elements"), nblk,
+                                   dt_per_node=True)
+        kms = res["mixed"]["element_ms"]
+        achieved = kbytes / (kms * 1e-3) / 1e9
+        eb, nb = kbytes, 0
+        step_gbs = kbytes / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9
+    else:
+        kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
+            res, shapes, model, elem_kernel, "void akx::k_node<2, 1, false>(akx::NodeArgs)",
+            E, N, n, Q, d, dt_per_node=True)
+    traffic = traffic_for(kname, shapes, "ensemble_tri3_traffic.json")
     out = base_record(args, res, "f32 element kernels / f64 nodal state", {
         "workload": "ensemble: 512 plane-strain triangle_3 specimens, %d elements and %d nodes "
                     "in total, per-specimen E, nu, rho and stable dt, sinusoidal initial "
@@ -620,7 +650,7 @@ def bench_ensemble(args, akx):
         "elements": E, "nodes": N, "specimens": 512, "qp_per_element": Q,
         "precision": "mixed", "parallelism": "1 GPU"})
     out.update({
-        "fp64": fp64_record(res, shapes),
+        "fp64": fp64_record(res, shapes, step64),
         "launch_mixed": shapes["mixed"],
         "accuracy": {"steps": steps,
                      "per_specimen_error_vs_amplitude_max": float(err_amp.max()),
@@ -631,7 +661,7 @@ def bench_ensemble(args, akx):
                              "/ max over the samples of |u_fp64| per specimen"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": kbytes, "avg_ms": kms},
+                     "traffic": traffic, "algorithmic_bytes": kbytes, "avg_ms": kms},
         "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": None,
