@@ -236,7 +236,7 @@ struct Surrogate {
   int64_t step = 0;
   __bf16* Wbf[5] = {};
   __bf16* WT[5] = {};
-  mlp::bf16x8* img = nullptr;
+  mlp::f16x8* img = nullptr;
   float* imgb = nullptr;
   unsigned long long* fallbacks = nullptr;
   // training set

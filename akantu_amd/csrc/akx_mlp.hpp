@@ -1,7 +1,8 @@
 // akx_mlp.hpp -- the constitutive surrogate network on gfx950 matrix cores.
 //
 // Network: 7 -> 128 -> 128 -> 128 -> 128 -> 7, SiLU after every hidden layer,
-// BF16 weights and activations, FP32 accumulation and biases.
+// FP32 accumulation and biases; training in BF16 (weights, activations,
+// errors), inference in FP16 (weights, activations; see below).
 //   inputs  x = (E_xx, E_yy, E_zz, E_yz, E_xz, E_xy) / eps_max, damage
 //           (E = Green-Lagrange strain, tensor components, Voigt order of
 //            aka_voigthelper.cc:52-61)
@@ -10,11 +11,12 @@
 //           S_lin = lambda tr(E) I + 2 mu E, W_lin = lambda/2 tr(E)^2 + mu E:E,
 //           S_NH, W_NH the neo-Hookean PK2 stress and energy
 //           (material_neohookean_inline_impl.hh:46-65, 145-158).
-// The input layer sees every input twice, as its BF16 value and as the BF16
-// value of the rounding residual, with the same weights: its 16-wide K slot
-// (7 + 7 + 2 zeros) costs nothing extra and keeps the inputs at ~16 bits.
+// The input layer sees every input twice, as its 16-bit value and as the
+// 16-bit value of the rounding residual, with the same weights: its 16-wide K
+// slot (7 + 7 + 2 zeros) costs nothing extra and keeps the inputs at ~16 bits
+// (BF16) or ~22 bits (FP16).
 //
-// MFMA v_mfma_f32_32x32x16_bf16 (one wave, 64 lanes):
+// MFMA v_mfma_f32_32x32x16_bf16 / _f16 (one wave, 64 lanes, same layout):
 //   A operand (32 x 16): lane l supplies row l % 32 and 8 values of the
 //   K half l / 32; B operand (16 x 32): lane l supplies column l % 32 and 8
 //   values of the K half l / 32.  The product sums A-slot s of lane (i, h)
@@ -51,7 +53,7 @@ constexpr int NPARAM = OFF_B4 + NOUT;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// permuted BF16 weight image (units of bf16x8 = 8 weights = one lane's operand)
+// permuted inference weight image (units of 8 FP16 weights = one lane's operand)
 constexpr int PW_L0 = 0;                      // [mb 4][lane 64]
 constexpr int PW_L1 = PW_L0 + 4 * 64;         // [mb 4][t 8][lane 64]
 constexpr int PW_L2 = PW_L1 + 4 * 8 * 64;
@@ -85,12 +87,129 @@ __device__ __forceinline__ float silu_grad(float z) {
 __device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
 __device__ __forceinline__ float from_bf16(__bf16 x) { return static_cast<float>(x); }
 
-// Weights and biases of one network, as staged in LDS by the kernels that
-// evaluate it.
+// ---------------------------------------------------------------------------
+// Inference (k_sur_infer): FP16 weights and activations on
+// v_mfma_f32_32x32x16_f16, FP32 accumulation, and a SiLU with one
+// transcendental per activation whose tail runs on packed FP16 data.
+//
+// The layers carry y = z log2(e) instead of the pre-activation z, and
+// g = 2 log2(e) SiLU(z) instead of SiLU(z):
+//   g = (y + |y|) + |y| X Q(X),  X = 2^-|y|,  Q(X) ~ -2 / (1 + X),
+// which is y + |y| (1 - X) / (1 + X) = y (1 + tanh(z / 2)) = 2 y sigmoid(z).
+// The scale factors go into the weights: layer 0 W log2(e), hidden layers
+// W / 2 (exact), output layer W ln(2) / 2; biases of layers 0-3 times
+// log2(e), the output bias as is.  2^-|y| is one v_exp_f32 (its -|.| is a
+// source modifier); y and 2^-|y| are rounded in pairs to FP16
+// (v_cvt_pk_f16_f32, to nearest) and the rest is four v_pk_fma_f16 for Q,
+// one bit mask, one v_pk_mul_f16, one v_pk_add_f16 and one v_pk_fma_f16 per
+// pair of activations.  Q is quartic with FP16 coefficients fitted for the
+// least maximum of |y| X |Q(X) + 2 / (1 + X)| over X in [0, 1]: 1.13e-4, the
+// absolute error of g that Q itself adds.  (A cubic, 5.25e-4, is one
+// instruction cheaper but its error, the same for every point where a
+// neuron's input sits near its bias, adds up over a layer into an output
+// offset of a few 1e-3 at small strains.)  For y < 0, y + |y| = 0 exactly
+// and g = |y| X Q(X) keeps its relative precision however small it is.
+// Bias: the MFMA's C operand of a layer's first K step.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+constexpr float SCALE_IN = 1.4426950408889634f;     // log2(e), layer 0 weights and biases 0-3
+constexpr float SCALE_HID = 0.5f;                    // hidden layers 1-3
+constexpr float SCALE_OUT = 0.34657359027997264f;    // ln(2) / 2, output layer
+// Q(X) = q0 + q1 X + q2 X^2 + q3 X^3 + q4 X^4 (FP16 values)
+constexpr float SILU_Q0 = -1.9990234375f, SILU_Q1 = 1.970703125f, SILU_Q2 = -1.7568359375f,
+                SILU_Q3 = 1.126953125f, SILU_Q4 = -0.343994140625f;
+
+__device__ __forceinline__ f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f16x2 splat2(float v) {
+  const _Float16 h = static_cast<_Float16>(v);
+  return f16x2{h, h};
+}
+
+// g for the two accumulator values y0, y1
+__device__ __forceinline__ f16x2 silu2(float y0, float y1) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const float x0 = __builtin_amdgcn_exp2f(-__builtin_fabsf(y0));
+  const float x1 = __builtin_amdgcn_exp2f(-__builtin_fabsf(y1));
+  const f16x2 Y = __builtin_convertvector(f32x2{y0, y1}, f16x2);
+  const f16x2 X = __builtin_convertvector(f32x2{x0, x1}, f16x2);
+  const f16x2 A = __builtin_bit_cast(f16x2, __builtin_bit_cast(uint32_t, Y) & 0x7fff7fffu);
+  f16x2 Q = __builtin_elementwise_fma(splat2(SILU_Q4), X, splat2(SILU_Q3));
+  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q2));
+  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q1));
+  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q0));
+  const f16x2 AX = A * X;
+  const f16x2 YA = Y + A;
+  return __builtin_elementwise_fma(AX, Q, YA);
+}
+
+__device__ __forceinline__ f16x8 join4(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
+  const f16x4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+  const f16x4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
+  return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// Weights (scaled, FP16, permuted as the chained layout needs) and biases
+// (FP32, layers 0-3 scaled by log2(e), in feature order) of one network, as
+// staged in LDS by the inference kernel.
 struct Image {
-  bf16x8 w[PW_SIZE];
+  f16x8 w[PW_SIZE];
   float b[PB_SIZE];
 };
+
+// The C operand holding layer `layer`'s bias for row block mb: register v of
+// lane half h is feature 32 mb + crow(v, h); registers 4g..4g+3 are the four
+// consecutive features 32 mb + 8 g + 4 h + 0..3 (one 16-byte LDS read).
+__device__ __forceinline__ f32x16 bias_block(const float* B, int layer, int mb, int h) {
+  f32x16 c;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 bb = *reinterpret_cast<const float4*>(&B[layer * HID + 32 * mb + 8 * g + 4 * h]);
+    c[4 * g + 0] = bb.x;
+    c[4 * g + 1] = bb.y;
+    c[4 * g + 2] = bb.z;
+    c[4 * g + 3] = bb.w;
+  }
+  return c;
+}
+
+// g of the 16 values of one C row block: act slots of K steps 2 mb (registers
+// 0-7) and 2 mb + 1 (registers 8-15) of the next layer (chain_k order)
+__device__ __forceinline__ void activate_block(const f32x16& a, f16x8& lo, f16x8& hi) {
+  lo = join4(silu2(a[0], a[1]), silu2(a[2], a[3]), silu2(a[4], a[5]), silu2(a[6], a[7]));
+  hi = join4(silu2(a[8], a[9]), silu2(a[10], a[11]), silu2(a[12], a[13]), silu2(a[14], a[15]));
+}
+
+// Row block mb of a hidden layer: its 8 K steps chained on one accumulator,
+// which starts from the bias.
+__device__ __forceinline__ f32x16 hidden_block(const float* B, const f16x8* wl,
+                                               const f16x8 (&in)[8], int layer, int mb, int h) {
+  f32x16 c = bias_block(B, layer, mb, h);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) c = mfma16(wl[(mb * 8 + t) * 64], in[t], c);
+  return c;
+}
+
+// One hidden layer, in -> out.  The row blocks run one after the other so
+// that the activation of block mb (vector pipe) overlaps the MFMAs of block
+// mb + 1 (matrix pipe) within the wave; the scheduling groups interleave
+// them one MFMA to a share of the activation's vector instructions.
+__device__ __forceinline__ void hidden_layer(const f16x8* W, const float* B, int layer, int lane,
+                                             int h, const f16x8 (&in)[8], f16x8 (&out)[8]) {
+  const f16x8* wl = W + PW_L1 + (layer - 1) * (PW_L2 - PW_L1) + lane;
+  f32x16 a0 = hidden_block(B, wl, in, layer, 0, h);
+  f32x16 a1 = hidden_block(B, wl, in, layer, 1, h);
+  activate_block(a0, out[0], out[1]);
+  f32x16 a2 = hidden_block(B, wl, in, layer, 2, h);
+  activate_block(a1, out[2], out[3]);
+  f32x16 a3 = hidden_block(B, wl, in, layer, 3, h);
+  activate_block(a2, out[4], out[5]);
+  activate_block(a3, out[6], out[7]);
+}
 
 // Evaluate the network for the 32 points of one wave: lanes j and j + 32 both
 // carry point j's inputs x[0..6] and both receive its outputs y[0..6].  Every
@@ -98,70 +217,42 @@ struct Image {
 __device__ __forceinline__ void eval_wave(const Image& L, const float* x, float* y) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
-  // input layer: K slot 8h + s holds x_hi[k] (k < 7), x_lo[k - 7] (7 <= k < 14), 0
-  bf16x8 bx;
+  // A zero offset that passes through an empty asm statement on every call:
+  // the weight reads cannot be hoisted out of the caller's loop over tiles
+  // (they would not fit in registers and would spill).
+  int zero = 0;
+  asm volatile("" : "+v"(zero));
+  const f16x8* W = L.w + zero;
+  const float* B = L.b + zero;
+  // input layer: K slot 8h + s holds x_hi[k] (k < 7), x_lo[k - 7] (7 <= k < 14), 0;
+  // FP16 round to nearest, x_lo = x - x_hi exactly in FP32
+  _Float16 xh[NIN], xl[NIN];
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) {
+    xh[k] = static_cast<_Float16>(x[k]);
+    xl[k] = static_cast<_Float16>(x[k] - static_cast<float>(xh[k]));
+  }
+  f16x8 bx;
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
-    const int k = 8 * h + s;
-    float v = 0.f;
-    if (k < NIN) {
-      v = x[k];
-    } else if (k < 2 * NIN) {
-      const float xv = x[k - NIN];
-      v = xv - from_bf16(to_bf16(xv));
-    }
-    bx[s] = to_bf16(v);
+    const _Float16 first = s < NIN ? xh[s] : xl[s - NIN];           // k = s
+    const _Float16 second = s + 8 < 2 * NIN ? xl[s + 8 - NIN] : _Float16(0);  // k = 8 + s
+    bx[s] = h ? second : first;
   }
-  f32x16 acc[4];
+  f16x8 actA[8], actB[8];  // the activations, ping-pong between the layers
+  {
+    f32x16 a[4];
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) {
-    acc[mb] = f32x16{};
-    acc[mb] = mfma(L.w[PW_L0 + mb * 64 + lane], bx, acc[mb]);
+    for (int mb = 0; mb < 4; ++mb) a[mb] = mfma16(W[PW_L0 + mb * 64 + lane], bx, bias_block(B, 0, mb, h));
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) activate_block(a[mb], actA[2 * mb], actA[2 * mb + 1]);
   }
-  bf16x8 act[8];
-  auto activate = [&](int layer) {
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        // slots 4g..4g+3 carry 4 consecutive features: one 16-byte bias read
-        const float4 bb =
-            *reinterpret_cast<const float4*>(&L.b[layer * HID + chain_k(t, h, 4 * g)]);
-        const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int s = 4 * g + r;
-          const float z = acc[t >> 1][8 * (t & 1) + s] + bv[r];
-          act[t][s] = to_bf16(silu(z));
-        }
-      }
-  };
-  activate(0);
-  // One copy of the hidden-layer code (unrolling the three layers lets the
-  // scheduler hoist every weight load and spill).  K steps outermost: the
-  // four MFMAs of a step use independent accumulators; the LDS latency of
-  // the weight reads is covered by the other wave of the SIMD.
-#pragma unroll 1
-  for (int layer = 1; layer <= 3; ++layer) {
-    const bf16x8* wl = L.w + PW_L1 + (layer - 1) * (PW_L2 - PW_L1) + lane;
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x16{};
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      bf16x8 w[4];
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) w[mb] = wl[(mb * 8 + t) * 64];
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) acc[mb] = mfma(w[mb], act[t], acc[mb]);
-    }
-    activate(layer);
-  }
+  hidden_layer(W, B, 1, lane, h, actA, actB);
+  hidden_layer(W, B, 2, lane, h, actB, actA);
+  hidden_layer(W, B, 3, lane, h, actA, actB);
   f32x16 o = f32x16{};
-  bf16x8 w4[8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) w4[t] = L.w[PW_L4 + t * 64 + lane];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) o = mfma(w4[t], act[t], o);
+  for (int t = 0; t < 8; ++t) o = mfma16(W[PW_L4 + t * 64 + lane], actB[t], o);
   // rows 0..3 sit in registers 0..3 of half 0, rows 4..7 in registers 0..3 of half 1
   float mine[4], other[4];
 #pragma unroll
@@ -172,13 +263,13 @@ __device__ __forceinline__ void eval_wave(const Image& L, const float* x, float*
   const float* lo = h ? other : mine;
   const float* hi = h ? mine : other;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) y[r] = lo[r] + L.b[4 * HID + r];
+  for (int r = 0; r < 4; ++r) y[r] = lo[r] + B[4 * HID + r];
 #pragma unroll
-  for (int r = 4; r < NOUT; ++r) y[r] = hi[r - 4] + L.b[4 * HID + r];
+  for (int r = 4; r < NOUT; ++r) y[r] = hi[r - 4] + B[4 * HID + r];
 }
 
 // Cooperative copy of a permuted image (global) into LDS by one block.
-__device__ __forceinline__ void stage_image(Image& L, const bf16x8* __restrict__ w,
+__device__ __forceinline__ void stage_image(Image& L, const f16x8* __restrict__ w,
                                             const float* __restrict__ b) {
   for (int i = threadIdx.x; i < PW_SIZE; i += blockDim.x) L.w[i] = w[i];
   for (int i = threadIdx.x; i < PB_SIZE; i += blockDim.x) L.b[i] = b[i];

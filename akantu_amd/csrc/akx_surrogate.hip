@@ -337,11 +337,12 @@ __global__ void __launch_bounds__(256) k_sur_update(LayerDesc L, int nch, float*
 }
 
 // BF16 copies of the master weights (forward layout, transposed layout) and
-// the permuted inference image
+// the permuted FP16 inference image, its weights and biases scaled as the
+// inference kernel's SiLU form needs (akx_mlp.hpp)
 __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P, __bf16* W0,
                                                      __bf16* W1, __bf16* W2, __bf16* W3,
                                                      __bf16* W4, __bf16* T1, __bf16* T2,
-                                                     __bf16* T3, __bf16* T4, bf16x8* img,
+                                                     __bf16* T3, __bf16* T4, f16x8* img,
                                                      float* imgb) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   // forward / transposed copies
@@ -370,23 +371,24 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
     if (vec < PW_L1) {
       const int mb = vec / 64, k = 8 * h + s, row = 32 * mb + i;
       w = k < NIN ? P[OFF_W0 + row * NIN + k] : k < 2 * NIN ? P[OFF_W0 + row * NIN + k - NIN] : 0.f;
+      w *= SCALE_IN;
     } else if (vec < PW_L4) {
       const int l = (vec - PW_L1) / (4 * 8 * 64), rem = (vec - PW_L1) % (4 * 8 * 64);
       const int mb = rem / (8 * 64), t = (rem / 64) % 8, row = 32 * mb + i;
       const int off = l == 0 ? OFF_W1 : l == 1 ? OFF_W2 : OFF_W3;
-      w = P[off + row * HID + chain_k(t, h, s)];
+      w = P[off + row * HID + chain_k(t, h, s)] * SCALE_HID;
     } else {
       const int t = (vec - PW_L4) / 64;
-      w = i < NOUT ? P[OFF_W4 + i * HID + chain_k(t, h, s)] : 0.f;
+      w = i < NOUT ? P[OFF_W4 + i * HID + chain_k(t, h, s)] * SCALE_OUT : 0.f;
     }
-    img[vec][s] = to_bf16(w);
+    img[vec][s] = static_cast<_Float16>(w);
   }
   if (idx < PB_SIZE) {
     float b = 0.f;
     if (idx < 4 * HID) {
       const int l = idx / HID, n = idx % HID;
       const int off = l == 0 ? OFF_B0 : l == 1 ? OFF_B1 : l == 2 ? OFF_B2 : OFF_B3;
-      b = P[off + n];
+      b = P[off + n] * SCALE_IN;
     } else if (idx - 4 * HID < NOUT) {
       b = P[OFF_B4 + idx - 4 * HID];
     }
@@ -397,7 +399,7 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
 // evaluation of the network on n points (scaled inputs, sx floats per point
 // -> outputs, sy floats per point); persistent blocks of 8 waves, the image
 // staged in LDS once per block
-__global__ void __launch_bounds__(512) k_sur_infer(const bf16x8* __restrict__ img,
+__global__ void __launch_bounds__(512) k_sur_infer(const f16x8* __restrict__ img,
                                                    const float* __restrict__ imgb, int64_t n,
                                                    const float* __restrict__ X, int sx, float* Y,
                                                    int sy) {
@@ -409,12 +411,18 @@ __global__ void __launch_bounds__(512) k_sur_infer(const bf16x8* __restrict__ im
   const int waves = blockDim.x >> 6;
   const int64_t stride = int64_t(gridDim.x) * waves;
   int64_t tile = int64_t(blockIdx.x) * waves + (threadIdx.x >> 6);
-  // the next tile's inputs are loaded while the current one is evaluated
+  // the next tile's inputs are loaded while the current one is evaluated;
+  // lanes past the end read point 0 (n > 0) and use zeros
   float xn[7];
   auto load = [&](int64_t tl, float* x) {
     const int64_t p = tl * 32 + j;
+    const bool ok = tl < tiles && p < n;
+    const float* src = X + (ok ? p * sx : 0);
 #pragma unroll
-    for (int k = 0; k < 7; ++k) x[k] = (tl < tiles && p < n) ? X[p * sx + k] : 0.f;
+    for (int k = 0; k < 7; ++k) {
+      const float v = src[k];
+      x[k] = ok ? v : 0.f;
+    }
   };
   load(tile, xn);
   for (; tile < tiles; tile += stride) {
@@ -704,7 +712,7 @@ hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int
   // two waves per SIMD, so one wave's SiLU overlaps the other's MFMAs
   const int64_t tiles = (n + 31) / 32;
   const unsigned g = unsigned(std::min<int64_t>(256, (tiles + 7) / 8));
-  k_sur_infer<<<g, 512, 0, s>>>(static_cast<const bf16x8*>(img), imgb, n, x, sx, y, sy);
+  k_sur_infer<<<g, 512, 0, s>>>(static_cast<const f16x8*>(img), imgb, n, x, sx, y, sy);
   return hipGetLastError();
 }
 

@@ -81,6 +81,64 @@ def forward(p, x, rounded=True):
     return Z, cache
 
 
+# Inference form of the network (k_sur_infer): FP16 weights and activations,
+# layers carrying y = z log2(e) and g = 2 log2(e) SiLU(z),
+# g = (y + |y|) + |y| X Q(X), X = 2^-|y|, Q(X) = q0 + ... + q4 X^4 ~ -2 / (1 + X).
+SCALE_IN = np.float32(1.4426950408889634)
+SCALE_HID = np.float32(0.5)
+SCALE_OUT = np.float32(0.34657359027997264)
+SILU_Q = (-1.9990234375, 1.970703125, -1.7568359375, 1.126953125, -0.343994140625)
+
+
+def f16(x):
+    """Round to the nearest FP16 (ties to even), returned as float64."""
+    return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def silu_poly_error(y):
+    """|y| X |Q(X) + 2 / (1 + X)| with X = 2^-|y|: the error of g that the
+    polynomial adds, exact arithmetic."""
+    y = np.abs(np.asarray(y, dtype=np.float64))
+    X = 2.0 ** -y
+    Q = sum(c * X ** k for k, c in enumerate(SILU_Q))
+    return y * X * np.abs(Q + 2.0 / (1.0 + X))
+
+
+def silu_f16(y):
+    """g of the inference kernel for the accumulator values y, in its FP16
+    operations: each operation rounds once to nearest (products and sums of
+    FP16 values are exact in float64 before that rounding)."""
+    y = np.asarray(y, dtype=np.float64)
+    X = f16(np.exp2(-np.abs(y)))
+    Y = f16(y)
+    A = np.abs(Y)
+    Q = np.full_like(X, SILU_Q[-1])
+    for c in SILU_Q[-2::-1]:
+        Q = f16(Q * X + c)
+    AX = f16(A * X)
+    YA = f16(Y + A)
+    return f16(AX * Q + YA)
+
+
+def forward_inference(p, x):
+    """Outputs of the inference kernel's network form (float64 accumulation
+    where the kernel accumulates in FP32)."""
+    P = unpack(p)
+    p32 = {k: np.asarray(v, dtype=np.float32) for k, v in P.items()}
+    x = np.asarray(x, dtype=np.float32)
+    hi = f16(x).astype(np.float32)
+    lo = f16(x - hi)
+    X0 = np.concatenate([hi.astype(np.float64), lo], axis=1)
+    W0 = f16(p32["W0"] * SCALE_IN)
+    Y = X0 @ np.concatenate([W0, W0], axis=1).T + (p32["b0"] * SCALE_IN).astype(np.float64)
+    for layer in (1, 2, 3):
+        G = silu_f16(Y)
+        W = f16(p32["W%d" % layer] * SCALE_HID)
+        Y = G @ W.T + (p32["b%d" % layer] * SCALE_IN).astype(np.float64)
+    G = silu_f16(Y)
+    return G @ f16(p32["W4"] * SCALE_OUT).T + p32["b4"].astype(np.float64)
+
+
 def loss_and_grad(p, x, t, rounded=True):
     """Mean squared error over all outputs and its gradient (flat vector)."""
     r = (lambda a: bf16(a).astype(np.float64)) if rounded else (lambda a: np.asarray(a, np.float64))

@@ -35,18 +35,25 @@ def _rel(a, b):
 
 
 def test_inference_kernel_matches_reference():
-    """Chained 32x32x16 MFMA layers with the permuted weight image give the
-    network of the reference (checks the operand / result lane layout)."""
+    """Chained 32x32x16 FP16 MFMA layers with the permuted, scaled weight
+    image give the reference's inference form of the network (checks the
+    operand / result lane layout, the scale factors and the FP16 SiLU), and
+    the exact FP64 network to FP16 precision."""
     model = _plate_model()
     p = ref.init_params(11).astype(np.float32)
     model.surrogate_params(p)
     assert np.array_equal(model.surrogate_params(), p)
     rng = np.random.default_rng(12)
     x = rng.uniform(-1, 1, size=(1000, 7)).astype(np.float32)  # not a multiple of 32
+    x[:, 6] = np.abs(x[:, 6])  # damage input in [0, 1]
     y = model.surrogate_infer(x)
-    yr, _ = ref.forward(p.astype(np.float64), x, rounded=True)
-    assert _rel(y, yr) < 1e-2
-    assert np.abs(y - yr).max() < 3e-2 * np.abs(yr).max()
+    yi = ref.forward_inference(p.astype(np.float64), x)
+    # FP32 accumulation and the hardware exp2 against float64: an FP16
+    # rounding of an activation may go the other way now and then
+    assert _rel(y, yi) < 5e-4
+    assert np.abs(y - yi).max() < 3e-3 * np.abs(yi).max()
+    ye, _ = ref.forward(p.astype(np.float64), x, rounded=False)
+    assert _rel(y, ye) < 3e-3
 
 
 def test_training_gradients_match_reference():

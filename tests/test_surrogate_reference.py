@@ -72,3 +72,63 @@ def test_neohookean_green_matches_oracle_point_stress():
     Sl, Wl = ref.linear_green(Ev * 1e-4, lam, mu)
     Sn, Wn = ref.neohookean_green(Ev * 1e-4, lam, mu)
     assert np.abs(Sl - Sn).max() <= 1e-3 * np.abs(Sn).max()
+
+
+def test_silu_polynomial_error_bound():
+    """Q stands for -2 / (1 + X) within |y| X |dQ| <= 1.14e-4 over the whole
+    range (the stated bound of the inference kernel's SiLU); a sign or a 1 %
+    slip in one coefficient breaks it (by 2.5x or more)."""
+    y = np.linspace(0.0, 80.0, 800001)
+    assert ref.silu_poly_error(y).max() <= 1.14e-4
+    X = 2.0 ** -y
+    for k in range(5):
+        for bad in (-ref.SILU_Q[k], ref.SILU_Q[k] * 1.01):
+            c = list(ref.SILU_Q)
+            c[k] = bad
+            Q = sum(ck * X ** j for j, ck in enumerate(c))
+            assert (y * X * np.abs(Q + 2.0 / (1.0 + X))).max() > 2.9e-4
+    # every coefficient is an FP16 value (the kernel holds them as such)
+    assert all(float(np.float16(c)) == c for c in ref.SILU_Q)
+
+
+def test_silu_f16_against_silu():
+    """The FP16 evaluation gives g = 2 log2(e) SiLU(z) to FP16 precision:
+    within 1.5e-3 of max(1, |g|) everywhere (4e-4 rms for |z| <= 8), and
+    below 5e-5 in absolute value for z < -8, where SiLU itself is tiny."""
+    z = np.linspace(-60.0, 60.0, 1200001)
+    y = z / np.log(2.0)
+    exact = 2.0 * y / (1.0 + np.exp(-z))
+    g = ref.silu_f16(y)
+    rel = np.abs(g - exact) / np.maximum(1.0, np.abs(exact))
+    near = np.abs(z) <= 8.0
+    assert rel.max() < 1.5e-3
+    assert np.sqrt((rel[near] ** 2).mean()) < 4e-4
+    assert np.abs(g - exact)[z < -8.0].max() < 5e-5
+    assert (ref.silu_f16(np.array([-30.0, -100.0, -1e4])) == 0.0).all()
+
+
+def test_f16_rounding():
+    v = np.array([1.0, 1.0 + 2 ** -11, 1.0 + 3 * 2 ** -11, -(1.0 + 3 * 2 ** -11), 1.0 + 2 ** -12])
+    # ties to even
+    assert list(ref.f16(v)) == [1.0, 1.0, 1.0 + 2 ** -9, -(1.0 + 2 ** -9), 1.0]
+
+
+def test_inference_form_is_the_network():
+    """The scaled FP16 inference form equals the exact FP64 network to FP16
+    precision: the scale factors cancel (log2 e into layer 0, 1/2 per hidden
+    layer, ln 2 / 2 into the output layer)."""
+    rng = np.random.default_rng(8)
+    p = ref.init_params(4).astype(np.float32).astype(np.float64)
+    x = rng.uniform(-1, 1, size=(512, 7)).astype(np.float32)
+    ye, _ = ref.forward(p, x, rounded=False)
+    yi = ref.forward_inference(p, x)
+    yb, _ = ref.forward(p, x, rounded=True)
+    ei = np.linalg.norm(yi - ye) / np.linalg.norm(ye)
+    eb = np.linalg.norm(yb - ye) / np.linalg.norm(ye)
+    assert ei < 2e-3
+    assert ei < eb  # closer than the BF16 training form
+    # a wrong scale (no log2 e on the biases) is caught
+    P = ref.unpack(p)
+    P["b1"] = P["b1"] / float(ref.SCALE_IN)
+    yw = ref.forward_inference(ref.pack(P), x)
+    assert np.linalg.norm(yw - ye) / np.linalg.norm(ye) > 1e-2
