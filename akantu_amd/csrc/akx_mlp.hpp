@@ -1,8 +1,15 @@
 // akx_mlp.hpp -- the constitutive surrogate network on gfx950 matrix cores.
 //
-// Network: 7 -> 128 -> 128 -> 128 -> 128 -> 7, SiLU after every hidden layer,
-// FP32 accumulation and biases; training in BF16 (weights, activations,
-// errors), inference in FP16 (weights, activations; see below).
+// Network: 7 -> 128 -> 128 -> 128 -> 128 -> 7, the smoothstep swish
+//   a(z) = z S(z),  S = t^2 (3 - 2 t),  t = clamp(k z + 1/2, 0, 1),  k = 1/6
+// after every hidden layer (k is the FP16 value next to 1/6, so that the
+// FP16 inference evaluates the same function the training fits); FP32
+// accumulation and biases; training in BF16 (weights, activations, errors),
+// inference in FP16 (weights, activations).  S is a cubic smoothstep from 0 at
+// z = -1/(2k) to 1 at z = 1/(2k), with S(0) = 1/2 and S'(0) = 1/4 as the
+// logistic sigmoid; a is continuously differentiable (S' vanishes at both
+// ends) and needs no transcendental: one clamped fused multiply-add, three
+// multiplies and one multiply-add per value, on packed FP16 in inference.
 //   inputs  x = (E_xx, E_yy, E_zz, E_yz, E_xz, E_xy) / eps_max, damage
 //           (E = Green-Lagrange strain, tensor components, Voigt order of
 //            aka_voigthelper.cc:52-61)
@@ -26,7 +33,7 @@
 //   holds row crow(v, l / 32).
 // Layers are chained without moving data: features are the M (row) index,
 // points the N (column) index, so a layer's C registers are, after bias and
-// SiLU, the next layer's B operand in the K order chain_k; the weights are
+// activation, the next layer's B operand in the K order chain_k; the weights are
 // stored pre-permuted to that order.
 #pragma once
 
@@ -72,16 +79,18 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// sigmoid through the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, about
-// 1 ulp each): the result is rounded to BF16 right after, so an IEEE division
-// (a ten-instruction sequence per value) would buy nothing
-__device__ __forceinline__ float sigmoid(float z) {
-  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+constexpr float ACT_K = 0.1666259765625f;  // the FP16 value next to 1/6
+
+// the activation and its derivative in FP32 (training)
+__device__ __forceinline__ float act(float z) {
+  const float t = fminf(fmaxf(ACT_K * z + 0.5f, 0.f), 1.f);
+  return z * (t * t * (3.f - 2.f * t));
 }
-__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
-__device__ __forceinline__ float silu_grad(float z) {
-  const float s = sigmoid(z);
-  return s * (1.f + z * (1.f - s));
+__device__ __forceinline__ float act_grad(float z) {
+  const float u = ACT_K * z + 0.5f;
+  const float t = fminf(fmaxf(u, 0.f), 1.f);
+  const float dS = (u > 0.f && u < 1.f) ? 6.f * ACT_K * t * (1.f - t) : 0.f;
+  return t * t * (3.f - 2.f * t) + z * dS;
 }
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
@@ -89,37 +98,13 @@ __device__ __forceinline__ float from_bf16(__bf16 x) { return static_cast<float>
 
 // ---------------------------------------------------------------------------
 // Inference (k_sur_infer): FP16 weights and activations on
-// v_mfma_f32_32x32x16_f16, FP32 accumulation, and a SiLU with one
-// transcendental per activation whose tail runs on packed FP16 data.
-//
-// The layers carry y = z log2(e) instead of the pre-activation z, and
-// g = 2 log2(e) SiLU(z) instead of SiLU(z):
-//   g = (y + |y|) + |y| X Q(X),  X = 2^-|y|,  Q(X) ~ -2 / (1 + X),
-// which is y + |y| (1 - X) / (1 + X) = y (1 + tanh(z / 2)) = 2 y sigmoid(z).
-// The scale factors go into the weights: layer 0 W log2(e), hidden layers
-// W / 2 (exact), output layer W ln(2) / 2; biases of layers 0-3 times
-// log2(e), the output bias as is.  2^-|y| is one v_exp_f32 (its -|.| is a
-// source modifier); y and 2^-|y| are rounded in pairs to FP16
-// (v_cvt_pk_f16_f32, to nearest) and the rest is four v_pk_fma_f16 for Q,
-// one bit mask, one v_pk_mul_f16, one v_pk_add_f16 and one v_pk_fma_f16 per
-// pair of activations.  Q is quartic with FP16 coefficients fitted for the
-// least maximum of |y| X |Q(X) + 2 / (1 + X)| over X in [0, 1]: 1.13e-4, the
-// absolute error of g that Q itself adds.  (A cubic, 5.25e-4, is one
-// instruction cheaper but its error, the same for every point where a
-// neuron's input sits near its bias, adds up over a layer into an output
-// offset of a few 1e-3 at small strains.)  For y < 0, y + |y| = 0 exactly
-// and g = |y| X Q(X) keeps its relative precision however small it is.
-// Bias: the MFMA's C operand of a layer's first K step.
+// v_mfma_f32_32x32x16_f16, FP32 accumulation, the bias as the MFMA's C
+// operand of a layer's first K step, and the activation on packed FP16: per
+// pair of values one v_cvt_pk_f16_f32 (to nearest), one v_pk_fma_f16 with the
+// clamp bit for t, and four v_pk_mul_f16 / v_pk_fma_f16.
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-
-constexpr float SCALE_IN = 1.4426950408889634f;     // log2(e), layer 0 weights and biases 0-3
-constexpr float SCALE_HID = 0.5f;                    // hidden layers 1-3
-constexpr float SCALE_OUT = 0.34657359027997264f;    // ln(2) / 2, output layer
-// Q(X) = q0 + q1 X + q2 X^2 + q3 X^3 + q4 X^4 (FP16 values)
-constexpr float SILU_Q0 = -1.9990234375f, SILU_Q1 = 1.970703125f, SILU_Q2 = -1.7568359375f,
-                SILU_Q3 = 1.126953125f, SILU_Q4 = -0.343994140625f;
 
 __device__ __forceinline__ f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -130,21 +115,14 @@ __device__ __forceinline__ f16x2 splat2(float v) {
   return f16x2{h, h};
 }
 
-// g for the two accumulator values y0, y1
-__device__ __forceinline__ f16x2 silu2(float y0, float y1) {
+// a(z) for the two accumulator values z0, z1: Z (S), with S = (t t)(3 - 2 t)
+__device__ __forceinline__ f16x2 act2(float z0, float z1) {
   typedef float f32x2 __attribute__((ext_vector_type(2)));
-  const float x0 = __builtin_amdgcn_exp2f(-__builtin_fabsf(y0));
-  const float x1 = __builtin_amdgcn_exp2f(-__builtin_fabsf(y1));
-  const f16x2 Y = __builtin_convertvector(f32x2{y0, y1}, f16x2);
-  const f16x2 X = __builtin_convertvector(f32x2{x0, x1}, f16x2);
-  const f16x2 A = __builtin_bit_cast(f16x2, __builtin_bit_cast(uint32_t, Y) & 0x7fff7fffu);
-  f16x2 Q = __builtin_elementwise_fma(splat2(SILU_Q4), X, splat2(SILU_Q3));
-  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q2));
-  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q1));
-  Q = __builtin_elementwise_fma(Q, X, splat2(SILU_Q0));
-  const f16x2 AX = A * X;
-  const f16x2 YA = Y + A;
-  return __builtin_elementwise_fma(AX, Q, YA);
+  const f16x2 Z = __builtin_convertvector(f32x2{z0, z1}, f16x2);
+  f16x2 t = __builtin_elementwise_fma(Z, splat2(ACT_K), splat2(0.5f));
+  t = __builtin_elementwise_min(__builtin_elementwise_max(t, splat2(0.f)), splat2(1.f));
+  const f16x2 u = __builtin_elementwise_fma(t, splat2(-2.f), splat2(3.f));
+  return Z * ((t * t) * u);
 }
 
 __device__ __forceinline__ f16x8 join4(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
@@ -153,9 +131,8 @@ __device__ __forceinline__ f16x8 join4(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
   return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-// Weights (scaled, FP16, permuted as the chained layout needs) and biases
-// (FP32, layers 0-3 scaled by log2(e), in feature order) of one network, as
-// staged in LDS by the inference kernel.
+// Weights (FP16, permuted as the chained layout needs) and biases (FP32, in
+// feature order) of one network, as staged in LDS by the inference kernel.
 struct Image {
   f16x8 w[PW_SIZE];
   float b[PB_SIZE];
@@ -180,8 +157,8 @@ __device__ __forceinline__ f32x16 bias_block(const float* B, int layer, int mb, 
 // g of the 16 values of one C row block: act slots of K steps 2 mb (registers
 // 0-7) and 2 mb + 1 (registers 8-15) of the next layer (chain_k order)
 __device__ __forceinline__ void activate_block(const f32x16& a, f16x8& lo, f16x8& hi) {
-  lo = join4(silu2(a[0], a[1]), silu2(a[2], a[3]), silu2(a[4], a[5]), silu2(a[6], a[7]));
-  hi = join4(silu2(a[8], a[9]), silu2(a[10], a[11]), silu2(a[12], a[13]), silu2(a[14], a[15]));
+  lo = join4(act2(a[0], a[1]), act2(a[2], a[3]), act2(a[4], a[5]), act2(a[6], a[7]));
+  hi = join4(act2(a[8], a[9]), act2(a[10], a[11]), act2(a[12], a[13]), act2(a[14], a[15]));
 }
 
 // Row block mb of a hidden layer: its 8 K steps chained on one accumulator,

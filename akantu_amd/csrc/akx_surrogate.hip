@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_sur_batch(const float* __restrict__ X,
 
 // forward layer: Z^T = W H^T + b over a 32 x 32 tile per wave.
 //   Hin  [B x KIN] bf16 (row = point), W [NP x KIN] bf16, rows >= nout zero
-//   ACT:  Zt [nout x B] fp32, Hout [B x nout], HoutT [nout x B] bf16 (SiLU)
+//   ACT:  Zt [nout x B] fp32, Hout [B x nout], HoutT [nout x B] bf16 (activation)
 //   !ACT: Y [B x 8] fp32
 template <int KIN, int NP, bool ACT>
 __global__ void __launch_bounds__(256) k_sur_fwd(const __bf16* __restrict__ Hin,
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_sur_fwd(const __bf16* __restrict__ Hin,
     const float z = acc[v] + bias[r];
     if constexpr (ACT) {
       Zt[int64_t(r) * B + c] = z;
-      const __bf16 hv = to_bf16(silu(z));
+      const __bf16 hv = to_bf16(act(z));
       Hout[c * nout + r] = hv;
       HoutT[int64_t(r) * B + c] = hv;
     } else {
@@ -221,9 +221,9 @@ __global__ void __launch_bounds__(256) k_sur_loss(const float* __restrict__ Y,
   if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// backward through a layer's weights and the previous SiLU:
+// backward through a layer's weights and the previous activation:
 //   dH^T = W^T dZ^T (tile rows: input features 0..127, columns: points),
-//   dZprev = dH * silu'(Zprev), stored bf16 in both layouts.
+//   dZprev = dH * a'(Zprev), stored bf16 in both layouts.
 //   WT [128 x KC] bf16, dZ [B x KC] bf16, Zt_prev [128 x B] fp32
 template <int KC>
 __global__ void __launch_bounds__(256) k_sur_bwd_data(const __bf16* __restrict__ WT,
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_sur_bwd_data(const __bf16* __restrict__
   for (int v = 0; v < 16; ++v) {
     const int r = 32 * mb + crow(v, h);
     const float z = Zt_prev[int64_t(r) * B + c];
-    const __bf16 g = to_bf16(acc[v] * silu_grad(z));
+    const __bf16 g = to_bf16(acc[v] * act_grad(z));
     dZp[c * HID + r] = g;
     dZpT[int64_t(r) * B + c] = g;
   }
@@ -337,8 +337,7 @@ __global__ void __launch_bounds__(256) k_sur_update(LayerDesc L, int nch, float*
 }
 
 // BF16 copies of the master weights (forward layout, transposed layout) and
-// the permuted FP16 inference image, its weights and biases scaled as the
-// inference kernel's SiLU form needs (akx_mlp.hpp)
+// the permuted FP16 inference image
 __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P, __bf16* W0,
                                                      __bf16* W1, __bf16* W2, __bf16* W3,
                                                      __bf16* W4, __bf16* T1, __bf16* T2,
@@ -371,15 +370,14 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
     if (vec < PW_L1) {
       const int mb = vec / 64, k = 8 * h + s, row = 32 * mb + i;
       w = k < NIN ? P[OFF_W0 + row * NIN + k] : k < 2 * NIN ? P[OFF_W0 + row * NIN + k - NIN] : 0.f;
-      w *= SCALE_IN;
     } else if (vec < PW_L4) {
       const int l = (vec - PW_L1) / (4 * 8 * 64), rem = (vec - PW_L1) % (4 * 8 * 64);
       const int mb = rem / (8 * 64), t = (rem / 64) % 8, row = 32 * mb + i;
       const int off = l == 0 ? OFF_W1 : l == 1 ? OFF_W2 : OFF_W3;
-      w = P[off + row * HID + chain_k(t, h, s)] * SCALE_HID;
+      w = P[off + row * HID + chain_k(t, h, s)];
     } else {
       const int t = (vec - PW_L4) / 64;
-      w = i < NOUT ? P[OFF_W4 + i * HID + chain_k(t, h, s)] * SCALE_OUT : 0.f;
+      w = i < NOUT ? P[OFF_W4 + i * HID + chain_k(t, h, s)] : 0.f;
     }
     img[vec][s] = static_cast<_Float16>(w);
   }
@@ -388,7 +386,7 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
     if (idx < 4 * HID) {
       const int l = idx / HID, n = idx % HID;
       const int off = l == 0 ? OFF_B0 : l == 1 ? OFF_B1 : l == 2 ? OFF_B2 : OFF_B3;
-      b = P[off + n] * SCALE_IN;
+      b = P[off + n];
     } else if (idx - 4 * HID < NOUT) {
       b = P[OFF_B4 + idx - 4 * HID];
     }
@@ -709,7 +707,7 @@ hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int
   (void)hipGetLastError();
   if (n <= 0) return hipSuccess;
   // one block of 8 waves per CU (the weight image fills most of the LDS):
-  // two waves per SIMD, so one wave's SiLU overlaps the other's MFMAs
+  // two waves per SIMD, so one wave's activations overlap the other's MFMAs
   const int64_t tiles = (n + 31) / 32;
   const unsigned g = unsigned(std::min<int64_t>(256, (tiles + 7) / 8));
   k_sur_infer<<<g, 512, 0, s>>>(static_cast<const f16x8*>(img), imgb, n, x, sx, y, sy);

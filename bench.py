@@ -790,12 +790,12 @@ def bench_plate_surrogate(args, akx):
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": res["surrogate"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16 MFMA network (f32 accumulation) / f32 element math / f64 nodal state",
+        "dtype": "f16 MFMA network (f32 accumulation; trained in bf16) / f32 element math / f64 nodal state",
         "data": "synthetic",
         "config": {"workload": "plate-surrogate: notched plate 0.1 x 0.1 x 0.02 m, %d "
                                "tetrahedron_4, %d nodes, neo-Hookean + Marigo damage E=1 GPa "
                                "nu=0.3 rho=1200 Sd=5 MPa, Yd ~ 1 MPa Weibull(m=5), tension at "
-                               "20 m/s; 7-128x4-7 SiLU surrogate trained on 10M GPU-generated "
+                               "20 m/s; 7-128x4-7 smoothstep-swish surrogate trained on 10M GPU-generated "
                                "samples" % (E, N),
                    "elements": E, "nodes": N, "qp_per_element": Q, "parallelism": "1 GPU"},
         "plain": res["plain"], "surrogate": res["surrogate"],

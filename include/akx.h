@@ -262,8 +262,10 @@ int akx_query(akx_ctx* ctx, const char* key, double* value);
 int akx_autotune(akx_ctx* ctx, int precision, int nsteps, int* chosen, double* best_ms);
 
 /* ---- constitutive surrogate -------------------------------------------------
- * A 7 -> 128 x 4 -> 7 SiLU network (BF16 weights and activations, FP32
- * accumulation, gfx950 MFMA) that stands in for the neo-Hookean part of a
+ * A 7 -> 128 x 4 -> 7 network with the smoothstep swish activation
+ * a(z) = z S(z), S = t^2 (3 - 2 t), t = clamp(z / 6 + 1/2, 0, 1) (trained in
+ * BF16, evaluated in FP16, FP32 accumulation, gfx950 MFMA) that stands in for
+ * the neo-Hookean part of a
  * neohookean_damage material (material_neohookean_inline_impl.hh:46-65,
  * 145-158); the Marigo damage update (material_marigo_inline_impl.hh:67-76)
  * stays closed-form because it needs each point's own threshold Yd.

@@ -49,13 +49,21 @@ def init_params(seed=0):
     return pack(d)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+# The smoothstep swish a(z) = z S(z), S = t^2 (3 - 2 t), t = clamp(k z + 1/2, 0, 1),
+# k the FP16 value next to 1/6.
+ACT_K = 0.1666259765625
 
 
-def silu_grad(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def act(z):
+    t = np.clip(ACT_K * z + 0.5, 0.0, 1.0)
+    return z * (t * t * (3.0 - 2.0 * t))
+
+
+def act_grad(z):
+    u = ACT_K * z + 0.5
+    t = np.clip(u, 0.0, 1.0)
+    dS = np.where((u > 0.0) & (u < 1.0), 6.0 * ACT_K * t * (1.0 - t), 0.0)
+    return t * t * (3.0 - 2.0 * t) + z * dS
 
 
 def forward(p, x, rounded=True):
@@ -74,7 +82,7 @@ def forward(p, x, rounded=True):
     Z = X0 @ W0.T + P["b0"]
     for layer in range(1, 5):
         cache["Z"].append(Z)
-        H = r(silu(Z))
+        H = r(act(Z))
         cache["H"].append(H)
         W = r(P["W%d" % layer])
         Z = H @ W.T + P["b%d" % layer]
@@ -82,12 +90,7 @@ def forward(p, x, rounded=True):
 
 
 # Inference form of the network (k_sur_infer): FP16 weights and activations,
-# layers carrying y = z log2(e) and g = 2 log2(e) SiLU(z),
-# g = (y + |y|) + |y| X Q(X), X = 2^-|y|, Q(X) = q0 + ... + q4 X^4 ~ -2 / (1 + X).
-SCALE_IN = np.float32(1.4426950408889634)
-SCALE_HID = np.float32(0.5)
-SCALE_OUT = np.float32(0.34657359027997264)
-SILU_Q = (-1.9990234375, 1.970703125, -1.7568359375, 1.126953125, -0.343994140625)
+# FP32 accumulation, the activation in FP16 operations.
 
 
 def f16(x):
@@ -95,48 +98,32 @@ def f16(x):
     return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
 
 
-def silu_poly_error(y):
-    """|y| X |Q(X) + 2 / (1 + X)| with X = 2^-|y|: the error of g that the
-    polynomial adds, exact arithmetic."""
-    y = np.abs(np.asarray(y, dtype=np.float64))
-    X = 2.0 ** -y
-    Q = sum(c * X ** k for k, c in enumerate(SILU_Q))
-    return y * X * np.abs(Q + 2.0 / (1.0 + X))
-
-
-def silu_f16(y):
-    """g of the inference kernel for the accumulator values y, in its FP16
-    operations: each operation rounds once to nearest (products and sums of
-    FP16 values are exact in float64 before that rounding)."""
-    y = np.asarray(y, dtype=np.float64)
-    X = f16(np.exp2(-np.abs(y)))
-    Y = f16(y)
-    A = np.abs(Y)
-    Q = np.full_like(X, SILU_Q[-1])
-    for c in SILU_Q[-2::-1]:
-        Q = f16(Q * X + c)
-    AX = f16(A * X)
-    YA = f16(Y + A)
-    return f16(AX * Q + YA)
+def act_f16(z):
+    """a(z) as the inference kernel evaluates it from the accumulator values
+    z: Z = FP16(z), t = clamp(FP16(Z k + 1/2)), u = FP16(t (-2) + 3),
+    a = FP16(Z FP16(FP16(t t) u)); each operation rounds once to nearest
+    (products and sums of FP16 values are exact in float64 before that)."""
+    Z = f16(z)
+    t = np.clip(f16(Z * ACT_K + 0.5), 0.0, 1.0)
+    u = f16(t * -2.0 + 3.0)
+    return f16(Z * f16(f16(t * t) * u))
 
 
 def forward_inference(p, x):
-    """Outputs of the inference kernel's network form (float64 accumulation
-    where the kernel accumulates in FP32)."""
+    """Outputs of the inference kernel's network (float64 accumulation where
+    the kernel accumulates in FP32)."""
     P = unpack(p)
     p32 = {k: np.asarray(v, dtype=np.float32) for k, v in P.items()}
     x = np.asarray(x, dtype=np.float32)
     hi = f16(x).astype(np.float32)
     lo = f16(x - hi)
     X0 = np.concatenate([hi.astype(np.float64), lo], axis=1)
-    W0 = f16(p32["W0"] * SCALE_IN)
-    Y = X0 @ np.concatenate([W0, W0], axis=1).T + (p32["b0"] * SCALE_IN).astype(np.float64)
-    for layer in (1, 2, 3):
-        G = silu_f16(Y)
-        W = f16(p32["W%d" % layer] * SCALE_HID)
-        Y = G @ W.T + (p32["b%d" % layer] * SCALE_IN).astype(np.float64)
-    G = silu_f16(Y)
-    return G @ f16(p32["W4"] * SCALE_OUT).T + p32["b4"].astype(np.float64)
+    W0 = f16(p32["W0"])
+    Z = X0 @ np.concatenate([W0, W0], axis=1).T + p32["b0"].astype(np.float64)
+    for layer in (1, 2, 3, 4):
+        H = act_f16(Z)
+        Z = H @ f16(p32["W%d" % layer]).T + p32["b%d" % layer].astype(np.float64)
+    return Z
 
 
 def loss_and_grad(p, x, t, rounded=True):
@@ -158,7 +145,7 @@ def loss_and_grad(p, x, t, rounded=True):
             break
         W = r(P["W%d" % layer])
         dH = dZ @ W
-        dZ = r(dH * silu_grad(c["Z"][layer - 1]))
+        dZ = r(dH * act_grad(c["Z"][layer - 1]))
     if rounded:
         g["W0"] = g["W0"][:, :NIN] + g["W0"][:, NIN:]
     return loss, pack(g)

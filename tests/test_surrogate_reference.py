@@ -74,37 +74,48 @@ def test_neohookean_green_matches_oracle_point_stress():
     assert np.abs(Sl - Sn).max() <= 1e-3 * np.abs(Sn).max()
 
 
-def test_silu_polynomial_error_bound():
-    """Q stands for -2 / (1 + X) within |y| X |dQ| <= 1.14e-4 over the whole
-    range (the stated bound of the inference kernel's SiLU); a sign or a 1 %
-    slip in one coefficient breaks it (by 2.5x or more)."""
-    y = np.linspace(0.0, 80.0, 800001)
-    assert ref.silu_poly_error(y).max() <= 1.14e-4
-    X = 2.0 ** -y
-    for k in range(5):
-        for bad in (-ref.SILU_Q[k], ref.SILU_Q[k] * 1.01):
-            c = list(ref.SILU_Q)
-            c[k] = bad
-            Q = sum(ck * X ** j for j, ck in enumerate(c))
-            assert (y * X * np.abs(Q + 2.0 / (1.0 + X))).max() > 2.9e-4
-    # every coefficient is an FP16 value (the kernel holds them as such)
-    assert all(float(np.float16(c)) == c for c in ref.SILU_Q)
+def test_activation_shape():
+    """The smoothstep swish: a(0) = 0, a'(0) = 1/2 as SiLU, 0 for z <= -1/(2k),
+    z for z >= 1/(2k), and continuously differentiable: its derivative agrees
+    with central differences everywhere, also across both knots."""
+    k = ref.ACT_K
+    assert float(np.float16(k)) == k and abs(k - 1.0 / 6.0) < 1e-4
+    knot = 0.5 / k
+    z = np.concatenate([np.linspace(-10, 10, 20001), [-knot, knot]])
+    assert ref.act(np.array([0.0]))[0] == 0.0
+    assert abs(ref.act_grad(np.array([0.0]))[0] - 0.5) < 1e-15
+    assert (ref.act(z[z <= -knot]) == 0.0).all()
+    assert np.allclose(ref.act(z[z >= knot]), z[z >= knot], rtol=0, atol=1e-12)
+    def fd_error(f, df):
+        h = 1e-6
+        return np.abs((f(z + h) - f(z - h)) / (2 * h) - df(z)).max()
+
+    assert fd_error(ref.act, ref.act_grad) < 1e-6
+    # the derivative is continuous: no jump across the knots
+    for zk in (-knot, knot):
+        assert abs(ref.act_grad(np.array([zk - 1e-9]))[0] - ref.act_grad(np.array([zk + 1e-9]))[0]) < 1e-8
+    # planted slips fail: a derivative without the z S'(z) term, and a
+    # quadratic S (t^2) whose derivative jumps at the upper knot
+    def grad_no_dS(zz):
+        t = np.clip(k * zz + 0.5, 0.0, 1.0)
+        return t * t * (3.0 - 2.0 * t)
+    assert fd_error(ref.act, grad_no_dS) > 0.1
+    quad = lambda zz: zz * np.clip(k * zz + 0.5, 0.0, 1.0) ** 2
+    above = (quad(knot + 2e-6) - quad(knot + 1e-6)) / 1e-6
+    below = (quad(knot - 1e-6) - quad(knot - 2e-6)) / 1e-6
+    assert abs(above - below) > 0.5
 
 
-def test_silu_f16_against_silu():
-    """The FP16 evaluation gives g = 2 log2(e) SiLU(z) to FP16 precision:
-    within 1.5e-3 of max(1, |g|) everywhere (4e-4 rms for |z| <= 8), and
-    below 5e-5 in absolute value for z < -8, where SiLU itself is tiny."""
+def test_act_f16_against_act():
+    """The FP16 evaluation gives a(z) within 2.5e-3 of max(1, |a|) (about 2.5
+    FP16 steps; the rounding of z to FP16 alone moves a by up to
+    max|a'| x 2^-11 |z|)."""
     z = np.linspace(-60.0, 60.0, 1200001)
-    y = z / np.log(2.0)
-    exact = 2.0 * y / (1.0 + np.exp(-z))
-    g = ref.silu_f16(y)
-    rel = np.abs(g - exact) / np.maximum(1.0, np.abs(exact))
-    near = np.abs(z) <= 8.0
-    assert rel.max() < 1.5e-3
-    assert np.sqrt((rel[near] ** 2).mean()) < 4e-4
-    assert np.abs(g - exact)[z < -8.0].max() < 5e-5
-    assert (ref.silu_f16(np.array([-30.0, -100.0, -1e4])) == 0.0).all()
+    a = ref.act(z)
+    g = ref.act_f16(z)
+    rel = np.abs(g - a) / np.maximum(1.0, np.abs(a))
+    assert rel.max() < 2.5e-3
+    assert (g[z <= -0.5 / ref.ACT_K - 0.1] == 0.0).all()
 
 
 def test_f16_rounding():
@@ -114,9 +125,8 @@ def test_f16_rounding():
 
 
 def test_inference_form_is_the_network():
-    """The scaled FP16 inference form equals the exact FP64 network to FP16
-    precision: the scale factors cancel (log2 e into layer 0, 1/2 per hidden
-    layer, ln 2 / 2 into the output layer)."""
+    """The FP16 inference form equals the exact FP64 network to FP16
+    precision, closer than the BF16 training form."""
     rng = np.random.default_rng(8)
     p = ref.init_params(4).astype(np.float32).astype(np.float64)
     x = rng.uniform(-1, 1, size=(512, 7)).astype(np.float32)
@@ -127,8 +137,4 @@ def test_inference_form_is_the_network():
     eb = np.linalg.norm(yb - ye) / np.linalg.norm(ye)
     assert ei < 2e-3
     assert ei < eb  # closer than the BF16 training form
-    # a wrong scale (no log2 e on the biases) is caught
-    P = ref.unpack(p)
-    P["b1"] = P["b1"] / float(ref.SCALE_IN)
-    yw = ref.forward_inference(ref.pack(P), x)
-    assert np.linalg.norm(yw - ye) / np.linalg.norm(ye) > 1e-2
+
