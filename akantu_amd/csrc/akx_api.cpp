@@ -301,7 +301,6 @@ bool uses_partials(const akx_ctx* c, int precision) {
          !uses_fused(c, precision) &&
          c->patch_ok && c->groups.size() == 1 &&
          !c->groups[0].filter &&
-         !(c->sur.enabled && c->groups[0].kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) &&
          // the partials (padded stride) live in the slot buffer
          c->patch_blocks * int64_t(c->patch_max_nodes) <= c->E * int64_t(c->n);
 }
@@ -359,8 +358,6 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
     A.sur_fallbacks = c->sur.fallbacks;
     A.sur_x = c->sur.pt_x;
     A.sur_y = c->sur.pt_y;
-    A.patch = 0;
-    A.partials = 0;
   }
   return A;
 }

@@ -392,6 +392,26 @@ __device__ __forceinline__ void hex8_mixed_forces(const ElemArgs& A, int64_t e, 
     for (int i = 0; i < 3; ++i) fe[i + a * 3] = F[corner[a]][i];
 }
 
+// Kind of the mixed element kernels for neo-Hookean + damage through the
+// constitutive surrogate (tetrahedron_4): the point's neo-Hookean stress and
+// energy come from the network's outputs (sur_y) when its strain is inside
+// the envelope (flag in sur_x), from the plain FP32 law otherwise.
+constexpr int NH_DAMAGE_SUR = 3;
+
+// Green-Lagrange strain E = (G + G^T + G^T G) / 2, Voigt xx yy zz yz xz xy
+__device__ __forceinline__ void green_voigt(const float* G, float* Ev) {
+#pragma clang fp contract(fast)
+  const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = vi[k], b = vj[k];
+    float gtg = 0.f;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) gtg += G[l + a * 3] * G[l + b * 3];
+    Ev[k] = 0.5f * (G[a + b * 3] + G[b + a * 3] + gtg);
+  }
+}
+
 template <int ET, int KIND, bool RECOMPUTE>
 __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t e,
                                                      const MatParams& m, const float* du,
@@ -481,8 +501,10 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
   // 128-VGPR cap).  For the other laws the B form is the faster one (320 vs
   // 335 us on the 128^3 neo-Hookean hexahedron mesh), though it needs more
   // multiply-adds.
-  constexpr bool on_the_fly = RECOMPUTE && !simplex && KIND == NH_DAMAGE;
-  constexpr int QUNROLL = KIND == NH_DAMAGE ? 1 : Q;
+  constexpr bool damage = KIND == NH_DAMAGE || KIND == NH_DAMAGE_SUR;
+  static_assert(KIND != NH_DAMAGE_SUR || ET == TET4, "the surrogate is for tetrahedron_4");
+  constexpr bool on_the_fly = RECOMPUTE && !simplex && damage;
+  constexpr int QUNROLL = damage ? 1 : Q;
 #pragma unroll QUNROLL
   for (int q = 0; q < Q; ++q) {
     float Bq[on_the_fly ? 1 : dn];
@@ -602,8 +624,29 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
     } else if constexpr (d == 3) {
       float S6[6], W;
       float scale = jq;
-      neohookean_stress_sym3_f32(m.lambda_f, m.mu_f, G, 1.f, S6, KIND == NH_DAMAGE ? &W : nullptr);
-      if constexpr (KIND == NH_DAMAGE) {
+      if constexpr (KIND == NH_DAMAGE_SUR) {
+        // the surrogate's point index is the element's (one point, no filter)
+        if (A.sur_x[e * 8 + 7] != 0.f) {
+          // S = S_lin + sS y[0..5], W = W_lin + sW y[6] (Voigt xx yy zz yz xz xy)
+          float Ev[6];
+          green_voigt(G, Ev);
+          const float4* yv = reinterpret_cast<const float4*>(A.sur_y + e * 8);
+          const float4 y0 = yv[0], y1 = yv[1];
+          const float y[7] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z};
+          const float tr = Ev[0] + Ev[1] + Ev[2];
+          const float ee = Ev[0] * Ev[0] + Ev[1] * Ev[1] + Ev[2] * Ev[2] +
+                           2.f * (Ev[3] * Ev[3] + Ev[4] * Ev[4] + Ev[5] * Ev[5]);
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            S6[k] = 2.f * m.mu_f * Ev[k] + (k < 3 ? m.lambda_f * tr : 0.f) + A.sur_sS * y[k];
+          W = 0.5f * m.lambda_f * tr * tr + m.mu_f * ee + A.sur_sW * y[6];
+        } else {
+          neohookean_stress_sym3_f32(m.lambda_f, m.mu_f, G, 1.f, S6, &W);
+        }
+      } else {
+        neohookean_stress_sym3_f32(m.lambda_f, m.mu_f, G, 1.f, S6, KIND == NH_DAMAGE ? &W : nullptr);
+      }
+      if constexpr (damage) {
         const int64_t eq = e * Q + q;
         double dam = A.damage[eq];
         const double Fd = double(W) - A.Yd[eq] - m.Sd * dam;
@@ -730,10 +773,13 @@ __global__ void __launch_bounds__(256) k_element_mixed(ElemArgs A) {
 //                 inputs and the envelope flag (|E_k| <= eps_max, damage in
 //                 [0, 1]) to sur_x;
 //   network       persistent MFMA blocks, sur_x -> sur_y (akx_surrogate.hip);
-//   k_sur_forces  one thread per element: S_NH, W_NH from the network (or the
-//                 plain FP32 law outside the envelope, counted), Marigo update,
-//                 B^T (F S) det(J) w into the force slots.
-// The element gathers run at full occupancy; the network kernel streams its
+//   forces        S_NH, W_NH from the network (or the plain FP32 law outside
+//                 the envelope, counted by k_sur_strain), Marigo update,
+//                 B^T (F S) det(J) w: with the LDS patches, the patch element
+//                 kernel of kind NH_DAMAGE_SUR (per-patch partial sums or
+//                 contiguous runs, as for the plain law); without them
+//                 k_sur_forces, one thread per element, into the force slots.
+// The strain gathers run at full occupancy; the network kernel streams its
 // inputs.
 // ===========================================================================
 struct SurPoint {
@@ -803,23 +849,9 @@ __device__ __forceinline__ void sur_point(const ElemArgs& A, int64_t e, SurPoint
     }
 }
 
-// Green-Lagrange strain E = (G + G^T + G^T G) / 2, Voigt xx yy zz yz xz xy
-__device__ __forceinline__ void green_voigt(const float* G, float* Ev) {
-#pragma clang fp contract(fast)
-  const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int a = vi[k], b = vj[k];
-    float gtg = 0.f;
-#pragma unroll
-    for (int l = 0; l < 3; ++l) gtg += G[l + a * 3] * G[l + b * 3];
-    Ev[k] = 0.5f * (G[a + b * 3] + G[b + a * 3] + gtg);
-  }
-}
-
-__global__ void __launch_bounds__(256) k_sur_strain(ElemArgs A) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= A.count) return;
+// Network inputs of point t (scaled Green strain, damage) and its envelope
+// flag to sur_x; returns the flag.
+__device__ __forceinline__ bool sur_strain_point(const ElemArgs& A, int64_t t) {
   const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
   SurPoint P;
   sur_point(A, e, P);
@@ -839,15 +871,26 @@ __global__ void __launch_bounds__(256) k_sur_strain(ElemArgs A) {
   float4* o = reinterpret_cast<float4*>(A.sur_x + t * 8);
   o[0] = make_float4(x[0], x[1], x[2], x[3]);
   o[1] = make_float4(x[4], x[5], x[6], x[7]);
+  return inside;
 }
+
+__global__ void __launch_bounds__(256) k_sur_strain(ElemArgs A) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = t < A.count;
+  // points outside the envelope take the plain law: count them, one atomic
+  // per wave
+  bool outside = false;
+  if (live) outside = !sur_strain_point(A, t);
+  const unsigned long long wave_fb = __popcll(__ballot(outside));
+  if ((threadIdx.x & 63) == 0 && wave_fb) atomicAdd(A.sur_fallbacks, wave_fb);
+}
+
 
 __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
 #pragma clang fp contract(fast)
   constexpr int d = 3, n = 4, dn = 12;
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool live = t < A.count;
-  unsigned long long fallback = 0;
-  if (live) {
+  if (t < A.count) {
     const int64_t e = A.filter ? int64_t(A.filter[t]) : t;
     const MatParams m = A.mats[A.elem_mat ? A.elem_mat[e] : 0];
     SurPoint P;
@@ -874,7 +917,6 @@ __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
       W = 0.5f * m.lambda_f * tr * tr + m.mu_f * ee + A.sur_sW * y[6];
     } else {
       neohookean_stress_f32<d>(m.lambda_f, m.mu_f, G, S, &W);
-      fallback = 1;
     }
     // Marigo update (material_marigo_inline_impl.hh:67-76) with Y = W
     double dam = A.damage[e];
@@ -905,9 +947,6 @@ __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
       }
     store_slots<d, n>(A, e, fe);
   }
-  // count the fallbacks of the wave with one atomic
-  const unsigned long long wave_fb = __popcll(__ballot(fallback != 0));
-  if ((threadIdx.x & 63) == 0 && wave_fb) atomicAdd(A.sur_fallbacks, wave_fb);
 }
 
 // ===========================================================================
@@ -1892,7 +1931,10 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
       err = launch_sur_mlp(s, A.sur_img, A.sur_b, A.count, A.sur_x, 8, A.sur_y, 8);
       if (err != hipSuccess) return err;
       if (A.net_ev1) (void)hipEventRecord(A.net_ev1, s);
-      k_sur_forces<<<ge, 256, 0, s>>>(A);
+      if (A.patch)
+        k_element_patch<TET4, NH_DAMAGE_SUR><<<unsigned(A.patch_blocks), 256, A.patch_lds, s>>>(A);
+      else
+        k_sur_forces<<<ge, 256, 0, s>>>(A);
     } else {
       return hipErrorInvalidValue;
     }
