@@ -949,6 +949,65 @@ __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
   }
 }
 
+// Kind of the patch element kernel that only writes the surrogate's inputs:
+// the Green strain of each tetrahedron_4 point from the patch's staged node
+// differences (no forces).
+constexpr int SUR_STRAIN = 4;
+
+// Network inputs of element e (scaled Green strain, damage) and its envelope
+// flag to sur_x, from its node differences to node 0 (du, dX, FP32); the
+// gradient with the arithmetic of mixed_element_forces<TET4, ..., true>.
+// Returns the flag.
+__device__ __forceinline__ bool sur_strain_from_differences(const ElemArgs& A, int64_t e,
+                                                            const float* du, const float* dX) {
+#pragma clang fp contract(fast)
+  constexpr int d = 3, n = 4;
+  float J[9];
+#pragma unroll
+  for (int j = 0; j < d; ++j)
+#pragma unroll
+    for (int i = 0; i < d; ++i) J[i + j * d] = dX[j + (i + 1) * d];
+  float Ad[9];
+  Ad[0] = J[4] * J[8] - J[7] * J[5];
+  Ad[1] = J[2] * J[7] - J[8] * J[1];
+  Ad[2] = J[1] * J[5] - J[4] * J[2];
+  Ad[3] = J[5] * J[6] - J[8] * J[3];
+  Ad[4] = J[0] * J[8] - J[6] * J[2];
+  Ad[5] = J[2] * J[3] - J[5] * J[0];
+  Ad[6] = J[3] * J[7] - J[6] * J[4];
+  Ad[7] = J[1] * J[6] - J[7] * J[0];
+  Ad[8] = J[0] * J[4] - J[3] * J[1];
+  const float det = J[0] * Ad[0] + J[3] * Ad[1] + J[6] * Ad[2];
+  const float idet = __builtin_amdgcn_rcpf(det);
+  float G[9];
+#pragma unroll
+  for (int j = 0; j < d; ++j)
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      float g = 0.f;
+#pragma unroll
+      for (int a = 1; a < n; ++a) g += du[i + a * d] * (Ad[j + (a - 1) * d] * idet);
+      G[i + j * d] = g;
+    }
+  float Ev[6];
+  green_voigt(G, Ev);
+  const double dam = A.damage[e];
+  bool inside = dam >= 0. && dam <= 1.;
+  const float ieps = 1.f / A.sur_eps;
+  float x[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    inside = inside && fabsf(Ev[k]) <= A.sur_eps;
+    x[k] = Ev[k] * ieps;
+  }
+  x[6] = float(dam);
+  x[7] = inside ? 1.f : 0.f;
+  float4* o = reinterpret_cast<float4*>(A.sur_x + e * 8);
+  o[0] = make_float4(x[0], x[1], x[2], x[3]);
+  o[1] = make_float4(x[4], x[5], x[6], x[7]);
+  return inside;
+}
+
 // ===========================================================================
 // element kernel, mixed precision, LDS-staged element patches
 //   One block = one patch of up to PATCH consecutive (Morton-ordered)
@@ -1040,13 +1099,16 @@ k_element_patch(ElemArgs A) {
     float2* dst = reinterpret_cast<float2*>(s_ux + size_t(k) * 2 * d);
 #pragma unroll
     for (int j = 0; j < d; ++j) dst[j] = make_float2(v[2 * j], v[2 * j + 1]);
-    // run start (layout 2) or the partial's node-major position (layout 5)
-    if (!A.partials) s_rs[k] = A.run_start[k0 + k];
-    else if (A.part_pos) s_rs[k] = A.part_pos[k0 + k];
-    s_rb[k] = A.run_base[b * int64_t(S + 1) + k];
+    if constexpr (KIND != SUR_STRAIN) {
+      // run start (layout 2) or the partial's node-major position (layout 5)
+      if (!A.partials) s_rs[k] = A.run_start[k0 + k];
+      else if (A.part_pos) s_rs[k] = A.part_pos[k0 + k];
+      s_rb[k] = A.run_base[b * int64_t(S + 1) + k];
+    }
   }
-  if (threadIdx.x == 0) s_rb[S] = A.run_base[b * int64_t(S + 1) + S];
+  if (KIND != SUR_STRAIN && threadIdx.x == 0) s_rb[S] = A.run_base[b * int64_t(S + 1) + S];
   __syncthreads();
+  bool outside = false;  // surrogate strain pass: point outside the envelope
   if (t < ne) {
     const MatParams m = A.mats[mat];
     auto node_ux = [&](int k, float* v) {
@@ -1073,14 +1135,25 @@ k_element_patch(ElemArgs A) {
         dX[i + a * d] = v[d + i] - v0[d + i];
       }
     }
-    float fe[dn];
-    mixed_element_forces<ET, KIND, true>(A, e, m, du, dX, fe);
+    if constexpr (KIND == SUR_STRAIN) {
+      outside = !sur_strain_from_differences(A, e, du, dX);
+    } else {
+      float fe[dn];
+      mixed_element_forces<ET, KIND, true>(A, e, m, du, dX, fe);
 #pragma unroll
-    for (int a = 0; a < n; ++a) {
-      const int q = s_rb[ln[a]] + rk[a];
+      for (int a = 0; a < n; ++a) {
+        const int q = s_rb[ln[a]] + rk[a];
 #pragma unroll
-      for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
+        for (int i = 0; i < d; ++i) s_f[q * d + i] = fe[i + a * d];
+      }
     }
+  }
+  if constexpr (KIND == SUR_STRAIN) {
+    // the network's inputs are written: count the points outside the
+    // envelope, one atomic per wave; no forces in this pass
+    const unsigned long long wave_fb = __popcll(__ballot(outside));
+    if ((threadIdx.x & 63) == 0 && wave_fb) atomicAdd(A.sur_fallbacks, wave_fb);
+    return;
   }
   __syncthreads();
   if (A.partials) {
@@ -1924,7 +1997,10 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
     if constexpr (ET == TET4) {
       if (kind != NH_DAMAGE) return hipErrorInvalidValue;
       const unsigned ge = grid_for(A.count, 256);
-      k_sur_strain<<<ge, 256, 0, s>>>(A);
+      if (A.patch)
+        k_element_patch<TET4, SUR_STRAIN><<<unsigned(A.patch_blocks), 256, A.patch_lds, s>>>(A);
+      else
+        k_sur_strain<<<ge, 256, 0, s>>>(A);
       hipError_t err = hipGetLastError();
       if (err != hipSuccess) return err;
       if (A.net_ev0) (void)hipEventRecord(A.net_ev0, s);
