@@ -1685,7 +1685,7 @@ __device__ __forceinline__ double block_reduce(double v, Op op, double init) {
   if (lane == 0) sh[wid] = v;
   __syncthreads();
   const int nw = (blockDim.x + 63) >> 6;
-  v = (threadIdx.x < nw) ? sh[threadIdx.x] : init;
+  v = (int(threadIdx.x) < nw) ? sh[threadIdx.x] : init;
   if (wid == 0) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_down(v, o, 64));
