@@ -179,6 +179,76 @@ def test_hex8_damage_law():
     assert gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u")) <= 1e-12
 
 
+@pytest.mark.parametrize("layout", [1, 5])
+def test_hex8_damage_law_mixed(layout):
+    """The same damage run on the mixed path (FP32 finite-strain law of the
+    8-point hexahedron, FP64 Marigo update): displacements within 1e-5 of the
+    oracle after 150 steps; the damaged / undamaged state of every point
+    agrees with the oracle except on at most 0.5 % of the points (those
+    whose driving energy sits within the FP32 error of their threshold), and
+    the damage itself is within 1e-4."""
+    m = akm.cube_hex8(5)
+    rng = np.random.default_rng(3)
+    Yd = rng.weibull(5.0, size=m.nb_elements * 8) * 5e3
+    p = gh.hex_pair(5, kind=akx.NEOHOOKEAN_DAMAGE, Yd=Yd, Sd=1e5, speed=0.5)
+    p.gpu.set_launch(slot_layout=layout)
+    p.gpu.solve_step(150, akx.MIXED)
+    gh.oracle_hex_step(p, 150)
+    dg, dc = p.gpu.get("damage"), p.cpu.field("damage")
+    assert (dc > 0).any()
+    assert np.mean((dg > 0) != (dc > 0)) <= 5e-3
+    assert np.abs(dg - dc).max() <= 1e-4
+    assert gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u")) <= 1e-5
+
+
+def _damage_pair_2d(kind_mesh):
+    """Plane-strain rectangle (triangle_3 or quadrangle_4), neo-Hookean +
+    damage with per-point Weibull thresholds, stretched by an initial
+    displacement field of ~2 % strain and a velocity field."""
+    if kind_mesh == "tri3":
+        m = akm.rectangle_tri3(24, 12, 0.2, 0.1)
+    else:
+        m = akm.rectangle_quad4(24, 12, 0.2, 0.1)
+    Q = 1 if kind_mesh == "tri3" else 4
+    rng = np.random.default_rng(7)
+    # 40 % (tri3) / 66 % (quad4) of the points damaged after 100 steps, none broken
+    Yd = rng.weibull(5.0, size=m.nb_elements * Q) * 6e5
+    p = gh.Pair(m, [dict(kind=akx.NEOHOOKEAN_DAMAGE, E=1e9, nu=0.3, rho=1200.0, Sd=1e7, Yd=Yd)])
+    X = m.nodes
+    u = np.stack([0.02 * X[:, 0] * (1 + 5 * X[:, 1]), -0.006 * X[:, 1] + 0.01 * X[:, 0] ** 2], axis=1)
+    v = np.stack([np.sin(30 * X[:, 1]), np.cos(20 * X[:, 0])], axis=1)
+    p.set_nodal("displacement", u)
+    p.set_nodal("velocity", v)
+    mask = np.zeros_like(X, dtype=bool)
+    mask[np.isclose(X[:, 0], 0.0)] = True
+    p.block(mask)
+    p.set_time_step(0.8 * p.cpu.stable_time_step())
+    return p
+
+
+@pytest.mark.parametrize("kind_mesh", ["tri3", "quad4"])
+def test_damage_law_2d(kind_mesh):
+    """The damage law in plane strain on triangle_3 and quadrangle_4: FP64
+    equals the oracle (<= 1e-12, same damage branch on every point); the
+    mixed path on its default launch is within 1e-5 in displacement, 1e-4
+    in damage, with at most 0.5 % of the points on the other side of their
+    threshold."""
+    p = _damage_pair_2d(kind_mesh)
+    p.step(100, akx.FP64)
+    dg, dc = p.gpu.get("damage"), p.cpu.field("damage")
+    assert (dc > 0).any() and (dc == 0).any()
+    assert np.array_equal(dg > 0, dc > 0)
+    assert np.abs(dg - dc).max() <= 1e-12
+    assert gh.rel(p.gpu.get_displacement(), p.cpu.nodal("u")) <= 1e-12
+    q = _damage_pair_2d(kind_mesh)
+    q.gpu.solve_step(100, akx.MIXED)
+    q.cpu.solve_step(100)
+    dg, dc = q.gpu.get("damage"), q.cpu.field("damage")
+    assert np.mean((dg > 0) != (dc > 0)) <= 5e-3
+    assert np.abs(dg - dc).max() <= 1e-4
+    assert gh.rel(q.gpu.get_displacement(), q.cpu.nodal("u")) <= 1e-5
+
+
 def test_two_materials_assembly_order():
     """Two materials interleaved by element parity: the reference assembles
     material by material (material.cc:190-237), so the FP64 sums must follow
