@@ -625,13 +625,14 @@ __device__ __forceinline__ void mixed_element_forces(const ElemArgs& A, int64_t 
       float S6[6], W;
       float scale = jq;
       if constexpr (KIND == NH_DAMAGE_SUR) {
-        // the surrogate's point index is the element's (one point, no filter)
-        if (A.sur_x[e * 8 + 7] != 0.f) {
+        // the surrogate's point index is the element's (one point, no filter);
+        // the network's outputs, and in their 8th float the envelope flag
+        const float4* yv = reinterpret_cast<const float4*>(A.sur_y + e * 8);
+        const float4 y0 = yv[0], y1 = yv[1];
+        if (y1.w != 0.f) {
           // S = S_lin + sS y[0..5], W = W_lin + sW y[6] (Voigt xx yy zz yz xz xy)
           float Ev[6];
           green_voigt(G, Ev);
-          const float4* yv = reinterpret_cast<const float4*>(A.sur_y + e * 8);
-          const float4 y0 = yv[0], y1 = yv[1];
           const float y[7] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z};
           const float tr = Ev[0] + Ev[1] + Ev[2];
           const float ee = Ev[0] * Ev[0] + Ev[1] * Ev[1] + Ev[2] * Ev[2] +
@@ -896,7 +897,7 @@ __global__ void __launch_bounds__(256) k_sur_forces(ElemArgs A) {
     SurPoint P;
     sur_point(A, e, P);
     const float* G = P.G;
-    const bool inside = A.sur_x[t * 8 + 7] != 0.f;
+    const bool inside = A.sur_y[t * 8 + 7] != 0.f;  // the flag, carried by the network
     const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
     float S[9], W;
     if (inside) {

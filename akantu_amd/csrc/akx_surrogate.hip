@@ -396,7 +396,10 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
 
 // evaluation of the network on n points (scaled inputs, sx floats per point
 // -> outputs, sy floats per point); persistent blocks of 8 waves, the image
-// staged in LDS once per block
+// staged in LDS once per block.  With 8 floats per point on both sides (the
+// explicit step's buffers) the points move as two 16-byte words and the 8th
+// input float, the strain pass's envelope flag, is copied to the 8th output
+// float, where the force pass reads it with the outputs.
 __global__ void __launch_bounds__(512) k_sur_infer(const f16x8* __restrict__ img,
                                                    const float* __restrict__ imgb, int64_t n,
                                                    const float* __restrict__ X, int sx, float* Y,
@@ -408,31 +411,47 @@ __global__ void __launch_bounds__(512) k_sur_infer(const f16x8* __restrict__ img
   const int64_t tiles = (n + 31) / 32;
   const int waves = blockDim.x >> 6;
   const int64_t stride = int64_t(gridDim.x) * waves;
+  const bool vec8 = sx == 8 && sy == 8;
   int64_t tile = int64_t(blockIdx.x) * waves + (threadIdx.x >> 6);
   // the next tile's inputs are loaded while the current one is evaluated;
   // lanes past the end read point 0 (n > 0) and use zeros
-  float xn[7];
+  float xn[8];
   auto load = [&](int64_t tl, float* x) {
     const int64_t p = tl * 32 + j;
     const bool ok = tl < tiles && p < n;
     const float* src = X + (ok ? p * sx : 0);
+    float v[8];
+    if (vec8) {
+      const float4 a = reinterpret_cast<const float4*>(src)[0];
+      const float4 b = reinterpret_cast<const float4*>(src)[1];
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      const float v = src[k];
-      x[k] = ok ? v : 0.f;
+      for (int k = 0; k < 7; ++k) v[k] = src[k];
+      v[7] = 0.f;
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ok ? v[k] : 0.f;
   };
   load(tile, xn);
   for (; tile < tiles; tile += stride) {
-    float x[7], y[7];
+    float x[8], y[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) x[k] = xn[k];
+    for (int k = 0; k < 8; ++k) x[k] = xn[k];
     load(tile + stride, xn);
     eval_wave(L, x, y);
     const int64_t p = tile * 32 + j;
-    if (p < n && h == 0)
+    if (p < n && h == 0) {
+      if (vec8) {
+        float4* dst = reinterpret_cast<float4*>(Y + p * 8);
+        dst[0] = make_float4(y[0], y[1], y[2], y[3]);
+        dst[1] = make_float4(y[4], y[5], y[6], x[7]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 7; ++k) Y[p * sy + k] = y[k];
+        for (int k = 0; k < 7; ++k) Y[p * sy + k] = y[k];
+      }
+    }
   }
 }
 
