@@ -843,7 +843,7 @@ def main():
         print(json.dumps({"copy_bytes": 1 << 30, "copy_gbs": copy_bandwidth(1 << 30, 3)}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
