@@ -1,0 +1,42 @@
+"""bench.py's contract on the CPU: one GPU only (--gpus above 1 exits with a
+message before touching a device), and the byte models it reports rooflines
+with follow the arrays each step touches."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_leap_form_drops_the_acceleration():
+    """The half-step velocity form reads and writes u and v only: 2 N d 8
+    bytes less nodal state than with the acceleration."""
+    E, N, n, Q, d = 1000, 300, 4, 1, 3
+    _, with_a = bench.algorithmic_bytes(E, N, n, Q, d, 4)
+    _, leap = bench.algorithmic_bytes(E, N, n, Q, d, 4, leap=True)
+    assert with_a - leap == 2 * N * d * 8
+
+
+def test_fused_mixed_against_fused_fp64():
+    """The mixed fused step differs from the FP64 one by the triangles'
+    16-byte geometry records instead of the coordinates, the acceleration
+    (absent in the leap form) and the per-node time step."""
+    E, N, d, ev, nblk = 10000, 5200, 2, 11700, 21
+    fp64 = bench.fused_bytes(E, N, 3, d, ev, nblk)
+    mixed = bench.fused_mixed_bytes(E, N, 3, d, ev, nblk, dt_per_node=True)
+    assert mixed - fp64 == pytest.approx(E * 16 - N * d * 8 - 2 * N * d * 8 + N * 8)
+    # tetrahedra keep the coordinates
+    fp64_t = bench.fused_bytes(E, N, 4, 3, ev, nblk)
+    mixed_t = bench.fused_mixed_bytes(E, N, 4, 3, ev, nblk)
+    assert fp64_t - mixed_t == pytest.approx(2 * N * 3 * 8)
