@@ -478,7 +478,8 @@ def bench_hex8(args, akx):
     model.set_dirichlet_increment(top, 2, -0.5 * dt)
     reset(model, v0)
     log("E=%d N=%d dt=%.6e" % (E, N, dt))
-    shapes = tune(model, args, akx)
+    profiled = "hex8_neohookean_traffic.json" if nh == 128 else None
+    shapes = tune(model, args, akx, profiled)
     res = measure(model, shapes, args, akx, v0, E * Q)
     steps = args.error_steps if args.error_steps != 1000 else 500
     log("error run: %d steps mixed and FP64" % steps)
@@ -517,6 +518,7 @@ def bench_hex8(args, akx):
     kname, kbytes, kms, achieved, eb, nb, step_gbs = dominant(
         res, shapes, model, elem_kernel, "void akx::k_node<3, 1, false>(akx::NodeArgs)",
         E, N, n, Q, d)
+    traffic = traffic_for(kname, shapes, profiled) if profiled else None
     out = base_record(args, res, "f32 element kernels / f64 nodal state", {
         "workload": "hex8-neohookean: %d^3 hexahedron_8 (2x2x2 Gauss), %d nodes, finite-strain "
                     "neo-Hookean E=10 MPa nu=0.45 rho=1000, bottom blocked, top compressed "
@@ -530,7 +532,7 @@ def bench_hex8(args, akx):
                      "to_10_percent_strain": strain},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": kbytes, "avg_ms": kms},
+                     "traffic": traffic, "algorithmic_bytes": kbytes, "avg_ms": kms},
         "roofline_step": {"algorithmic_bytes": eb + nb, "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
