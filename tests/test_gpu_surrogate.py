@@ -247,3 +247,52 @@ def test_surrogate_step_in_a_graph(layout):
     assert np.array_equal(out["eager"][1], out["graph"][1])
     assert out["eager"][2] == out["graph"][2]
     assert np.abs(out["eager"][0][-1]).max() > 0
+
+
+def test_surrogate_on_one_of_two_materials():
+    """The surrogate bound to the neohookean_damage material of a two-material
+    plate (the other elastic): its kernels then run on that material's
+    element list (a filtered launch group). With an envelope no strained
+    point fits, the step equals the plain law's within 1e-6; with a trained
+    network it stays within 5e-3 of it, and the elastic part is untouched by
+    the switch (same law on both runs)."""
+    m = akm.notched_plate_tet4(12, 12, 3)
+    E = m.nb_elements
+    X = m.nodes
+    cent = X[m.conn].mean(axis=1)
+    em = (cent[:, 0] > 0.5 * X[:, 0].max()).astype(np.int32)  # right half: damage law
+    yd = akrng.weibull_thresholds(E, 1e6, 5.0, seed=4)
+
+    def model(eps):
+        g = akx.SolidMechanicsModel(m)
+        g.add_material(E=PLATE["E"], nu=PLATE["nu"], rho=PLATE["rho"])
+        g.add_material(kind=akx.NEOHOOKEAN_DAMAGE, Sd=SD, Yd=yd, **PLATE)
+        g.assign_materials(em)
+        g.init_full()
+        g.surrogate_setup(1, eps)
+        return g
+
+    def run(g, enable):
+        blocked = np.zeros_like(X, dtype=bool)
+        blocked[np.isclose(X[:, 1], 0.0)] = True
+        top = np.nonzero(np.isclose(X[:, 1], X[:, 1].max()))[0]
+        g.set("blocked_dofs", blocked)
+        dt = 0.8 * g.get_stable_time_step()
+        g.set_time_step(dt)
+        g.set_dirichlet_increment(top, 1, 4.0 * dt)
+        g.surrogate_enable(enable)
+        g.solve_step(200, akx.MIXED)
+        return g.get_displacement(), g.get("damage")
+
+    u_plain, d_plain = run(model(0.05), False)
+    g = model(1e-9)
+    u_fb, d_fb = run(g, True)
+    assert g.surrogate_fallbacks() > 0
+    assert _rel(u_fb, u_plain) < 1e-6
+    assert np.abs(d_fb - d_plain).max() < 1e-6
+    g = model(0.05)
+    g.surrogate_generate(1000, 250, seed=5)
+    g.surrogate_train(1500, batch=4096, lr=2e-3)
+    u_sur, d_sur = run(g, True)
+    assert _rel(u_sur, u_plain) < 5e-3
+    assert (d_sur[em == 0] == 0).all() and (d_plain[em == 0] == 0).all()
