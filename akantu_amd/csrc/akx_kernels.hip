@@ -2115,9 +2115,9 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
       // differences to node 0 formed in FP64, then rounded.  For triangles
       // the coordinate differences come from the element's record (the same
       // values, formed by k_fused_geometry): one 16-byte load instead of
-      // three coordinate gathers, each element being evaluated by about 1.6
-      // node blocks.  Larger elements share their nodes among more elements
-      // and gather X (cached) faster than they would read a record.
+      // three coordinate gathers, for every node block that evaluates the
+      // element.  Larger elements share their nodes among more elements and
+      // gather X (cached) faster than they would read a record.
       constexpr bool RECORD = ET == TRI3;
       constexpr int G = fused_geo_stride(n, d);
       float gx[RECORD ? G : 1];
