@@ -74,7 +74,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
   dfree(c->fel_off); dfree(c->fel); dfree(c->flslot); dfree(c->fch_off); dfree(c->fcbase);
-  dfree(c->u_alt); dfree(c->fgeo);
+  dfree(c->u_alt); dfree(c->fgeo); dfree(c->frec); dfree(c->frecx);
   c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
@@ -1014,6 +1014,14 @@ int akx_init(akx_ctx* c) {
     HIP_TRY(c, dalloc(&c->elem_mat_d, size_t(E)));
     HIP_TRY(c, hipMemcpy(c->elem_mat_d, em.data(), size_t(E) * 2, hipMemcpyHostToDevice));
   }
+  // the fused mixed step on triangles streams one record per evaluation
+  // (built here, after the material indices it carries are on the device)
+  if (c->fused_ok && c->et == AKX_TRIANGLE_3) {
+    HIP_TRY(c, dalloc(&c->frec, size_t(c->fused_elements) * 2));
+    HIP_TRY(c, dalloc(&c->frecx, size_t(c->fused_elements)));
+    HIP_TRY(c, launch_fused_records(c->stream, c->fused_elements, E, c->fel, c->conn_soa, c->fgeo,
+                                    c->flslot, c->elem_mat_d, c->frec, c->frecx));
+  }
   if (any_damage) {
     std::vector<double> yd(size_t(E) * Q, 0.), ydi(size_t(E) * Q);
     for (int64_t e = 0; e < E; ++e) {
@@ -1409,6 +1417,8 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       F.cbase = c->fcbase;
       F.u_out = unext;
       F.geo = c->fgeo;
+      F.rec = c->frec;
+      F.recx = c->frecx;
       F.nblk = c->fused_blocks;
       HIP_TRY(c, launch_fused(c->et, g.kind, precision, c->mixed_recompute != 0, c->has_fext,
                               c->stream, EA, NA, F, c->fused_lds));

@@ -110,6 +110,12 @@ struct FusedArgs {
   const uint16_t* cbase;   // [chunk * (FUSED_NB + 1) + k] first slot of block node k in the chunk
   double* u_out;           // displacement written by the step (read from NodeArgs::u)
   const float* geo;        // mixed path on triangles: per element x_a - x_0 (a > 0), FP32
+  // mixed path on triangles: per evaluation j (entry of el), everything the
+  // element needs but u, read as a stream: rec[2 j] = the element's
+  // coordinate differences (4 FP32), rec[2 j + 1] = (node 0, node 1, node 2,
+  // LDS slot 0 | slot 1 << 16); recx[j] = LDS slot 2 | material << 16
+  const uint4* rec;
+  const uint32_t* recx;
   int64_t nblk;
 };
 
@@ -162,6 +168,9 @@ hipError_t launch_fused(int et, int kind, int precision, bool recompute, bool ha
                         int lds);
 hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const NodeArgs& A,
                        int block);
+hipError_t launch_fused_records(hipStream_t s, int64_t J, int64_t E, const uint32_t* el,
+                                const uint32_t* conn_soa, const float* geo, const uint16_t* lslot,
+                                const uint16_t* elem_mat, uint4* rec, uint32_t* recx);
 hipError_t launch_fused_geometry(int et, hipStream_t s, int64_t E, const double* X,
                                  const uint32_t* conn_soa, float* geo);
 hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
@@ -356,6 +365,8 @@ struct akx_ctx {
   uint16_t *flslot = nullptr, *fcbase = nullptr;
   double* u_alt = nullptr;  // second displacement buffer of the fused step
   float* fgeo = nullptr;     // geometry records of the fused mixed step (triangles)
+  uint4* frec = nullptr;     // per-evaluation records of the fused mixed step (triangles)
+  uint32_t* frecx = nullptr;
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal

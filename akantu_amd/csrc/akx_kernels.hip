@@ -2107,24 +2107,39 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
   for (uint32_t jc = j0; jc < j1; jc += FUSED_CH, ++c) {
     const uint32_t j = jc + uint32_t(t);
     if (j < j1) {
-      const int64_t e = F.el[j];
-      const MatParams m = EA.mats[EA.elem_mat ? EA.elem_mat[e] : 0];
-      int64_t nd[n];
-#pragma unroll
-      for (int a = 0; a < n; ++a) nd[a] = EA.conn[a * E + e];
-      // differences to node 0 formed in FP64, then rounded.  For triangles
-      // the coordinate differences come from the element's record (the same
-      // values, formed by k_fused_geometry): one 16-byte load instead of
-      // three coordinate gathers, for every node block that evaluates the
-      // element.  Larger elements share their nodes among more elements and
-      // gather X (cached) faster than they would read a record.
+      // Triangles read one record per evaluation, in the block's evaluation
+      // order: the coordinate differences (the same values k_fused_geometry
+      // formed from X), the node ids, the LDS slots and the material, 36
+      // bytes streamed instead of an element id and its dependent gathers
+      // of connectivity, geometry and material.  Larger elements share
+      // their nodes among more elements and gather X (cached) faster than
+      // they would read a record.
       constexpr bool RECORD = ET == TRI3;
       constexpr int G = fused_geo_stride(n, d);
+      int64_t e = 0;  // element id (not needed with a record)
+      int mat = 0;
+      int64_t nd[n];
+      uint32_t lsl[n];
       float gx[RECORD ? G : 1];
       if constexpr (RECORD) {
-        const float4 v = reinterpret_cast<const float4*>(F.geo)[e];
-        gx[0] = v.x; gx[1] = v.y; gx[2] = v.z; gx[3] = v.w;
+        static_assert(G == 4 && n == 3, "triangle records: 4 coordinate differences, 3 nodes");
+        const uint4 r0 = F.rec[size_t(j) * 2], r1 = F.rec[size_t(j) * 2 + 1];
+        const uint32_t rx = F.recx[j];
+        gx[0] = __uint_as_float(r0.x); gx[1] = __uint_as_float(r0.y);
+        gx[2] = __uint_as_float(r0.z); gx[3] = __uint_as_float(r0.w);
+        nd[0] = r1.x; nd[1] = r1.y; nd[2] = r1.z;
+        lsl[0] = r1.w & 0xFFFFu; lsl[1] = r1.w >> 16; lsl[2] = rx & 0xFFFFu;
+        mat = int(rx >> 16);
+      } else {
+        e = F.el[j];
+        mat = EA.elem_mat ? EA.elem_mat[e] : 0;
+#pragma unroll
+        for (int a = 0; a < n; ++a) nd[a] = EA.conn[a * E + e];
+#pragma unroll
+        for (int a = 0; a < n; ++a) lsl[a] = F.lslot[size_t(j) * n + a];
       }
+      const MatParams m = EA.mats[mat];
+      // differences to node 0 formed in FP64, then rounded
       float du[dn], dX[dn];
 #pragma unroll
       for (int i = 0; i < d; ++i) {
@@ -2143,7 +2158,7 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
       mixed_element_forces<ET, KIND, true>(EA, e, m, du, dX, fe);
 #pragma unroll
       for (int a = 0; a < n; ++a) {
-        const uint32_t ls = F.lslot[size_t(j) * n + a];
+        const uint32_t ls = lsl[a];
         if (ls != 0xFFFFu)
 #pragma unroll
           for (int i = 0; i < d; ++i) s_f[ls * d + i] = fe[i + a * d];
@@ -2208,6 +2223,38 @@ __global__ void __launch_bounds__(256) k_fused_geometry(int64_t E, const double*
   }
 #pragma unroll
   for (int k = 0; k < G; ++k) geo[e * G + k] = r[k];
+}
+
+// Per-evaluation records of the fused mixed step on triangles (FusedArgs::rec,
+// recx), one thread per evaluation j of the blocks' element lists.
+__global__ void __launch_bounds__(256) k_fused_records(int64_t J, int64_t E,
+                                                       const uint32_t* __restrict__ el,
+                                                       const uint32_t* __restrict__ conn,
+                                                       const float* __restrict__ geo,
+                                                       const uint16_t* __restrict__ lslot,
+                                                       const uint16_t* __restrict__ elem_mat,
+                                                       uint4* __restrict__ rec,
+                                                       uint32_t* __restrict__ recx) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  const int64_t e = el[j];
+  const float4 g = reinterpret_cast<const float4*>(geo)[e];
+  rec[j * 2] = make_uint4(__float_as_uint(g.x), __float_as_uint(g.y), __float_as_uint(g.z),
+                          __float_as_uint(g.w));
+  rec[j * 2 + 1] = make_uint4(conn[e], conn[E + e], conn[2 * E + e],
+                              uint32_t(lslot[j * 3]) | (uint32_t(lslot[j * 3 + 1]) << 16));
+  const uint32_t mat = elem_mat ? uint32_t(elem_mat[e]) : 0u;
+  recx[j] = uint32_t(lslot[j * 3 + 2]) | (mat << 16);
+}
+
+hipError_t launch_fused_records(hipStream_t s, int64_t J, int64_t E, const uint32_t* el,
+                                const uint32_t* conn_soa, const float* geo, const uint16_t* lslot,
+                                const uint16_t* elem_mat, uint4* rec, uint32_t* recx) {
+  clear_error();
+  if (J == 0) return hipSuccess;
+  k_fused_records<<<unsigned((J + 255) / 256), 256, 0, s>>>(J, E, el, conn_soa, geo, lslot,
+                                                             elem_mat, rec, recx);
+  return hipGetLastError();
 }
 
 hipError_t launch_fused_geometry(int et, hipStream_t s, int64_t E, const double* X,

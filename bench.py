@@ -120,13 +120,18 @@ def fused_bytes(E, N, n, d, evaluations, nblk, node_block=256, chunk=256):
 def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_block=256,
                       chunk=256):
     """Compulsory HBM bytes of one fused mixed step (slot layout 6 on the mixed
-    path): as the FP64 one, but triangles read a 16-byte FP32 record of their
-    coordinate differences instead of gathering X, and within a run the node
-    update carries the half-step velocity (v read and written, no a)."""
+    path): as the FP64 one, but within a run the node update carries the
+    half-step velocity (v read and written, no a), and triangles stream one
+    36-byte record per evaluation (coordinate differences, node ids, LDS
+    slots, material) instead of the element list, LDS slots, connectivity
+    and coordinates."""
     chunks = evaluations / chunk + nblk
-    coords = E * 16 if n == 3 else N * d * 8
-    return (evaluations * (4 + 2 * n) + chunks * (node_block + 1) * 2 + (nblk + 1) * 8
-            + E * n * 4 + coords + 2 * N * d * 8 + 2 * N * d * 8 + N * 8 + N * d
+    tables = chunks * (node_block + 1) * 2 + (nblk + 1) * 8
+    if n == 3:
+        elements = evaluations * 36
+    else:
+        elements = evaluations * (4 + 2 * n) + E * n * 4 + N * d * 8
+    return (elements + tables + 2 * N * d * 8 + 2 * N * d * 8 + N * 8 + N * d
             + (N * 8 if dt_per_node else 0))
 
 
