@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libakx.so")
 TRIANGLE_3, QUADRANGLE_4, TETRAHEDRON_4, HEXAHEDRON_8 = 0, 1, 2, 3
 ELASTIC, NEOHOOKEAN, NEOHOOKEAN_DAMAGE = 0, 1, 2
 FP64, MIXED = 0, 1
+SURROGATE_FORMATS = {"fp16": 0, "bf16": 1}
 FIELDS = {"displacement": 0, "velocity": 1, "acceleration": 2, "external_force": 3,
           "internal_force": 4, "mass": 5, "blocked_dofs": 6, "previous_displacement": 7,
           "displacement_increment": 8, "grad_u": 9, "stress": 10, "damage": 11,
@@ -75,6 +76,7 @@ SIGNATURES = {
     "akx_surrogate_dataset": (_I, [_P, _I64, _I64, _P, _P]),
     "akx_surrogate_enable": (_I, [_P, _I]),
     "akx_surrogate_fallbacks": (_I, [_P, ctypes.POINTER(ctypes.c_uint64), _I]),
+    "akx_surrogate_format": (_I, [_P, _I]),
     "akx_fe_gradient_on_integration_points": (_I, [_P, _P, _I, _P, _I64, _P]),
     "akx_fe_interpolate_on_integration_points": (_I, [_P, _P, _I, _P, _I64, _P]),
     "akx_fe_compute_btd": (_I, [_P, _P, _I, _P, _I64, _P]),
@@ -464,6 +466,14 @@ class SolidMechanicsModel:
         v = ctypes.c_uint64()
         self.ctx.check(lib.akx_surrogate_fallbacks(self.ctx.h, ctypes.byref(v), int(bool(reset))))
         return v.value
+
+    def surrogate_format(self, fmt=None):
+        """16-bit format of the inference kernel, "fp16" (default) or "bf16";
+        with no argument, the current one."""
+        if fmt is not None:
+            self.ctx.check(lib.akx_surrogate_format(self.ctx.h, SURROGATE_FORMATS[fmt]))
+        code = int(self.query("surrogate_format"))
+        return {v: k for k, v in SURROGATE_FORMATS.items()}[code]
 
     # -- output (SolidMechanicsModel::setBaseName / addDumpField / dump) --
     def set_base_name(self, name, directory="paraview", asynchronous=True, binary=True):

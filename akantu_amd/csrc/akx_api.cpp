@@ -345,6 +345,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   A.tab64 = c->tab64_d;
   A.sur_img = nullptr;
   A.sur_b = nullptr;
+  A.sur_bf16 = false;
   A.sur_eps = A.sur_sS = A.sur_sW = 0.f;
   A.sur_fallbacks = nullptr;
   A.sur_x = A.sur_y = nullptr;
@@ -352,6 +353,7 @@ ElemArgs elem_args(akx_ctx* c, const KindGroup& g, int precision) {
   if (precision == AKX_MIXED && c->sur.enabled && g.kind == AKX_MATERIAL_NEOHOOKEAN_DAMAGE) {
     A.sur_img = c->sur.img;
     A.sur_b = c->sur.imgb;
+    A.sur_bf16 = c->sur.infer_bf16;
     A.sur_eps = float(c->sur.eps);
     A.sur_sS = float(c->sur.sS);
     A.sur_sW = float(c->sur.sW);
@@ -1673,6 +1675,7 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "surrogate_samples") *value = double(c->sur.nsamples);
   else if (k == "surrogate_parameters") *value = double(akx::mlp::NPARAM);
   else if (k == "surrogate_enabled") *value = c->sur.enabled ? 1. : 0.;
+  else if (k == "surrogate_format") *value = c->sur.infer_bf16 ? AKX_SURROGATE_BF16 : AKX_SURROGATE_FP16;
   else if (k == "surrogate_network_ms") *value = c->last_net_ms;
   else return fail(c, AKX_ERR_ARGUMENT, "unknown query key " + k);
   return AKX_OK;
@@ -1786,6 +1789,23 @@ int akx_surrogate_enable(akx_ctx* c, int on) {
   rc = surrogate_check(c);
   if (rc) return rc;
   c->sur.enabled = true;
+  return AKX_OK;
+}
+
+int akx_surrogate_format(akx_ctx* c, int format) {
+  if (!c) return AKX_ERR_ARGUMENT;
+  if (format != AKX_SURROGATE_FP16 && format != AKX_SURROGATE_BF16)
+    return fail(c, AKX_ERR_ARGUMENT, "surrogate format: AKX_SURROGATE_FP16 or AKX_SURROGATE_BF16");
+  (void)hipSetDevice(c->device);
+  // the captured steps hold the network kernel of the previous format
+  free_graph(c);
+  auto& S = c->sur;
+  S.infer_bf16 = format == AKX_SURROGATE_BF16;
+  if (S.params) {
+    // the image in the new format, from the same master weights
+    HIP_TRY(c, S.refresh(c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
   return AKX_OK;
 }
 

@@ -67,6 +67,7 @@ struct ElemArgs {
   // constitutive surrogate (k_sur_strain / network / k_sur_forces); sur_img == nullptr: plain law
   const void* sur_img;
   const float* sur_b;
+  bool sur_bf16;             // image and activations in BF16 (else FP16)
   float sur_eps, sur_sS, sur_sW;
   unsigned long long* sur_fallbacks;
   float* sur_x;              // [count][8]: scaled network inputs + envelope flag
@@ -151,8 +152,8 @@ inline __host__ __device__ size_t patch_force_offset(int S, int d) {
 
 // Network evaluation on n points: scaled inputs x (sx floats per point) ->
 // outputs y (sy floats per point) (akx_surrogate.hip).
-hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int64_t n,
-                          const float* x, int sx, float* y, int sy);
+hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, bool bf16,
+                          int64_t n, const float* x, int sx, float* y, int sy);
 
 // Launchers (akx_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_precompute(int et, hipStream_t s, int64_t E, int64_t N, const double* X,
@@ -245,8 +246,9 @@ struct Surrogate {
   int64_t step = 0;
   __bf16* Wbf[5] = {};
   __bf16* WT[5] = {};
-  mlp::f16x8* img = nullptr;
+  mlp::f16x8* img = nullptr;  // 16-byte units; FP16, or BF16 when infer_bf16
   float* imgb = nullptr;
+  bool infer_bf16 = false;
   unsigned long long* fallbacks = nullptr;
   // training set
   float *X = nullptr, *T = nullptr;

@@ -2005,7 +2005,7 @@ static hipError_t launch_element_et(int kind, int precision, hipStream_t s, cons
       hipError_t err = hipGetLastError();
       if (err != hipSuccess) return err;
       if (A.net_ev0) (void)hipEventRecord(A.net_ev0, s);
-      err = launch_sur_mlp(s, A.sur_img, A.sur_b, A.count, A.sur_x, 8, A.sur_y, 8);
+      err = launch_sur_mlp(s, A.sur_img, A.sur_b, A.sur_bf16, A.count, A.sur_x, 8, A.sur_y, 8);
       if (err != hipSuccess) return err;
       if (A.net_ev1) (void)hipEventRecord(A.net_ev1, s);
       if (A.patch)

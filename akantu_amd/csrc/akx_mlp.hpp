@@ -5,7 +5,8 @@
 // after every hidden layer (k is the FP16 value next to 1/6, so that the
 // FP16 inference evaluates the same function the training fits); FP32
 // accumulation and biases; training in BF16 (weights, activations, errors),
-// inference in FP16 (weights, activations).  S is a cubic smoothstep from 0 at
+// inference in FP16 (the default) or BF16 (weights, activations).  S is a
+// cubic smoothstep from 0 at
 // z = -1/(2k) to 1 at z = 1/(2k), with S(0) = 1/2 and S'(0) = 1/4 as the
 // logistic sigmoid; a is continuously differentiable (S' vanishes at both
 // ends) and needs no transcendental: one clamped fused multiply-add, three
@@ -60,7 +61,7 @@ constexpr int NPARAM = OFF_B4 + NOUT;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// permuted inference weight image (units of 8 FP16 weights = one lane's operand)
+// permuted inference weight image (units of 8 16-bit weights = one lane's operand)
 constexpr int PW_L0 = 0;                      // [mb 4][lane 64]
 constexpr int PW_L1 = PW_L0 + 4 * 64;         // [mb 4][t 8][lane 64]
 constexpr int PW_L2 = PW_L1 + 4 * 8 * 64;
@@ -97,44 +98,74 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return static_cast<__bf16>(
 __device__ __forceinline__ float from_bf16(__bf16 x) { return static_cast<float>(x); }
 
 // ---------------------------------------------------------------------------
-// Inference (k_sur_infer): FP16 weights and activations on
-// v_mfma_f32_32x32x16_f16, FP32 accumulation, the bias as the MFMA's C
-// operand of a layer's first K step, and the activation on packed FP16: per
-// pair of values one v_cvt_pk_f16_f32 (to nearest), one v_pk_fma_f16 with the
-// clamp bit for t, and four v_pk_mul_f16 / v_pk_fma_f16.
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+// Inference (k_sur_infer), in one of two 16-bit formats, FP32 accumulation and
+// the bias as the MFMA's C operand of a layer's first K step in both:
+//   Fp16: FP16 weights and activations on v_mfma_f32_32x32x16_f16, the
+//         activation on packed FP16: per pair of values one v_cvt_pk_f16_f32
+//         (to nearest), one v_pk_fma_f16 with the clamp bit for t, and four
+//         v_pk_mul_f16 / v_pk_fma_f16;
+//   Bf16: BF16 weights and activations on v_mfma_f32_32x32x16_bf16 (the
+//         training kernels' format); gfx950 has no packed BF16 arithmetic, so
+//         the activation runs in FP32, one value per instruction, and each
+//         result is rounded once to BF16 (v_cvt_pk_bf16_f32, to nearest).
+struct Fp16 {
+  typedef _Float16 S;
+  typedef _Float16 x8 __attribute__((ext_vector_type(8)));
+  typedef _Float16 x4 __attribute__((ext_vector_type(4)));
+  typedef _Float16 x2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ f32x16 mfma(x8 a, x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ x2 splat2(float v) {
+    const S h = static_cast<S>(v);
+    return x2{h, h};
+  }
+  // a(z) for the two accumulator values z0, z1: Z (S), with S = (t t)(3 - 2 t)
+  static __device__ __forceinline__ x2 act2(float z0, float z1) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const x2 Z = __builtin_convertvector(f32x2{z0, z1}, x2);
+    x2 t = __builtin_elementwise_fma(Z, splat2(ACT_K), splat2(0.5f));
+    t = __builtin_elementwise_min(__builtin_elementwise_max(t, splat2(0.f)), splat2(1.f));
+    const x2 u = __builtin_elementwise_fma(t, splat2(-2.f), splat2(3.f));
+    return Z * ((t * t) * u);
+  }
+};
 
-__device__ __forceinline__ f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-}
+struct Bf16 {
+  typedef __bf16 S;
+  typedef __bf16 x8 __attribute__((ext_vector_type(8)));
+  typedef __bf16 x4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 x2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ f32x16 mfma(x8 a, x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  // a(z0), a(z1) in FP32 (t = clamp(fma(z, k, 1/2)), u = fma(t, -2, 3),
+  // a = z ((t t) u)), each rounded to BF16
+  static __device__ __forceinline__ x2 act2(float z0, float z1) {
+    const float t0 = fminf(fmaxf(__builtin_fmaf(z0, ACT_K, 0.5f), 0.f), 1.f);
+    const float t1 = fminf(fmaxf(__builtin_fmaf(z1, ACT_K, 0.5f), 0.f), 1.f);
+    const float a0 = z0 * ((t0 * t0) * __builtin_fmaf(t0, -2.f, 3.f));
+    const float a1 = z1 * ((t1 * t1) * __builtin_fmaf(t1, -2.f, 3.f));
+    return x2{static_cast<S>(a0), static_cast<S>(a1)};
+  }
+};
 
-__device__ __forceinline__ f16x2 splat2(float v) {
-  const _Float16 h = static_cast<_Float16>(v);
-  return f16x2{h, h};
-}
+// the Fp16 operand vector: the unit in which the weight image is stored
+typedef Fp16::x8 f16x8;
 
-// a(z) for the two accumulator values z0, z1: Z (S), with S = (t t)(3 - 2 t)
-__device__ __forceinline__ f16x2 act2(float z0, float z1) {
-  typedef float f32x2 __attribute__((ext_vector_type(2)));
-  const f16x2 Z = __builtin_convertvector(f32x2{z0, z1}, f16x2);
-  f16x2 t = __builtin_elementwise_fma(Z, splat2(ACT_K), splat2(0.5f));
-  t = __builtin_elementwise_min(__builtin_elementwise_max(t, splat2(0.f)), splat2(1.f));
-  const f16x2 u = __builtin_elementwise_fma(t, splat2(-2.f), splat2(3.f));
-  return Z * ((t * t) * u);
-}
-
-__device__ __forceinline__ f16x8 join4(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
-  const f16x4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
-  const f16x4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
+template <class F>
+__device__ __forceinline__ typename F::x8 join4(typename F::x2 a, typename F::x2 b,
+                                                typename F::x2 c, typename F::x2 d) {
+  const typename F::x4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+  const typename F::x4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
   return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-// Weights (FP16, permuted as the chained layout needs) and biases (FP32, in
+// Weights (16-bit, permuted as the chained layout needs) and biases (FP32, in
 // feature order) of one network, as staged in LDS by the inference kernel.
+template <class F>
 struct Image {
-  f16x8 w[PW_SIZE];
+  typename F::x8 w[PW_SIZE];
   float b[PB_SIZE];
 };
 
@@ -156,42 +187,52 @@ __device__ __forceinline__ f32x16 bias_block(const float* B, int layer, int mb, 
 
 // g of the 16 values of one C row block: act slots of K steps 2 mb (registers
 // 0-7) and 2 mb + 1 (registers 8-15) of the next layer (chain_k order)
-__device__ __forceinline__ void activate_block(const f32x16& a, f16x8& lo, f16x8& hi) {
-  lo = join4(act2(a[0], a[1]), act2(a[2], a[3]), act2(a[4], a[5]), act2(a[6], a[7]));
-  hi = join4(act2(a[8], a[9]), act2(a[10], a[11]), act2(a[12], a[13]), act2(a[14], a[15]));
+template <class F>
+__device__ __forceinline__ void activate_block(const f32x16& a, typename F::x8& lo,
+                                               typename F::x8& hi) {
+  lo = join4<F>(F::act2(a[0], a[1]), F::act2(a[2], a[3]), F::act2(a[4], a[5]),
+                F::act2(a[6], a[7]));
+  hi = join4<F>(F::act2(a[8], a[9]), F::act2(a[10], a[11]), F::act2(a[12], a[13]),
+                F::act2(a[14], a[15]));
 }
 
 // Row block mb of a hidden layer: its 8 K steps chained on one accumulator,
 // which starts from the bias.
-__device__ __forceinline__ f32x16 hidden_block(const float* B, const f16x8* wl,
-                                               const f16x8 (&in)[8], int layer, int mb, int h) {
+template <class F>
+__device__ __forceinline__ f32x16 hidden_block(const float* B, const typename F::x8* wl,
+                                               const typename F::x8 (&in)[8], int layer, int mb,
+                                               int h) {
   f32x16 c = bias_block(B, layer, mb, h);
 #pragma unroll
-  for (int t = 0; t < 8; ++t) c = mfma16(wl[(mb * 8 + t) * 64], in[t], c);
+  for (int t = 0; t < 8; ++t) c = F::mfma(wl[(mb * 8 + t) * 64], in[t], c);
   return c;
 }
 
 // One hidden layer, in -> out.  The row blocks run one after the other so
 // that the activation of block mb (vector pipe) overlaps the MFMAs of block
-// mb + 1 (matrix pipe) within the wave; the scheduling groups interleave
-// them one MFMA to a share of the activation's vector instructions.
-__device__ __forceinline__ void hidden_layer(const f16x8* W, const float* B, int layer, int lane,
-                                             int h, const f16x8 (&in)[8], f16x8 (&out)[8]) {
-  const f16x8* wl = W + PW_L1 + (layer - 1) * (PW_L2 - PW_L1) + lane;
-  f32x16 a0 = hidden_block(B, wl, in, layer, 0, h);
-  f32x16 a1 = hidden_block(B, wl, in, layer, 1, h);
-  activate_block(a0, out[0], out[1]);
-  f32x16 a2 = hidden_block(B, wl, in, layer, 2, h);
-  activate_block(a1, out[2], out[3]);
-  f32x16 a3 = hidden_block(B, wl, in, layer, 3, h);
-  activate_block(a2, out[4], out[5]);
-  activate_block(a3, out[6], out[7]);
+// mb + 1 (matrix pipe) within the wave.
+template <class F>
+__device__ __forceinline__ void hidden_layer(const typename F::x8* W, const float* B, int layer,
+                                             int lane, int h, const typename F::x8 (&in)[8],
+                                             typename F::x8 (&out)[8]) {
+  const typename F::x8* wl = W + PW_L1 + (layer - 1) * (PW_L2 - PW_L1) + lane;
+  f32x16 a0 = hidden_block<F>(B, wl, in, layer, 0, h);
+  f32x16 a1 = hidden_block<F>(B, wl, in, layer, 1, h);
+  activate_block<F>(a0, out[0], out[1]);
+  f32x16 a2 = hidden_block<F>(B, wl, in, layer, 2, h);
+  activate_block<F>(a1, out[2], out[3]);
+  f32x16 a3 = hidden_block<F>(B, wl, in, layer, 3, h);
+  activate_block<F>(a2, out[4], out[5]);
+  activate_block<F>(a3, out[6], out[7]);
 }
 
 // Evaluate the network for the 32 points of one wave: lanes j and j + 32 both
 // carry point j's inputs x[0..6] and both receive its outputs y[0..6].  Every
 // lane of the wave must call it (the MFMAs are wave-wide).
-__device__ __forceinline__ void eval_wave(const Image& L, const float* x, float* y) {
+template <class F>
+__device__ __forceinline__ void eval_wave(const Image<F>& L, const float* x, float* y) {
+  typedef typename F::S S;
+  typedef typename F::x8 V;
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
   // A zero offset that passes through an empty asm statement on every call:
@@ -199,37 +240,38 @@ __device__ __forceinline__ void eval_wave(const Image& L, const float* x, float*
   // (they would not fit in registers and would spill).
   int zero = 0;
   asm volatile("" : "+v"(zero));
-  const f16x8* W = L.w + zero;
+  const V* W = L.w + zero;
   const float* B = L.b + zero;
   // input layer: K slot 8h + s holds x_hi[k] (k < 7), x_lo[k - 7] (7 <= k < 14), 0;
-  // FP16 round to nearest, x_lo = x - x_hi exactly in FP32
-  _Float16 xh[NIN], xl[NIN];
+  // 16-bit round to nearest, x_lo = x - x_hi exactly in FP32
+  S xh[NIN], xl[NIN];
 #pragma unroll
   for (int k = 0; k < NIN; ++k) {
-    xh[k] = static_cast<_Float16>(x[k]);
-    xl[k] = static_cast<_Float16>(x[k] - static_cast<float>(xh[k]));
+    xh[k] = static_cast<S>(x[k]);
+    xl[k] = static_cast<S>(x[k] - static_cast<float>(xh[k]));
   }
-  f16x8 bx;
+  V bx;
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
-    const _Float16 first = s < NIN ? xh[s] : xl[s - NIN];           // k = s
-    const _Float16 second = s + 8 < 2 * NIN ? xl[s + 8 - NIN] : _Float16(0);  // k = 8 + s
+    const S first = s < NIN ? xh[s] : xl[s - NIN];                       // k = s
+    const S second = s + 8 < 2 * NIN ? xl[s + 8 - NIN] : static_cast<S>(0.f);  // k = 8 + s
     bx[s] = h ? second : first;
   }
-  f16x8 actA[8], actB[8];  // the activations, ping-pong between the layers
+  V actA[8], actB[8];  // the activations, ping-pong between the layers
   {
     f32x16 a[4];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) a[mb] = mfma16(W[PW_L0 + mb * 64 + lane], bx, bias_block(B, 0, mb, h));
+    for (int mb = 0; mb < 4; ++mb)
+      a[mb] = F::mfma(W[PW_L0 + mb * 64 + lane], bx, bias_block(B, 0, mb, h));
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) activate_block(a[mb], actA[2 * mb], actA[2 * mb + 1]);
+    for (int mb = 0; mb < 4; ++mb) activate_block<F>(a[mb], actA[2 * mb], actA[2 * mb + 1]);
   }
-  hidden_layer(W, B, 1, lane, h, actA, actB);
-  hidden_layer(W, B, 2, lane, h, actB, actA);
-  hidden_layer(W, B, 3, lane, h, actA, actB);
+  hidden_layer<F>(W, B, 1, lane, h, actA, actB);
+  hidden_layer<F>(W, B, 2, lane, h, actB, actA);
+  hidden_layer<F>(W, B, 3, lane, h, actA, actB);
   f32x16 o = f32x16{};
 #pragma unroll
-  for (int t = 0; t < 8; ++t) o = mfma16(W[PW_L4 + t * 64 + lane], actB[t], o);
+  for (int t = 0; t < 8; ++t) o = F::mfma(W[PW_L4 + t * 64 + lane], actB[t], o);
   // rows 0..3 sit in registers 0..3 of half 0, rows 4..7 in registers 0..3 of half 1
   float mine[4], other[4];
 #pragma unroll
@@ -246,7 +288,8 @@ __device__ __forceinline__ void eval_wave(const Image& L, const float* x, float*
 }
 
 // Cooperative copy of a permuted image (global) into LDS by one block.
-__device__ __forceinline__ void stage_image(Image& L, const f16x8* __restrict__ w,
+template <class F>
+__device__ __forceinline__ void stage_image(Image<F>& L, const typename F::x8* __restrict__ w,
                                             const float* __restrict__ b) {
   for (int i = threadIdx.x; i < PW_SIZE; i += blockDim.x) L.w[i] = w[i];
   for (int i = threadIdx.x; i < PB_SIZE; i += blockDim.x) L.b[i] = b[i];

@@ -337,12 +337,12 @@ __global__ void __launch_bounds__(256) k_sur_update(LayerDesc L, int nch, float*
 }
 
 // BF16 copies of the master weights (forward layout, transposed layout) and
-// the permuted FP16 inference image
+// the permuted inference image, FP16 or (bf16 != 0) BF16
 __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P, __bf16* W0,
                                                      __bf16* W1, __bf16* W2, __bf16* W3,
                                                      __bf16* W4, __bf16* T1, __bf16* T2,
                                                      __bf16* T3, __bf16* T4, f16x8* img,
-                                                     float* imgb) {
+                                                     float* imgb, int bf16) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   // forward / transposed copies
   if (idx < HID * K0) {
@@ -379,7 +379,10 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
       const int t = (vec - PW_L4) / 64;
       w = i < NOUT ? P[OFF_W4 + i * HID + chain_k(t, h, s)] : 0.f;
     }
-    img[vec][s] = static_cast<_Float16>(w);
+    if (bf16)
+      reinterpret_cast<Bf16::x8*>(img)[vec][s] = static_cast<__bf16>(w);
+    else
+      img[vec][s] = static_cast<_Float16>(w);
   }
   if (idx < PB_SIZE) {
     float b = 0.f;
@@ -400,12 +403,13 @@ __global__ void __launch_bounds__(256) k_sur_refresh(const float* __restrict__ P
 // explicit step's buffers) the points move as two 16-byte words and the 8th
 // input float, the strain pass's envelope flag, is copied to the 8th output
 // float, where the force pass reads it with the outputs.
-__global__ void __launch_bounds__(512) k_sur_infer(const f16x8* __restrict__ img,
+template <class F>
+__global__ void __launch_bounds__(512) k_sur_infer(const typename F::x8* __restrict__ img,
                                                    const float* __restrict__ imgb, int64_t n,
                                                    const float* __restrict__ X, int sx, float* Y,
                                                    int sy) {
-  __shared__ Image L;
-  stage_image(L, img, imgb);
+  __shared__ Image<F> L;
+  stage_image<F>(L, img, imgb);
   __syncthreads();
   const int lane = threadIdx.x & 63, j = lane & 31, h = lane >> 5;
   const int64_t tiles = (n + 31) / 32;
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(512) k_sur_infer(const f16x8* __restrict__ img
 #pragma unroll
     for (int k = 0; k < 8; ++k) x[k] = xn[k];
     load(tile + stride, xn);
-    eval_wave(L, x, y);
+    eval_wave<F>(L, x, y);
     const int64_t p = tile * 32 + j;
     if (p < n && h == 0) {
       if (vec8) {
@@ -536,7 +540,7 @@ hipError_t Surrogate::refresh(hipStream_t s) {
   const int nthreads = std::max({HID * HID, PW_SIZE * 8});
   k_sur_refresh<<<(nthreads + 255) / 256, 256, 0, s>>>(params, Wbf[0], Wbf[1], Wbf[2], Wbf[3],
                                                        Wbf[4], WT[1], WT[2], WT[3], WT[4], img,
-                                                       imgb);
+                                                       imgb, int(infer_bf16));
   return hipGetLastError();
 }
 
@@ -716,20 +720,25 @@ hipError_t Surrogate::infer(hipStream_t s, int64_t n, const float* x_host, float
   hipError_t e;
   if ((e = ensure_params()) || (e = ensure_io(n))) return e;
   if ((e = hipMemcpyAsync(io_x, x_host, size_t(n) * 7 * 4, hipMemcpyHostToDevice, s))) return e;
-  if ((e = launch_sur_mlp(s, img, imgb, n, io_x, 7, io_y, 7))) return e;
+  if ((e = launch_sur_mlp(s, img, imgb, infer_bf16, n, io_x, 7, io_y, 7))) return e;
   if ((e = hipMemcpyAsync(y_host, io_y, size_t(n) * 7 * 4, hipMemcpyDeviceToHost, s))) return e;
   return hipStreamSynchronize(s);
 }
 
-hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, int64_t n,
-                          const float* x, int sx, float* y, int sy) {
+hipError_t launch_sur_mlp(hipStream_t s, const void* img, const float* imgb, bool bf16,
+                          int64_t n, const float* x, int sx, float* y, int sy) {
   (void)hipGetLastError();
   if (n <= 0) return hipSuccess;
   // one block of 8 waves per CU (the weight image fills most of the LDS):
   // two waves per SIMD, so one wave's activations overlap the other's MFMAs
   const int64_t tiles = (n + 31) / 32;
   const unsigned g = unsigned(std::min<int64_t>(256, (tiles + 7) / 8));
-  k_sur_infer<<<g, 512, 0, s>>>(static_cast<const f16x8*>(img), imgb, n, x, sx, y, sy);
+  if (bf16)
+    k_sur_infer<Bf16><<<g, 512, 0, s>>>(static_cast<const Bf16::x8*>(img), imgb, n, x, sx, y,
+                                        sy);
+  else
+    k_sur_infer<Fp16><<<g, 512, 0, s>>>(static_cast<const Fp16::x8*>(img), imgb, n, x, sx, y,
+                                        sy);
   return hipGetLastError();
 }
 

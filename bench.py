@@ -735,7 +735,7 @@ def bench_plate_surrogate(args, akx):
                          element_ms=el_ms, node_ms=nd_ms, launch=shape)
         if on:
             # the network kernel alone, HIP events around its launch in the
-            # last step of the timed run
+            # last step of the kernel-timing run
             res[name]["network_ms"] = model.query("surrogate_network_ms")
         log("%s: %.4f ms/step (element %.4f ms, node %.4f ms)" % (name, res[name]["ms_per_step"],
                                                                 el_ms, nd_ms))
@@ -754,11 +754,9 @@ def bench_plate_surrogate(args, akx):
     lam, mu = nu * Eyoung / ((1 + nu) * (1 - 2 * nu)), Eyoung / (2 * (1 + nu))
     inside = np.all(np.abs(Ev) <= eps, axis=1)
     x = np.concatenate([Ev / eps, dmg[:, None]], axis=1).astype(np.float32)
-    y = model.surrogate_infer(x).astype(np.float64)
     trE = Ev[:, 0] + Ev[:, 1] + Ev[:, 2]
     S_lin = 2 * mu * Ev
     S_lin[:, :3] += (lam * trE)[:, None]
-    S_sur = S_lin + y[:, :6] * model.query("surrogate_stress_scale")
     # plain law in FP64 from C = I + 2E
     C = np.zeros((E, 3, 3))
     for k, (i, j) in enumerate(idx):
@@ -771,23 +769,51 @@ def bench_plate_surrogate(args, akx):
     S_nh = mu * (np.eye(3) - Ci) + (lam * lnJ)[:, None, None] * Ci
     S_ref = np.stack([S_nh[:, i, j] for i, j in idx], axis=1)
     wgt = (1.0 - dmg)[:, None]
-    diff = (S_sur - S_ref) * wgt
-    rel_stress = float(np.linalg.norm(diff[inside]) / np.linalg.norm((S_ref * wgt)[inside]))
-    per_qp = np.linalg.norm(diff, axis=1) / np.maximum(np.linalg.norm(S_ref * wgt, axis=1), 1e-30)
-    p99 = float(np.percentile(per_qp[inside & (np.linalg.norm(S_ref, axis=1) > 1e-3 *
-                                               np.linalg.norm(S_ref, axis=1).max())], 99))
-    # the surrogate run itself, from the same start
-    model.surrogate_enable(True)
-    model.set("damage", np.zeros(E))
-    reset(model, v0)
-    model.surrogate_fallbacks(reset=True)
-    model.solve_step(steps, akx.MIXED)
-    u_sur = model.get_displacement()
-    fb = model.surrogate_fallbacks()
-    rel_u = float(np.linalg.norm(u_sur - u_plain) / np.linalg.norm(u_plain))
+    strong = inside & (np.linalg.norm(S_ref, axis=1) > 1e-3 * np.linalg.norm(S_ref, axis=1).max())
+
+    def stress_error():
+        """The network's stress against the plain law on these strain
+        states: relative over the points inside the envelope, and the 99th
+        percentile per point."""
+        y = model.surrogate_infer(x).astype(np.float64)
+        diff = (S_lin + y[:, :6] * model.query("surrogate_stress_scale") - S_ref) * wgt
+        rel = float(np.linalg.norm(diff[inside]) / np.linalg.norm((S_ref * wgt)[inside]))
+        per_qp = np.linalg.norm(diff, axis=1) / np.maximum(np.linalg.norm(S_ref * wgt, axis=1),
+                                                            1e-30)
+        return rel, float(np.percentile(per_qp[strong], 99))
+
+    def surrogate_run():
+        """The surrogate run itself, from the same start: displacement error
+        against the plain run and fallback count."""
+        model.surrogate_enable(True)
+        model.set("damage", np.zeros(E))
+        reset(model, v0)
+        model.surrogate_fallbacks(reset=True)
+        model.solve_step(steps, akx.MIXED)
+        u_sur = model.get_displacement()
+        return (float(np.linalg.norm(u_sur - u_plain) / np.linalg.norm(u_plain)),
+                model.surrogate_fallbacks())
+
+    rel_stress, p99 = stress_error()
+    rel_u, fb = surrogate_run()
     log("after %d steps: stress rel error %.3e (p99 per point %.3e), %.2f%% points inside the "
         "envelope, displacement rel error %.3e, fallbacks %d" %
         (steps, rel_stress, p99, 100 * inside.mean(), rel_u, fb))
+    # the same network in BF16, the training format (FP16 above)
+    model.surrogate_format("bf16")
+    model.set_launch(**res["surrogate"]["launch"])
+    model.set("damage", np.zeros(E))
+    t = timed(model, akx.MIXED, args.steps, args.warmup, v0)
+    el_ms, nd_ms = kernel_times(model, akx.MIXED, min(args.steps, 100), v0)
+    bf = dict(ms_per_step=1e3 * t / args.steps, element_ms=el_ms, node_ms=nd_ms,
+              network_ms=model.query("surrogate_network_ms"))
+    bf["rel_stress_error"], bf["p99_point_rel_stress_error"] = stress_error()
+    bf["rel_u_error_vs_plain"], _ = surrogate_run()
+    model.surrogate_format("fp16")
+    log("bf16 network: %.4f ms/step (network %.4f ms), stress rel error %.3e (p99 %.3e), "
+        "displacement rel error %.3e" % (bf["ms_per_step"], bf["network_ms"],
+                                         bf["rel_stress_error"], bf["p99_point_rel_stress_error"],
+                                         bf["rel_u_error_vs_plain"]))
     flop_qp = 2 * (7 * 128 + 3 * 128 * 128 + 128 * 7)
     net_ms = res["surrogate"]["network_ms"]
     tflops = flop_qp * E * Q / (net_ms * 1e-3) / 1e12
@@ -805,7 +831,7 @@ def bench_plate_surrogate(args, akx):
                                "20 m/s; 7-128x4-7 smoothstep-swish surrogate trained on 10M GPU-generated "
                                "samples" % (E, N),
                    "elements": E, "nodes": N, "qp_per_element": Q, "parallelism": "1 GPU"},
-        "plain": res["plain"], "surrogate": res["surrogate"],
+        "plain": res["plain"], "surrogate": res["surrogate"], "surrogate_bf16_network": bf,
         "surrogate_over_plain": res["surrogate"]["qp_per_s"] / res["plain"]["qp_per_s"],
         "training": {"samples": int(model.query("surrogate_samples")), "generate_s": t_gen,
                      "iterations": iters, "batch": 8192, "train_s": t_train, "final_loss": loss},
@@ -813,7 +839,7 @@ def bench_plate_surrogate(args, akx):
                      "p99_point_rel_stress_error": p99, "inside_envelope": float(inside.mean()),
                      "rel_u_error_vs_plain": rel_u, "fallback_points": int(fb)},
         "roofline": {"bound": "mfma",
-                     "kernel": "akx::(anonymous namespace)::k_sur_infer(...)",
+                     "kernel": "akx::(anonymous namespace)::k_sur_infer<akx::mlp::Fp16>(...)",
                      "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tflops / 2500.0, "traffic": None,
                      "algorithmic_flops_per_qp": flop_qp, "avg_ms": net_ms},

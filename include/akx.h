@@ -264,7 +264,7 @@ int akx_autotune(akx_ctx* ctx, int precision, int nsteps, int* chosen, double* b
 /* ---- constitutive surrogate -------------------------------------------------
  * A 7 -> 128 x 4 -> 7 network with the smoothstep swish activation
  * a(z) = z S(z), S = t^2 (3 - 2 t), t = clamp(z / 6 + 1/2, 0, 1) (trained in
- * BF16, evaluated in FP16, FP32 accumulation, gfx950 MFMA) that stands in for
+ * BF16, evaluated in FP16 or BF16, FP32 accumulation, gfx950 MFMA) that stands in for
  * the neo-Hookean part of a
  * neohookean_damage material (material_neohookean_inline_impl.hh:46-65,
  * 145-158); the Marigo damage update (material_marigo_inline_impl.hh:67-76)
@@ -307,6 +307,13 @@ int akx_surrogate_dataset(akx_ctx* ctx, int64_t first, int64_t count, float* x, 
  * the plain law and are counted (akx_surrogate_fallbacks). */
 int akx_surrogate_enable(akx_ctx* ctx, int on);
 int akx_surrogate_fallbacks(akx_ctx* ctx, uint64_t* count, int reset);
+/* 16-bit format of the inference kernel's weights and activations (FP32
+ * accumulation in both): AKX_SURROGATE_FP16 (the default: the activation on
+ * packed FP16) or AKX_SURROGATE_BF16 (the training format: the activation in
+ * FP32, rounded to BF16).  The same master weights serve both; akx_query
+ * "surrogate_format" returns the current one. */
+enum akx_surrogate_format { AKX_SURROGATE_FP16 = 0, AKX_SURROGATE_BF16 = 1 };
+int akx_surrogate_format(akx_ctx* ctx, int format);
 
 /* ---------------------------------------------------------------------------
  * FEEngine operators on a caller's field (FEEngine, src/fe_engine/fe_engine.hh;

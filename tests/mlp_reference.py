@@ -89,8 +89,9 @@ def forward(p, x, rounded=True):
     return Z, cache
 
 
-# Inference form of the network (k_sur_infer): FP16 weights and activations,
-# FP32 accumulation, the activation in FP16 operations.
+# Inference forms of the network (k_sur_infer): FP16 or BF16 weights and
+# activations, FP32 accumulation; the activation in FP16 operations (FP16) or
+# in FP32 operations rounded once to BF16 (BF16).
 
 
 def f16(x):
@@ -99,8 +100,8 @@ def f16(x):
 
 
 def act_f16(z):
-    """a(z) as the inference kernel evaluates it from the accumulator values
-    z: Z = FP16(z), t = clamp(FP16(Z k + 1/2)), u = FP16(t (-2) + 3),
+    """a(z) as the FP16 inference kernel evaluates it from the accumulator
+    values z: Z = FP16(z), t = clamp(FP16(Z k + 1/2)), u = FP16(t (-2) + 3),
     a = FP16(Z FP16(FP16(t t) u)); each operation rounds once to nearest
     (products and sums of FP16 values are exact in float64 before that)."""
     Z = f16(z)
@@ -109,20 +110,37 @@ def act_f16(z):
     return f16(Z * f16(f16(t * t) * u))
 
 
-def forward_inference(p, x):
-    """Outputs of the inference kernel's network (float64 accumulation where
-    the kernel accumulates in FP32)."""
+def f32(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def act_bf16(z):
+    """a(z) as the BF16 inference kernel evaluates it from the accumulator
+    values z: Z = FP32(z), t = clamp(FP32(Z k + 1/2)) (one fused rounding),
+    u = FP32(t (-2) + 3), a = FP32(Z FP32(FP32(t t) u)), then rounded to BF16."""
+    Z = f32(z)
+    t = np.clip(f32(Z * ACT_K + 0.5), 0.0, 1.0)
+    u = f32(t * -2.0 + 3.0)
+    a = f32(Z * f32(f32(t * t) * u))
+    return bf16(a).astype(np.float64)
+
+
+def forward_inference(p, x, fmt="fp16"):
+    """Outputs of the inference kernel's network in the format fmt ("fp16"
+    or "bf16"; float64 accumulation where the kernel accumulates in FP32)."""
+    r16 = f16 if fmt == "fp16" else (lambda a: bf16(a).astype(np.float64))
+    act16 = act_f16 if fmt == "fp16" else act_bf16
     P = unpack(p)
     p32 = {k: np.asarray(v, dtype=np.float32) for k, v in P.items()}
     x = np.asarray(x, dtype=np.float32)
-    hi = f16(x).astype(np.float32)
-    lo = f16(x - hi)
+    hi = r16(x).astype(np.float32)
+    lo = r16(x - hi)
     X0 = np.concatenate([hi.astype(np.float64), lo], axis=1)
-    W0 = f16(p32["W0"])
+    W0 = r16(p32["W0"])
     Z = X0 @ np.concatenate([W0, W0], axis=1).T + p32["b0"].astype(np.float64)
     for layer in (1, 2, 3, 4):
-        H = act_f16(Z)
-        Z = H @ f16(p32["W%d" % layer]).T + p32["b%d" % layer].astype(np.float64)
+        H = act16(Z)
+        Z = H @ r16(p32["W%d" % layer]).T + p32["b%d" % layer].astype(np.float64)
     return Z
 
 

@@ -56,6 +56,31 @@ def test_inference_kernel_matches_reference():
     assert _rel(y, ye) < 3e-3
 
 
+def test_bf16_inference_kernel_matches_reference():
+    """The inference kernel in BF16 (weights and activations; the training
+    format) against the reference's BF16 inference form and the exact
+    network; switching back to FP16 rebuilds the FP16 image from the same
+    master weights (the same outputs bit for bit)."""
+    model = _plate_model()
+    p = ref.init_params(11).astype(np.float32)
+    model.surrogate_params(p)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, size=(1000, 7)).astype(np.float32)
+    x[:, 6] = np.abs(x[:, 6])
+    assert model.surrogate_format() == "fp16"
+    y16 = model.surrogate_infer(x)
+    assert model.surrogate_format("bf16") == "bf16"
+    y = model.surrogate_infer(x)
+    yi = ref.forward_inference(p.astype(np.float64), x, "bf16")
+    assert _rel(y, yi) < 5e-4
+    assert np.abs(y - yi).max() < 5e-3 * np.abs(yi).max()
+    ye, _ = ref.forward(p.astype(np.float64), x, rounded=False)
+    assert _rel(y, ye) < 1e-2
+    assert _rel(y16, ye) < _rel(y, ye)
+    model.surrogate_format("fp16")
+    assert np.array_equal(model.surrogate_infer(x), y16)
+
+
 def test_training_gradients_match_reference():
     """Forward + backward MFMA kernels (no update) against the CPU reference
     with the same BF16 rounding points, and the loss they report."""
@@ -247,6 +272,27 @@ def test_surrogate_step_in_a_graph(layout):
     assert np.array_equal(out["eager"][1], out["graph"][1])
     assert out["eager"][2] == out["graph"][2]
     assert np.abs(out["eager"][0][-1]).max() > 0
+
+
+def test_surrogate_step_bf16_format():
+    """The surrogate step with the network in BF16: a graph captured in FP16
+    is not replayed after the switch (the graph step equals the eager BF16
+    step bit for bit), and the two formats of one trained network give
+    displacements within 2e-3 of each other, not equal."""
+    model = _plate_model()
+    model.surrogate_generate(200, 250, seed=5)
+    model.surrogate_train(200, batch=4096, lr=2e-3)
+    u16 = _tension(model, 100, 5, True)[0]
+    _tension(model, 0, 5, True)
+    model.solve_step_graph(100, akx.MIXED)
+    assert np.array_equal(model.get_displacement(), u16)
+    model.surrogate_format("bf16")
+    ub = _tension(model, 100, 5, True)[0]
+    _tension(model, 0, 5, True)
+    model.solve_step_graph(100, akx.MIXED)
+    assert np.array_equal(model.get_displacement(), ub)
+    print("BF16 vs FP16 network after 100 steps: rel u %.3e" % _rel(ub, u16))
+    assert 0 < _rel(ub, u16) < 2e-3
 
 
 def test_surrogate_on_one_of_two_materials():

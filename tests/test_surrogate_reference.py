@@ -138,3 +138,38 @@ def test_inference_form_is_the_network():
     assert ei < 2e-3
     assert ei < eb  # closer than the BF16 training form
 
+
+
+def test_act_bf16_against_act():
+    """The BF16 inference kernel's activation (FP32 operations, one rounding
+    to BF16) gives a(z) within half a BF16 step (2^-8) of max(1, |a|), and is
+    the BF16 rounding of the exact a(z) except where the FP32 evaluation
+    lands on the other side of a rounding boundary (a few points in 10^6)."""
+    z = np.linspace(-60.0, 60.0, 1200001)
+    a = ref.act(z)
+    g = ref.act_bf16(z)
+    rel = np.abs(g - a) / np.maximum(1.0, np.abs(a))
+    assert rel.max() <= 2.0 ** -8 * 1.0001
+    assert np.mean(g != ref.bf16(a).astype(np.float64)) < 1e-5
+    # a planted slip in the FP32 steps (3 - t instead of 3 - 2t) is seen
+    t = np.clip(ref.f32(ref.f32(z) * ref.ACT_K + 0.5), 0.0, 1.0)
+    slip = ref.bf16(ref.f32(z * ref.f32(ref.f32(t * t) * (3.0 - t)))).astype(np.float64)
+    assert (np.abs(slip - a) / np.maximum(1.0, np.abs(a))).max() > 0.1
+
+
+def test_bf16_inference_form_is_the_training_form():
+    """The BF16 inference form is the BF16 training forward (weights, inputs
+    and activations rounded at the same places) up to the activation's FP32
+    evaluation, and the FP16 form is closer to the exact network than both."""
+    rng = np.random.default_rng(8)
+    p = ref.init_params(4).astype(np.float32).astype(np.float64)
+    x = rng.uniform(-1, 1, size=(512, 7)).astype(np.float32)
+    ye, _ = ref.forward(p, x, rounded=False)
+    yb, _ = ref.forward(p, x, rounded=True)
+    yi = ref.forward_inference(p, x, "bf16")
+    yf = ref.forward_inference(p, x, "fp16")
+    assert np.linalg.norm(yi - yb) / np.linalg.norm(yb) < 3e-4
+    ei = np.linalg.norm(yi - ye) / np.linalg.norm(ye)
+    ef = np.linalg.norm(yf - ye) / np.linalg.norm(ye)
+    assert ei < 1e-2
+    assert ef < 0.5 * ei
