@@ -74,7 +74,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
   dfree(c->fel_off); dfree(c->fel); dfree(c->flslot); dfree(c->fch_off); dfree(c->fcbase);
-  dfree(c->u_alt); dfree(c->fgeo); dfree(c->frec); dfree(c->frecx);
+  dfree(c->u_alt); dfree(c->frec);
   c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
@@ -648,13 +648,6 @@ int build_fused(akx_ctx* c, const std::vector<uint32_t>& order, const std::vecto
   HIP_TRY(c, hipMemcpy(c->flslot, lslot.data(), lslot.size() * 2, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->fch_off, ch_off.data(), ch_off.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->fcbase, cbase.data(), cbase.size() * 2, hipMemcpyHostToDevice));
-  // on triangles the mixed fused step reads each element's coordinate
-  // differences from a record instead of gathering X for its nodes (X is
-  // fixed after akx_init)
-  if (c->et == AKX_TRIANGLE_3) {
-    HIP_TRY(c, dalloc(&c->fgeo, size_t(E) * fused_geo_stride(n, d)));
-    HIP_TRY(c, launch_fused_geometry(c->et, c->stream, E, c->X, c->conn_soa, c->fgeo));
-  }
   c->fused_blocks = nblk;
   c->fused_elements = int64_t(el.size());
   c->fused_lds = FUSED_CH * n * d * 8;
@@ -1016,13 +1009,23 @@ int akx_init(akx_ctx* c) {
     HIP_TRY(c, dalloc(&c->elem_mat_d, size_t(E)));
     HIP_TRY(c, hipMemcpy(c->elem_mat_d, em.data(), size_t(E) * 2, hipMemcpyHostToDevice));
   }
-  // the fused mixed step on triangles streams one record per evaluation
-  // (built here, after the material indices it carries are on the device)
-  if (c->fused_ok && c->et == AKX_TRIANGLE_3) {
+  // the fused mixed step on triangles streams one 32-byte record per
+  // evaluation (built here, after the material indices it carries are on
+  // the device; X is fixed after akx_init): node ids in 23 bits and material
+  // indices in 17, else it gathers through element ids
+  if (c->fused_ok && c->et == AKX_TRIANGLE_3 && N <= FUSED_REC_MAX_NODES &&
+      int64_t(nm) <= FUSED_REC_MAX_MATERIALS) {
     HIP_TRY(c, dalloc(&c->frec, size_t(c->fused_elements) * 2));
-    HIP_TRY(c, dalloc(&c->frecx, size_t(c->fused_elements)));
-    HIP_TRY(c, launch_fused_records(c->stream, c->fused_elements, E, c->fel, c->conn_soa, c->fgeo,
-                                    c->flslot, c->elem_mat_d, c->frec, c->frecx));
+    float* geo = nullptr;  // per element, only to build the records
+    HIP_TRY(c, dalloc(&geo, size_t(E) * fused_geo_stride(n, d)));
+    hipError_t er = launch_fused_geometry(c->et, c->stream, E, c->X, c->conn_soa, geo);
+    if (er == hipSuccess)
+      er = launch_fused_records(c->stream, c->fused_elements, E, c->fel, c->conn_soa, geo,
+                                c->flslot, c->elem_mat_d, c->frec);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    dfree(geo);
+    HIP_TRY(c, er);
+    HIP_TRY(c, es);
   }
   if (any_damage) {
     std::vector<double> yd(size_t(E) * Q, 0.), ydi(size_t(E) * Q);
@@ -1418,9 +1421,10 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       F.ch_off = c->fch_off;
       F.cbase = c->fcbase;
       F.u_out = unext;
-      F.geo = c->fgeo;
-      F.rec = c->frec;
-      F.recx = c->frecx;
+      // B rebuilt from the streamed records (mixed_recompute 1) or from
+      // coordinates gathered through element ids (0): the same forces bit
+      // for bit
+      F.rec = c->mixed_recompute ? c->frec : nullptr;
       F.nblk = c->fused_blocks;
       HIP_TRY(c, launch_fused(c->et, g.kind, precision, c->mixed_recompute != 0, c->has_fext,
                               c->stream, EA, NA, F, c->fused_lds));
@@ -1662,6 +1666,7 @@ int akx_query(akx_ctx* c, const char* key, double* value) {
   else if (k == "patch_available") *value = c->patch_ok ? 1. : 0.;
   else if (k == "patch_packed") *value = (c->patch_ok && c->lcr) ? 1. : 0.;
   else if (k == "fused_available") *value = c->fused_ok ? 1. : 0.;
+  else if (k == "fused_records") *value = c->frec ? 1. : 0.;
   else if (k == "fused_elements") *value = double(c->fused_ok ? c->fused_elements : 0);
   else if (k == "element_block") *value = c->elem_block;
   else if (k == "node_block") *value = c->node_block;
@@ -1878,9 +1883,10 @@ int akx_autotune(akx_ctx* c, int precision, int nsteps, int* chosen, double* bes
             }
           }
   // the fused step (layout 6) has its own block shapes; on the mixed path it
-  // always rebuilds B
+  // rebuilds B, from per-evaluation records (mixed_recompute 1, triangles)
+  // or from coordinates gathered through element ids (0)
   if (c->fused_ok)
-    for (int rcmp = (precision == AKX_MIXED ? 1 : 0); rcmp < 2; ++rcmp)
+    for (int rcmp = (precision == AKX_MIXED && !c->frec ? 1 : 0); rcmp < 2; ++rcmp)
       for (int x = 0; x < 2; ++x) cand.push_back(Shape{256, 256, x, rcmp, 6, 0});
   for (const Shape& sh : cand) {
           const int eb = sh.eb, nb = sh.nb, x = sh.x, lay = sh.lay;

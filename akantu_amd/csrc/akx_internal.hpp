@@ -110,15 +110,29 @@ struct FusedArgs {
   const uint32_t* ch_off;  // [nblk + 1] first chunk table of each block
   const uint16_t* cbase;   // [chunk * (FUSED_NB + 1) + k] first slot of block node k in the chunk
   double* u_out;           // displacement written by the step (read from NodeArgs::u)
-  const float* geo;        // mixed path on triangles: per element x_a - x_0 (a > 0), FP32
   // mixed path on triangles: per evaluation j (entry of el), everything the
-  // element needs but u, read as a stream: rec[2 j] = the element's
-  // coordinate differences (4 FP32), rec[2 j + 1] = (node 0, node 1, node 2,
-  // LDS slot 0 | slot 1 << 16); recx[j] = LDS slot 2 | material << 16
+  // element needs but u, 32 bytes read as a stream (nullptr: element ids and
+  // gathers instead): rec[2 j] = the element's coordinate differences
+  // x_a - x_0, a = 1, 2 (4 FP32); rec[2 j + 1] = fused_record_ids(...)
   const uint4* rec;
-  const uint32_t* recx;
   int64_t nblk;
 };
+
+// Second word of a fused triangle record: the three node ids in the low 23
+// bits of x, y, z; LDS slots 0 and 1 in w (16 bits each, 0xFFFF: not a block
+// node); LDS slot 2 in 10 bits (0x3FF: not a block node) and the material in
+// 17 bits spread over the high 9 bits of x, y, z.
+constexpr int FUSED_REC_NODE_BITS = 23;
+constexpr int64_t FUSED_REC_MAX_NODES = int64_t(1) << FUSED_REC_NODE_BITS;
+constexpr int FUSED_REC_MAX_MATERIALS = 1 << 17;
+constexpr uint32_t FUSED_REC_SLOT2_NONE = 0x3FFu;
+__host__ __device__ inline uint4 fused_record_ids(uint32_t n0, uint32_t n1, uint32_t n2,
+                                                  uint32_t s0, uint32_t s1, uint32_t s2,
+                                                  uint32_t mat) {
+  const uint32_t q = s2 == 0xFFFFu ? FUSED_REC_SLOT2_NONE : s2;
+  return make_uint4(n0 | (q & 0x1FFu) << 23, n1 | (q >> 9) << 23 | (mat & 0xFFu) << 24,
+                    n2 | (mat >> 8) << 23, s0 | s1 << 16);
+}
 
 // Floats per element of the fused mixed step's geometry record: (n - 1) d
 // coordinate differences, padded to whole 16-byte vectors.
@@ -171,7 +185,7 @@ hipError_t launch_node(int d, int precision, bool has_fext, hipStream_t s, const
                        int block);
 hipError_t launch_fused_records(hipStream_t s, int64_t J, int64_t E, const uint32_t* el,
                                 const uint32_t* conn_soa, const float* geo, const uint16_t* lslot,
-                                const uint16_t* elem_mat, uint4* rec, uint32_t* recx);
+                                const uint16_t* elem_mat, uint4* rec);
 hipError_t launch_fused_geometry(int et, hipStream_t s, int64_t E, const double* X,
                                  const uint32_t* conn_soa, float* geo);
 hipError_t launch_stable_dt(int et, hipStream_t s, int64_t E, const double* X, const double* u,
@@ -366,9 +380,7 @@ struct akx_ctx {
   uint32_t *fel_off = nullptr, *fel = nullptr, *fch_off = nullptr;
   uint16_t *flslot = nullptr, *fcbase = nullptr;
   double* u_alt = nullptr;  // second displacement buffer of the fused step
-  float* fgeo = nullptr;     // geometry records of the fused mixed step (triangles)
   uint4* frec = nullptr;     // per-evaluation records of the fused mixed step (triangles)
-  uint32_t* frecx = nullptr;
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;
   // internal numbering: perm[internal] = caller id, inv[caller id] = internal

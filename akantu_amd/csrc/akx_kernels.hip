@@ -2081,8 +2081,10 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
 // arithmetic), contributions in LDS in FP32, each node's run summed in FP64,
 // the node update with the half-step velocity form of the mixed path.  Its
 // sums follow the FP64 path's assembly order (any fixed order will do here).
-template <int ET, int KIND, bool HAS_FEXT>
+// RECORD (triangles): the element data come from F.rec.
+template <int ET, int KIND, bool HAS_FEXT, bool RECORD>
 __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs NA, FusedArgs F) {
+  static_assert(!RECORD || ET == TRI3, "per-evaluation records exist for triangles only");
   static_assert(KIND != NH_DAMAGE, "the fused step recomputes elements: stateless laws only");
   static_assert(FUSED_NB <= 256 && FUSED_CH == 256, "one node and one element per thread of 256");
   using T = Elem<ET>;
@@ -2107,29 +2109,36 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
   for (uint32_t jc = j0; jc < j1; jc += FUSED_CH, ++c) {
     const uint32_t j = jc + uint32_t(t);
     if (j < j1) {
-      // Triangles read one record per evaluation, in the block's evaluation
-      // order: the coordinate differences (the same values k_fused_geometry
-      // formed from X), the node ids, the LDS slots and the material, 36
-      // bytes streamed instead of an element id and its dependent gathers
-      // of connectivity, geometry and material.  Larger elements share
-      // their nodes among more elements and gather X (cached) faster than
-      // they would read a record.
-      constexpr bool RECORD = ET == TRI3;
-      constexpr int G = fused_geo_stride(n, d);
+      // Triangles with records read one 32-byte record per evaluation, in
+      // the block's evaluation order: the coordinate differences (the same
+      // values k_fused_geometry formed from X), the node ids, the LDS slots
+      // and the material, streamed instead of an element id and its
+      // dependent gathers of connectivity, geometry and material; both forms
+      // give the same forces bit for bit.  Larger elements share their
+      // nodes among more elements and gather X (cached) faster than they
+      // would read a record.
+      constexpr int G = RECORD ? 4 : 1;
+      static_assert(FUSED_CH * 3 < int(FUSED_REC_SLOT2_NONE), "LDS slots fit the records");
       int64_t e = 0;  // element id (not needed with a record)
       int mat = 0;
       int64_t nd[n];
       uint32_t lsl[n];
-      float gx[RECORD ? G : 1];
+      float gx[G];
       if constexpr (RECORD) {
-        static_assert(G == 4 && n == 3, "triangle records: 4 coordinate differences, 3 nodes");
         const uint4 r0 = F.rec[size_t(j) * 2], r1 = F.rec[size_t(j) * 2 + 1];
-        const uint32_t rx = F.recx[j];
-        gx[0] = __uint_as_float(r0.x); gx[1] = __uint_as_float(r0.y);
-        gx[2] = __uint_as_float(r0.z); gx[3] = __uint_as_float(r0.w);
-        nd[0] = r1.x; nd[1] = r1.y; nd[2] = r1.z;
-        lsl[0] = r1.w & 0xFFFFu; lsl[1] = r1.w >> 16; lsl[2] = rx & 0xFFFFu;
-        mat = int(rx >> 16);
+        constexpr uint32_t LOW = (1u << FUSED_REC_NODE_BITS) - 1u;
+        gx[0] = __uint_as_float(r0.x);
+        gx[1] = __uint_as_float(r0.y);
+        gx[2] = __uint_as_float(r0.z);
+        gx[3] = __uint_as_float(r0.w);
+        nd[0] = r1.x & LOW;
+        nd[1] = r1.y & LOW;
+        nd[2] = r1.z & LOW;
+        lsl[0] = r1.w & 0xFFFFu;
+        lsl[1] = r1.w >> 16;
+        const uint32_t s2 = (r1.x >> 23) | ((r1.y >> 23) & 1u) << 9;
+        lsl[2] = s2 == FUSED_REC_SLOT2_NONE ? 0xFFFFu : s2;
+        mat = int((r1.y >> 24) | (r1.z >> 23) << 8);
       } else {
         e = F.el[j];
         mat = EA.elem_mat ? EA.elem_mat[e] : 0;
@@ -2233,27 +2242,25 @@ __global__ void __launch_bounds__(256) k_fused_records(int64_t J, int64_t E,
                                                        const float* __restrict__ geo,
                                                        const uint16_t* __restrict__ lslot,
                                                        const uint16_t* __restrict__ elem_mat,
-                                                       uint4* __restrict__ rec,
-                                                       uint32_t* __restrict__ recx) {
+                                                       uint4* __restrict__ rec) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= J) return;
   const int64_t e = el[j];
   const float4 g = reinterpret_cast<const float4*>(geo)[e];
   rec[j * 2] = make_uint4(__float_as_uint(g.x), __float_as_uint(g.y), __float_as_uint(g.z),
                           __float_as_uint(g.w));
-  rec[j * 2 + 1] = make_uint4(conn[e], conn[E + e], conn[2 * E + e],
-                              uint32_t(lslot[j * 3]) | (uint32_t(lslot[j * 3 + 1]) << 16));
   const uint32_t mat = elem_mat ? uint32_t(elem_mat[e]) : 0u;
-  recx[j] = uint32_t(lslot[j * 3 + 2]) | (mat << 16);
+  rec[j * 2 + 1] = fused_record_ids(conn[e], conn[E + e], conn[2 * E + e], lslot[j * 3],
+                                    lslot[j * 3 + 1], lslot[j * 3 + 2], mat);
 }
 
 hipError_t launch_fused_records(hipStream_t s, int64_t J, int64_t E, const uint32_t* el,
                                 const uint32_t* conn_soa, const float* geo, const uint16_t* lslot,
-                                const uint16_t* elem_mat, uint4* rec, uint32_t* recx) {
+                                const uint16_t* elem_mat, uint4* rec) {
   clear_error();
   if (J == 0) return hipSuccess;
   k_fused_records<<<unsigned((J + 255) / 256), 256, 0, s>>>(J, E, el, conn_soa, geo, lslot,
-                                                             elem_mat, rec, recx);
+                                                             elem_mat, rec);
   return hipGetLastError();
 }
 
@@ -2286,8 +2293,15 @@ template <int ET, int KIND>
 static hipError_t launch_fused_mixed_et(bool has_fext, hipStream_t s, const ElemArgs& EA,
                                         const NodeArgs& NA, const FusedArgs& F, int lds) {
   const unsigned g = unsigned(F.nblk);
-  if (has_fext) k_step_fused_mixed<ET, KIND, true><<<g, 256, lds, s>>>(EA, NA, F);
-  else k_step_fused_mixed<ET, KIND, false><<<g, 256, lds, s>>>(EA, NA, F);
+  if constexpr (ET == TRI3) {
+    if (F.rec) {
+      if (has_fext) k_step_fused_mixed<ET, KIND, true, true><<<g, 256, lds, s>>>(EA, NA, F);
+      else k_step_fused_mixed<ET, KIND, false, true><<<g, 256, lds, s>>>(EA, NA, F);
+      return hipGetLastError();
+    }
+  }
+  if (has_fext) k_step_fused_mixed<ET, KIND, true, false><<<g, 256, lds, s>>>(EA, NA, F);
+  else k_step_fused_mixed<ET, KIND, false, false><<<g, 256, lds, s>>>(EA, NA, F);
   return hipGetLastError();
 }
 

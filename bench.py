@@ -118,17 +118,17 @@ def fused_bytes(E, N, n, d, evaluations, nblk, node_block=256, chunk=256):
 
 
 def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_block=256,
-                      chunk=256):
+                      chunk=256, records=True):
     """Compulsory HBM bytes of one fused mixed step (slot layout 6 on the mixed
     path): as the FP64 one, but within a run the node update carries the
-    half-step velocity (v read and written, no a), and triangles stream one
-    36-byte record per evaluation (coordinate differences, node ids, LDS
-    slots, material) instead of the element list, LDS slots, connectivity
-    and coordinates."""
+    half-step velocity (v read and written, no a), and triangles with
+    records (mixed_recompute 1) stream one 32-byte record per evaluation
+    (coordinate differences, node ids, LDS slots, material) instead of the
+    element list, LDS slots, connectivity and coordinates."""
     chunks = evaluations / chunk + nblk
     tables = chunks * (node_block + 1) * 2 + (nblk + 1) * 8
-    if n == 3:
-        elements = evaluations * 36
+    if n == 3 and records:
+        elements = evaluations * 32
     else:
         elements = evaluations * (4 + 2 * n) + E * n * 4 + N * d * 8
     return (elements + tables + 2 * N * d * 8 + 2 * N * d * 8 + N * 8 + N * d
@@ -637,10 +637,12 @@ def bench_ensemble(args, akx):
         step64 = fused_bytes(E, N, n, d, model.query("fused_elements"), nblk) + N * 8
     if fused and shapes["mixed"]["slot_layout"] == 6:
         # one kernel per step: its roofline is the step's
-        kname = ("void akx::k_step_fused_mixed<0, 0, false>"
-                 "(akx::ElemArgs, akx::NodeArgs, akx::FusedArgs)")
+        records = (shapes["mixed"]["mixed_recompute"] == 1 and
+                   model.query("fused_records") == 1)
+        kname = ("void akx::k_step_fused_mixed<0, 0, false, %s>"
+                 "(akx::ElemArgs, akx::NodeArgs, akx::FusedArgs)" % ("true" if records else "false"))
         kbytes = fused_mixed_bytes(E, N, n, d, model.query("fused_elements"), nblk,
-                                   dt_per_node=True)
+                                   dt_per_node=True, records=records)
         kms = res["mixed"]["element_ms"]
         achieved = kbytes / (kms * 1e-3) / 1e9
         eb, nb = kbytes, 0

@@ -239,7 +239,12 @@ int akx_last_step_timing(akx_ctx* ctx, double* element_ms, double* node_ms);
  * the mixed path layouts 0/1, 2, 3 and 4/5 differ in rounding only
  * (patch-relative FP32 node values, partial sums), and so do the tables and
  * the rebuilt geometry (for hexahedron_8 rebuilt in the Walsh basis of the
- * trilinear shape functions, without B). */
+ * trilinear shape functions, without B).  The mixed fused step (6) always
+ * rebuilds B; on triangle_3 with mixed_recompute 1 from one 32-byte record
+ * per element evaluation streamed in the blocks' order (coordinate
+ * differences, node ids, LDS slots, material; meshes of up to 2^23 nodes
+ * and 2^17 materials), with 0 from coordinates gathered through element ids:
+ * the same result bit for bit. */
 int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_remap,
                    int mixed_recompute, int slot_layout, int node_stage);
 
@@ -249,6 +254,7 @@ int akx_set_launch(akx_ctx* ctx, int element_block, int node_block, int xcd_rema
  *   "patch_max_nodes", "patch_available" (1/0), "patch_packed" (1/0: 16-bit
  *   patch-local node | rank words), "fused_available" (1/0: slot layout 6),
  *   "fused_elements" (element evaluations of one fused step),
+ *   "fused_records" (1/0: per-evaluation records of the mixed fused step),
  *   "element_block", "node_block", "xcd_remap", "mixed_recompute", "slot_layout". */
 int akx_query(akx_ctx* ctx, const char* key, double* value);
 

@@ -31,13 +31,16 @@ def test_leap_form_drops_the_acceleration():
 def test_fused_mixed_against_fused_fp64():
     """The mixed fused step differs from the FP64 one by the acceleration
     (absent in the leap form), the per-node time step and, on triangles, the
-    36-byte record per evaluation in place of the element list, LDS slots,
-    connectivity and coordinates."""
+    32-byte record per evaluation in place of the element list, LDS slots,
+    connectivity and coordinates (without records, the same streams as the
+    FP64 step)."""
     E, N, d, ev, nblk = 10000, 5200, 2, 11700, 21
     fp64 = bench.fused_bytes(E, N, 3, d, ev, nblk)
     mixed = bench.fused_mixed_bytes(E, N, 3, d, ev, nblk, dt_per_node=True)
-    assert mixed - fp64 == pytest.approx(ev * 36 - ev * (4 + 6) - E * 3 * 4 - N * d * 8
+    assert mixed - fp64 == pytest.approx(ev * 32 - ev * (4 + 6) - E * 3 * 4 - N * d * 8
                                          - 2 * N * d * 8 + N * 8)
+    gathers = bench.fused_mixed_bytes(E, N, 3, d, ev, nblk, dt_per_node=True, records=False)
+    assert gathers - fp64 == pytest.approx(-2 * N * d * 8 + N * 8)
     # tetrahedra keep the element lists and gather the coordinates
     fp64_t = bench.fused_bytes(E, N, 4, 3, ev, nblk)
     mixed_t = bench.fused_mixed_bytes(E, N, 4, 3, ev, nblk)

@@ -83,6 +83,29 @@ def test_ensemble_mixed_error_per_specimen(layout):
         assert err < 1e-5, (k, err)
 
 
+def test_fused_mixed_records_equal_gathers():
+    """The mixed fused step on triangles rebuilds B from one 32-byte record
+    per element evaluation (mixed_recompute 1: node ids, LDS slots and the
+    material packed together) or from coordinates gathered through element
+    ids (0): the same displacements bit for bit, on 300 specimens (material
+    indices past 8 bits, in the record's second and third words), and each
+    specimen within 1e-5 of the oracle."""
+    en = _ensemble(count=300, seed=5)
+    out = []
+    for rc in (1, 0):
+        model, dt_mat = _gpu_model(en)
+        assert model.query("fused_available") == 1 and model.query("fused_records") == 1
+        model.set_launch(mixed_recompute=rc, slot_layout=6)
+        model.solve_step(300, akx.MIXED)
+        out.append(model.get_displacement())
+    assert np.array_equal(out[0], out[1])
+    assert np.abs(out[0]).max() > 0
+    for k in (0, 255, 256, 299):
+        _, ref = _oracle_specimen(en, k, 0.8 * dt_mat[k], 300)
+        n0, n1 = en.node_offsets[k], en.node_offsets[k + 1]
+        assert np.linalg.norm(out[0][n0:n1] - ref) / np.linalg.norm(ref) < 1e-5, k
+
+
 @pytest.mark.parametrize("precision,layout", [(akx.FP64, 0), (akx.FP64, 6), (akx.MIXED, 0),
                                               (akx.MIXED, 4), (akx.MIXED, 5), (akx.MIXED, 6)])
 def test_graph_replays_match_eager(precision, layout):
