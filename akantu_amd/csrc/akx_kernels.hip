@@ -1321,9 +1321,13 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   // The block's nodal state is staged through LDS: global loads and stores of
   // u, v, a, f_ext and the blocked flags are contiguous runs of doubles (fully
   // coalesced, each array written once), while each thread works on one node.
+  // The runs move as 16-byte pairs of doubles and 4-byte groups of flags:
+  // the block's first entry n0 D is a multiple of 64 (block sizes are
+  // multiples of 64) and the arrays are device allocations.
   constexpr int NBMAX = 256;
-  __shared__ double s_u[NBMAX * D], s_v[NBMAX * D], s_a[NBMAX * D], s_f[HAS_FEXT ? NBMAX * D : 1];
-  __shared__ uint8_t s_b[NBMAX * D];
+  __shared__ __align__(16) double s_u[NBMAX * D], s_v[NBMAX * D], s_a[NBMAX * D],
+      s_f[HAS_FEXT ? NBMAX * D : 2];
+  __shared__ __align__(16) uint8_t s_b[NBMAX * D];
   const int nb = blockDim.x;
   const int64_t n0 = logical_block(A.xcd_remap) * nb;
   if (n0 >= A.N) return;  // uniform over the block
@@ -1366,12 +1370,28 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
         for (int i = 0; i < D; ++i) x0[j * D + i] = slots[int64_t(sj[j]) * D + i];
     }
   }
-  for (int t = threadIdx.x; t < nk; t += nb) {
-    s_u[t] = A.u[k0 + t];
-    s_v[t] = A.v[k0 + t];
-    if (!leap_mid) s_a[t] = A.a[k0 + t];
-    s_b[t] = A.blocked[k0 + t];
-    if constexpr (HAS_FEXT) s_f[t] = A.fext[k0 + t];
+  {
+    const int nk2 = nk >> 1, nk4 = nk >> 2;
+    for (int t = threadIdx.x; t < nk2; t += nb) {
+      reinterpret_cast<double2*>(s_u)[t] = reinterpret_cast<const double2*>(A.u + k0)[t];
+      reinterpret_cast<double2*>(s_v)[t] = reinterpret_cast<const double2*>(A.v + k0)[t];
+      if (!leap_mid)
+        reinterpret_cast<double2*>(s_a)[t] = reinterpret_cast<const double2*>(A.a + k0)[t];
+      if constexpr (HAS_FEXT)
+        reinterpret_cast<double2*>(s_f)[t] = reinterpret_cast<const double2*>(A.fext + k0)[t];
+    }
+    for (int t = threadIdx.x; t < nk4; t += nb)
+      reinterpret_cast<uint32_t*>(s_b)[t] = reinterpret_cast<const uint32_t*>(A.blocked + k0)[t];
+    // the last block's odd tails
+    const int t2 = 2 * nk2 + int(threadIdx.x);
+    if (t2 < nk) {
+      s_u[t2] = A.u[k0 + t2];
+      s_v[t2] = A.v[k0 + t2];
+      if (!leap_mid) s_a[t2] = A.a[k0 + t2];
+      if constexpr (HAS_FEXT) s_f[t2] = A.fext[k0 + t2];
+    }
+    const int t4 = 4 * nk4 + int(threadIdx.x);
+    if (t4 < nk) s_b[t4] = A.blocked[k0 + t4];
   }
   // Node-ordered slots with A.stage_lds > 0: the block's slots are one
   // contiguous range [off[n0], off[n0 + cnt]); copy it into LDS with coalesced
@@ -1507,10 +1527,18 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   }
   __syncthreads();
   if (correct || predict) {
-    for (int t = threadIdx.x; t < nk; t += nb) {
-      A.u[k0 + t] = s_u[t];
-      A.v[k0 + t] = s_v[t];
-      if (correct && !leap_mid) A.a[k0 + t] = s_a[t];
+    const int nk2 = nk >> 1;
+    const bool wa = correct && !leap_mid;
+    for (int t = threadIdx.x; t < nk2; t += nb) {
+      reinterpret_cast<double2*>(A.u + k0)[t] = reinterpret_cast<const double2*>(s_u)[t];
+      reinterpret_cast<double2*>(A.v + k0)[t] = reinterpret_cast<const double2*>(s_v)[t];
+      if (wa) reinterpret_cast<double2*>(A.a + k0)[t] = reinterpret_cast<const double2*>(s_a)[t];
+    }
+    const int t2 = 2 * nk2 + int(threadIdx.x);
+    if (t2 < nk) {
+      A.u[k0 + t2] = s_u[t2];
+      A.v[k0 + t2] = s_v[t2];
+      if (wa) A.a[k0 + t2] = s_a[t2];
     }
   }
 }
