@@ -79,6 +79,9 @@ def test_bf16_inference_kernel_matches_reference():
     assert _rel(y16, ye) < _rel(y, ye)
     model.surrogate_format("fp16")
     assert np.array_equal(model.surrogate_infer(x), y16)
+    # an unknown format is refused and changes nothing
+    assert akx.lib.akx_surrogate_format(model.ctx.h, 7) == -1
+    assert model.surrogate_format() == "fp16"
 
 
 def test_training_gradients_match_reference():
