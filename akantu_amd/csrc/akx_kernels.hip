@@ -266,6 +266,20 @@ __device__ __forceinline__ void walsh8(float (&v)[8][3]) {
     }
 }
 
+// Gauss points of the 8-point hexahedron rule in the reference order (xi
+// fastest; gauss_integration_tmpl.hh:131-160): xi, eta, zeta = +-1/sqrt(3),
+// then eta zeta, xi zeta, xi eta (FP32 products), padded to 8 floats.
+constexpr float HEX8_G = 0.577350269189625764509f;
+#define AKX_HEX8_POINT(q)                                                        \
+  {((q)&1 ? HEX8_G : -HEX8_G), ((q)&2 ? HEX8_G : -HEX8_G), ((q)&4 ? HEX8_G : -HEX8_G), \
+   ((q)&2 ? HEX8_G : -HEX8_G) * ((q)&4 ? HEX8_G : -HEX8_G),                      \
+   ((q)&1 ? HEX8_G : -HEX8_G) * ((q)&4 ? HEX8_G : -HEX8_G),                      \
+   ((q)&1 ? HEX8_G : -HEX8_G) * ((q)&2 ? HEX8_G : -HEX8_G), 0.f, 0.f}
+__constant__ float k_hex8_points[8][8] = {AKX_HEX8_POINT(0), AKX_HEX8_POINT(1), AKX_HEX8_POINT(2),
+                                          AKX_HEX8_POINT(3), AKX_HEX8_POINT(4), AKX_HEX8_POINT(5),
+                                          AKX_HEX8_POINT(6), AKX_HEX8_POINT(7)};
+#undef AKX_HEX8_POINT
+
 // Mixed-precision forces of one 8-node hexahedron with the geometry rebuilt
 // at every point, in the Walsh basis of the trilinear shape functions.  With
 // node a at corner (s1, s2, s3) (element_class_hexahedron_8_inline_impl.hh
@@ -295,11 +309,12 @@ __device__ __forceinline__ void hex8_mixed_forces(const ElemArgs& A, int64_t e, 
     }
   walsh8<false>(hx);
   walsh8<false>(hu);
-  const float g = 0.577350269189625764509f;  // 1 / sqrt(3)
 #pragma unroll 1
   for (int q = 0; q < 8; ++q) {
-    const float xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g, zeta = (q & 4) ? g : -g;
-    const float ez = eta * zeta, xz = xi * zeta, xe = xi * eta;
+    // the point's coordinates and their products, uniform over the wave:
+    // scalar loads from the constant table instead of vector selects
+    const float* c = k_hex8_points[q];
+    const float xi = c[0], eta = c[1], zeta = c[2], ez = c[3], xz = c[4], xe = c[5];
     // J'[l + 3 j] = 8 dx_j / dxi_l,  H'[i + 3 l] = 8 du_i / dxi_l
     float J[9], H[9];
 #pragma unroll
@@ -376,13 +391,20 @@ __device__ __forceinline__ void hex8_mixed_forces(const ElemArgs& A, int64_t e, 
       const float q0 = P[i] * Ad[0] + P[i + 3] * Ad[1] + P[i + 6] * Ad[2];
       const float q1 = P[i] * Ad[3] + P[i + 3] * Ad[4] + P[i + 6] * Ad[5];
       const float q2 = P[i] * Ad[6] + P[i + 3] * Ad[7] + P[i + 6] * Ad[8];
+      // one fused multiply-add per term (a sum of products added at once
+      // would be formed first and cost one more instruction)
       F[1][i] += q0;
       F[2][i] += q1;
       F[4][i] += q2;
-      F[3][i] += eta * q0 + xi * q1;
-      F[5][i] += zeta * q0 + xi * q2;
-      F[6][i] += zeta * q1 + eta * q2;
-      F[7][i] += ez * q0 + xz * q1 + xe * q2;
+      F[3][i] += eta * q0;
+      F[3][i] += xi * q1;
+      F[5][i] += zeta * q0;
+      F[5][i] += xi * q2;
+      F[6][i] += zeta * q1;
+      F[6][i] += eta * q2;
+      F[7][i] += ez * q0;
+      F[7][i] += xz * q1;
+      F[7][i] += xe * q2;
     }
   }
   walsh8<true>(F);
