@@ -371,6 +371,8 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.csr_slot = precision == AKX_MIXED ? c->csr2_slot : c->csr_slot;
   A.slots = c->slots;
   A.u = c->u; A.v = c->v; A.a = c->a;
+  A.u_out = nullptr;
+  A.u_prev = nullptr;
   A.fext = c->fext;
   A.mass = c->mass;
   A.blocked = c->blocked;
@@ -409,10 +411,12 @@ int check_ready(akx_ctx* c) {
   return AKX_OK;
 }
 
-// time_net: record events around the surrogate network launch (if any)
-int run_elements(akx_ctx* c, int precision, bool time_net = false) {
+// time_net: record events around the surrogate network launch (if any);
+// u: the displacement the elements read (nullptr: c->u)
+int run_elements(akx_ctx* c, int precision, bool time_net = false, const double* u = nullptr) {
   for (auto& g : c->groups) {
     ElemArgs A = elem_args(c, g, precision);
+    if (u) A.u = u;
     if (time_net && A.sur_img) {
       for (auto& ev : c->net_ev)
         if (!ev) HIP_TRY(c, hipEventCreate(&ev));
@@ -1009,6 +1013,9 @@ int akx_init(akx_ctx* c) {
     HIP_TRY(c, dalloc(&c->elem_mat_d, size_t(E)));
     HIP_TRY(c, hipMemcpy(c->elem_mat_d, em.data(), size_t(E) * 2, hipMemcpyHostToDevice));
   }
+  // the second displacement buffer: the mixed path's previous displacement
+  // (position form), the fused step's output
+  if (!c->u_alt) HIP_TRY(c, dalloc(&c->u_alt, size_t(N) * d));
   // the fused mixed step on triangles streams one 32-byte record per
   // evaluation (built here, after the material indices it carries are on
   // the device; X is fixed after akx_init): node ids in 23 bits and material
@@ -1392,23 +1399,30 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
       c->pool.push_back(e);
     }
   }
-  // the mixed path carries the half-step velocity between the steps of a run
-  const int leap = precision == AKX_MIXED ? NODE_LEAP : 0;
-  // predictor of step 1
+  // the mixed path steps in the position form: u_{n-1} in place of v and a
+  // between the first predictor and the last corrector of a run, u_n and
+  // u_{n-1} in c->u and c->u_alt by turns
+  const int position = precision == AKX_MIXED ? NODE_POSITION : 0;
+  double* ucur = c->u;
+  double* uprev = c->u_alt;
+  // predictor of step 1 (the position form writes u_1 to the other buffer)
   {
-    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0) | leap, precision);
+    NodeArgs A = node_args(c, NODE_PREDICT | (nsteps == 1 ? NODE_SAVE_PREV : 0) | position,
+                           precision);
+    if (position) {
+      A.u_out = c->u_alt;
+      std::swap(ucur, uprev);
+    }
     HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
   }
   if (uses_fused(c, precision)) {
     // one fused launch per step; u alternates between c->u and c->u_alt
-    const int leap = precision == AKX_MIXED ? NODE_LEAP : 0;
-    double* ucur = c->u;
-    double* unext = c->u_alt;
+    double* unext = uprev;
     const KindGroup& g = c->groups[0];
     for (int s = 1; s <= nsteps; ++s) {
       const bool last = (s == nsteps);
       if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
-      int mode = NODE_CORRECT | leap;
+      int mode = NODE_CORRECT | position;
       if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
       else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
       NodeArgs NA = node_args(c, mode, precision);
@@ -1441,13 +1455,20 @@ int enqueue_steps(akx_ctx* c, int nsteps, int precision, bool timing) {
   for (int s = 1; s <= nsteps; ++s) {
     const bool last = (s == nsteps);
     if (timing && s == 1) HIP_TRY(c, hipEventRecord(c->pool[0], c->stream));
-    rc = run_elements(c, precision, timing && last);
+    rc = run_elements(c, precision, timing && last, ucur);
     if (rc) return rc;
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s - 1], c->stream));
-    int mode = NODE_CORRECT | leap;
+    int mode = NODE_CORRECT | position;
     if (last) mode |= NODE_STORE_FINT | NODE_STORE_INC;
     else mode |= NODE_PREDICT | (s + 1 == nsteps ? NODE_SAVE_PREV : 0);
     NodeArgs A = node_args(c, mode, precision);
+    if (position) {
+      // u_{n+1} over u_{n-1}; the last step leaves u_n in c->u
+      A.u = ucur;
+      A.u_prev = uprev;
+      A.u_out = last ? c->u : uprev;
+      if (!last) std::swap(ucur, uprev);
+    }
     HIP_TRY(c, launch_node(d, node_precision(c, precision), c->has_fext, c->stream, A, c->node_block));
     if (timing) HIP_TRY(c, hipEventRecord(c->pool[2 * s], c->stream));
   }

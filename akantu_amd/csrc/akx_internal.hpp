@@ -83,6 +83,8 @@ struct NodeArgs {
   double* u;
   double* v;
   double* a;
+  double* u_out;          // where the kernel writes u (nullptr: u itself)
+  const double* u_prev;   // NODE_POSITION: the previous step's displacement
   const double* fext;
   const double* mass;     // one value per node
   const uint8_t* blocked; // bit0 blocked, bit1 per-step Dirichlet increment
@@ -145,14 +147,16 @@ enum NodeMode {
   NODE_SAVE_PREV = 8,   // previous = u before the predictor
   NODE_STORE_INC = 16,  // increment = u - previous after the corrector
   NODE_GATHER = 32,     // gather forces only (with NODE_STORE_FINT)
-  // Half-step velocity form (mixed path): between the first predictor and the
-  // last corrector of a run, v holds v_{n+1/2} = v_n + dt/2 a_n and a is
-  // neither read nor written: a_n = (f_ext + f_int) / M, v_{n+1/2} =
-  // v_{n-1/2} + dt a_n, u_{n+1} = u_n + dt v_{n+1/2} -- the same central
-  // difference as the predictor / corrector pair, with one array less of
-  // traffic.  The first predictor forms v_{1/2} from (v, a), the last
-  // corrector returns v_n = v_{n-1/2} + dt/2 a_n and a_n.
-  NODE_LEAP = 64
+  // Position form (mixed path): between the first predictor and the last
+  // corrector of a run neither v nor a is read or written, the previous
+  // displacement u_{n-1} (NodeArgs::u_prev) is read instead:
+  // a_n = (f_ext + f_int) / M, u_{n+1} = u_n + ((u_n - u_{n-1}) + dt (dt a_n))
+  // -- the same central difference as the predictor / corrector pair with
+  // v_{n+1/2} = (u_{n+1} - u_n) / dt eliminated, two arrays less of traffic.
+  // The first predictor writes u_1 = u_0 + dt (v_0 + dt/2 a_0) to another
+  // buffer (NodeArgs::u_out) and keeps u_0; the last corrector returns
+  // v_n = (u_n - u_{n-1}) / dt + dt/2 a_n and a_n.
+  NODE_POSITION = 64
 };
 
 // Byte offset of the forces in the LDS of k_element_patch with list stride S:

@@ -1264,8 +1264,7 @@ __device__ __forceinline__ void node_correct(const NodeArgs& A, int64_t node, do
                                              const double* fi, const double* fext,
                                              const uint8_t* b, double* u, double* v, double* a) {
   const int mode = A.mode;
-  const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
-  const bool leap = mode & NODE_LEAP;
+  const bool correct = mode & NODE_CORRECT;
   const double M = A.mass[node];
   const double dcoef = 0.5 * dt;            // beta dt
   const double ecoef = 0. * 0.5 * dt * dt;  // alpha beta dt^2
@@ -1276,26 +1275,14 @@ __device__ __forceinline__ void node_correct(const NodeArgs& A, int64_t node, do
     double r = 0.;
     if constexpr (HAS_FEXT) r += fext[i];
     r += fi[i];
-    if (leap) {
-      if (!(b[i] & 1)) {
-        const double an = r / M;
-        if (predict) {
-          v[i] += dt * an;    // v_{n+1/2}
-        } else {
-          v[i] += dcoef * an; // v_n
-          a[i] = an;
-        }
-      }
-    } else {
-      double cache = 0.;
-      cache += -1. * a[i];
-      r += M * cache;
-      if (!(b[i] & 1)) {
-        const double delta = r / M;
-        u[i] += ecoef * delta;
-        v[i] += dcoef * delta;
-        a[i] += delta;
-      }
+    double cache = 0.;
+    cache += -1. * a[i];
+    r += M * cache;
+    if (!(b[i] & 1)) {
+      const double delta = r / M;
+      u[i] += ecoef * delta;
+      v[i] += dcoef * delta;
+      a[i] += delta;
     }
     if (mode & NODE_STORE_INC) {
       double inc = u[i];
@@ -1311,26 +1298,67 @@ template <int D>
 __device__ __forceinline__ void node_predict(const NodeArgs& A, int64_t node, double dt,
                                              const uint8_t* b, double* u, double* v, double* a) {
   const int mode = A.mode;
-  const bool correct = mode & NODE_CORRECT;
-  const bool leap = mode & NODE_LEAP;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     double ui = u[i];
     if (b[i] & 2) ui += A.bc_inc[i];
     if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = ui;
     if (!(b[i] & 1)) {
-      if (leap) {
-        // v_{n+1/2}: formed here on the first step, by the corrector after
-        if (!correct) v[i] += 0.5 * dt * a[i];
-        ui += dt * v[i];
-      } else {
-        const double vi = v[i], acc = a[i];
-        const double dt_a_n = dt * acc;
-        ui += dt * vi + 0.5 * dt * dt_a_n;
-        v[i] = vi + dt_a_n;
-      }
+      const double vi = v[i], acc = a[i];
+      const double dt_a_n = dt * acc;
+      ui += dt * vi + 0.5 * dt * dt_a_n;
+      v[i] = vi + dt_a_n;
     }
     u[i] = ui;
+  }
+}
+
+// The mixed path's node update in the position form (NODE_POSITION; see
+// NodeMode): the first predictor (NODE_PREDICT alone) from (u, v, a), the
+// steps between (NODE_CORRECT | NODE_PREDICT) from u and the previous
+// displacement up, the last corrector (NODE_CORRECT alone) returning v and
+// a.  Blocked degrees of freedom keep v and a and move by their Dirichlet
+// increment only, as in node_correct / node_predict.  fi: assembled internal
+// force (unused by the first predictor).
+template <int D, bool HAS_FEXT>
+__device__ __forceinline__ void node_position_form(const NodeArgs& A, int64_t node, double dt,
+                                                   const double* fi, const double* fext,
+                                                   const uint8_t* b, const double* up,
+                                                   double* u, double* v, double* a) {
+  const int mode = A.mode;
+  const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
+  const double M = correct ? A.mass[node] : 1.;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    if (mode & NODE_STORE_FINT) A.fint_out[node * D + i] = fi[i];
+    const bool free_dof = !(b[i] & 1);
+    double an = 0.;
+    if (correct) {
+      double r = 0.;
+      if constexpr (HAS_FEXT) r += fext[i];
+      r += fi[i];
+      an = r / M;
+    }
+    if (predict) {
+      double ui = u[i];
+      if (b[i] & 2) ui += A.bc_inc[i];
+      if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = ui;
+      if (free_dof) {
+        if (correct) ui += (u[i] - up[i]) + dt * (dt * an);
+        else ui += dt * (v[i] + 0.5 * dt * a[i]);
+      }
+      u[i] = ui;
+    } else if (correct) {
+      if (free_dof) {
+        v[i] = (u[i] - up[i]) / dt + 0.5 * dt * an;
+        a[i] = an;
+      }
+      if (mode & NODE_STORE_INC) {
+        double inc = u[i];
+        inc -= A.prev[node * D + i];
+        A.inc[node * D + i] = inc;
+      }
+    }
   }
 }
 
@@ -1358,9 +1386,10 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   const int nk = cnt * D;
   const int mode = A.mode;
   const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
-  const bool leap = mode & NODE_LEAP;
-  // the half-step form between two steps touches neither a nor its loads
-  const bool leap_mid = leap && correct && predict;
+  const bool position = mode & NODE_POSITION;
+  // the position form between two steps touches neither v nor a: s_v holds
+  // the previous displacement there
+  const bool pos_mid = position && correct && predict;
   const TS* __restrict__ slots = static_cast<const TS*>(A.slots);
   const bool gather = mode & (NODE_CORRECT | NODE_GATHER);
   const bool staged = A.node_ordered && A.stage_lds > 0 && gather;
@@ -1394,10 +1423,11 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   }
   {
     const int nk2 = nk >> 1, nk4 = nk >> 2;
+    const double* vsrc = pos_mid ? A.u_prev : A.v;
     for (int t = threadIdx.x; t < nk2; t += nb) {
       reinterpret_cast<double2*>(s_u)[t] = reinterpret_cast<const double2*>(A.u + k0)[t];
-      reinterpret_cast<double2*>(s_v)[t] = reinterpret_cast<const double2*>(A.v + k0)[t];
-      if (!leap_mid)
+      reinterpret_cast<double2*>(s_v)[t] = reinterpret_cast<const double2*>(vsrc + k0)[t];
+      if (!pos_mid)
         reinterpret_cast<double2*>(s_a)[t] = reinterpret_cast<const double2*>(A.a + k0)[t];
       if constexpr (HAS_FEXT)
         reinterpret_cast<double2*>(s_f)[t] = reinterpret_cast<const double2*>(A.fext + k0)[t];
@@ -1408,8 +1438,8 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
     const int t2 = 2 * nk2 + int(threadIdx.x);
     if (t2 < nk) {
       s_u[t2] = A.u[k0 + t2];
-      s_v[t2] = A.v[k0 + t2];
-      if (!leap_mid) s_a[t2] = A.a[k0 + t2];
+      s_v[t2] = vsrc[k0 + t2];
+      if (!pos_mid) s_a[t2] = A.a[k0 + t2];
       if constexpr (HAS_FEXT) s_f[t2] = A.fext[k0 + t2];
     }
     const int t4 = 4 * nk4 + int(threadIdx.x);
@@ -1540,26 +1570,46 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
         fi[i] = f[i];
         if constexpr (sizeof(TS) != 8) fi[i] = f[i] + c[i];
       }
-      node_correct<D, HAS_FEXT>(A, node, dt, fi, HAS_FEXT ? &s_f[ln * D] : nullptr, &s_b[ln * D],
-                                &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
+      if (position) {
+        // previous displacement: staged in s_v between two steps, read here
+        // by the last corrector (once per run)
+        double upl[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) upl[i] = pos_mid ? s_v[ln * D + i] : A.u_prev[node * D + i];
+        node_position_form<D, HAS_FEXT>(A, node, dt, fi, HAS_FEXT ? &s_f[ln * D] : nullptr,
+                                        &s_b[ln * D], upl, &s_u[ln * D], &s_v[ln * D],
+                                        &s_a[ln * D]);
+      } else {
+        node_correct<D, HAS_FEXT>(A, node, dt, fi, HAS_FEXT ? &s_f[ln * D] : nullptr,
+                                  &s_b[ln * D], &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
+      }
     }
-    if (predict) {
+    if (predict && !position) {
       node_predict<D>(A, node, dt, &s_b[ln * D], &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
+    } else if (predict && !gather) {
+      // the position form's first predictor
+      const double fz[D] = {};
+      node_position_form<D, HAS_FEXT>(A, node, dt, fz, HAS_FEXT ? &s_f[ln * D] : nullptr,
+                                      &s_b[ln * D], fz, &s_u[ln * D], &s_v[ln * D], &s_a[ln * D]);
     }
   }
   __syncthreads();
   if (correct || predict) {
+    // u to u_out (the position form's other buffer) or in place; v and a
+    // unless the position form leaves them (first predictor, between steps)
+    double* __restrict__ uo = A.u_out ? A.u_out : A.u;
     const int nk2 = nk >> 1;
-    const bool wa = correct && !leap_mid;
+    const bool wv = !position || (correct && !predict);
+    const bool wa = correct && !pos_mid;
     for (int t = threadIdx.x; t < nk2; t += nb) {
-      reinterpret_cast<double2*>(A.u + k0)[t] = reinterpret_cast<const double2*>(s_u)[t];
-      reinterpret_cast<double2*>(A.v + k0)[t] = reinterpret_cast<const double2*>(s_v)[t];
+      reinterpret_cast<double2*>(uo + k0)[t] = reinterpret_cast<const double2*>(s_u)[t];
+      if (wv) reinterpret_cast<double2*>(A.v + k0)[t] = reinterpret_cast<const double2*>(s_v)[t];
       if (wa) reinterpret_cast<double2*>(A.a + k0)[t] = reinterpret_cast<const double2*>(s_a)[t];
     }
     const int t2 = 2 * nk2 + int(threadIdx.x);
     if (t2 < nk) {
-      A.u[k0 + t2] = s_u[t2];
-      A.v[k0 + t2] = s_v[t2];
+      uo[k0 + t2] = s_u[t2];
+      if (wv) A.v[k0 + t2] = s_v[t2];
       if (wa) A.a[k0 + t2] = s_a[t2];
     }
   }
@@ -2129,7 +2179,7 @@ static hipError_t launch_node_d(int precision, bool has_fext, hipStream_t s, con
 // The same fused step on the mixed path: element forces in FP32 from node
 // differences formed in FP64 (mixed_element_forces, the patch kernel's
 // arithmetic), contributions in LDS in FP32, each node's run summed in FP64,
-// the node update with the half-step velocity form of the mixed path.  Its
+// the node update in the position form of the mixed path.  Its
 // sums follow the FP64 path's assembly order (any fixed order will do here).
 // RECORD (triangles): the element data come from F.rec.
 template <int ET, int KIND, bool HAS_FEXT, bool RECORD>
@@ -2235,27 +2285,30 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
   }
   if (!owner) return;
   const int mode = NA.mode;
-  // between the steps of a run the half-step form neither reads nor writes a
-  const bool leap_mid = (mode & NODE_LEAP) && (mode & NODE_CORRECT) && (mode & NODE_PREDICT);
+  // the position form (every launch of this kernel corrects): u_out holds
+  // the previous displacement until this node overwrites it; between the
+  // steps of a run neither v nor a is read or written
+  const bool mid = mode & NODE_PREDICT;
   const double dt = NA.dt_node ? NA.dt_node[node] : NA.dt;
-  double u[d], v[d], acc[d], fx[d];
+  double u[d], up[d], v[d], acc[d], fx[d];
   uint8_t bl[d];
 #pragma unroll
   for (int i = 0; i < d; ++i) {
     u[i] = u_in[node * d + i];
-    v[i] = NA.v[node * d + i];
-    acc[i] = leap_mid ? 0. : NA.a[node * d + i];
+    up[i] = F.u_out[node * d + i];
+    v[i] = mid ? 0. : NA.v[node * d + i];
+    acc[i] = mid ? 0. : NA.a[node * d + i];
     bl[i] = NA.blocked[node * d + i];
     fx[i] = HAS_FEXT ? NA.fext[node * d + i] : 0.;
   }
-  if (mode & (NODE_CORRECT | NODE_GATHER))
-    node_correct<d, HAS_FEXT>(NA, node, dt, f, fx, bl, u, v, acc);
-  if (mode & NODE_PREDICT) node_predict<d>(NA, node, dt, bl, u, v, acc);
+  node_position_form<d, HAS_FEXT>(NA, node, dt, f, fx, bl, up, u, v, acc);
 #pragma unroll
   for (int i = 0; i < d; ++i) {
     F.u_out[node * d + i] = u[i];
-    NA.v[node * d + i] = v[i];
-    if (!leap_mid) NA.a[node * d + i] = acc[i];
+    if (!mid) {
+      NA.v[node * d + i] = v[i];
+      NA.a[node * d + i] = acc[i];
+    }
   }
 }
 

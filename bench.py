@@ -53,7 +53,7 @@ def log(msg):
 
 def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_ordered=False,
                       patch_nodes=None, patch_blocks=0, dt_per_node=False, state_bytes=0,
-                      partials=0, packed=False, partials_node_major=False, leap=False):
+                      partials=0, packed=False, partials_node_major=False, position=False):
     """Compulsory HBM bytes of one step, each array touched once.
     pe: bytes of the element tables and force slots (4 mixed, 8 FP64);
     tables=False: B and det(J) w rebuilt from the coordinates (one more nodal
@@ -69,9 +69,10 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     packed: local node and run rank in one 16-bit word per (e, a), 3-node
     elements padded to 4; partials_node_major: the patch kernel reads each
     partial's node-major position and the node kernel streams its partials
-    (no entry list); leap: the node kernel carries the half-step velocity
-    between steps (mixed path) and reads / writes u and v only, not a."""
-    state = (4 if leap else 6) * N * d * 8
+    (no entry list); position: the node kernel steps in the position form
+    (mixed path) and reads u and the previous displacement and writes u,
+    neither v nor a."""
+    state = (3 if position else 6) * N * d * 8
     local = (E * (4 if n == 3 else n) * 2) if packed else E * n * (2 + 1)
     if partials:
         pos = patch_nodes * 4
@@ -120,8 +121,9 @@ def fused_bytes(E, N, n, d, evaluations, nblk, node_block=256, chunk=256):
 def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_block=256,
                       chunk=256, records=True):
     """Compulsory HBM bytes of one fused mixed step (slot layout 6 on the mixed
-    path): as the FP64 one, but within a run the node update carries the
-    half-step velocity (v read and written, no a), and triangles with
+    path): as the FP64 one, but within a run the node update is the position
+    form (the previous displacement read from the output buffer before it is
+    overwritten; neither v nor a), and triangles with
     records (mixed_recompute 1) stream one 32-byte record per evaluation
     (coordinate differences, node ids, LDS slots, material) instead of the
     element list, LDS slots, connectivity and coordinates."""
@@ -131,7 +133,7 @@ def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_blo
         elements = evaluations * 32
     else:
         elements = evaluations * (4 + 2 * n) + E * n * 4 + N * d * 8
-    return (elements + tables + 2 * N * d * 8 + 2 * N * d * 8 + N * 8 + N * d
+    return (elements + tables + 3 * N * d * 8 + N * 8 + N * d
             + (N * 8 if dt_per_node else 0))
 
 
@@ -289,7 +291,7 @@ def dominant(res, shapes, model, elem_kernel, node_kernel, E, N, n, Q, d, **kw):
                                patch_blocks=model.query("patch_blocks") if patch else 0,
                                partials=partials,
                                packed=patch and model.query("patch_packed") == 1,
-                               partials_node_major=patch and lay == 5, leap=True, **kw)
+                               partials_node_major=patch and lay == 5, position=True, **kw)
     el_ms, nd_ms = res["mixed"]["element_ms"], res["mixed"]["node_ms"]
     if el_ms >= nd_ms:
         kname, kbytes, kms = elem_kernel(patch, shapes["mixed"]["mixed_recompute"]), eb, el_ms

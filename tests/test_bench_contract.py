@@ -19,18 +19,19 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stderr
 
 
-def test_leap_form_drops_the_acceleration():
-    """The half-step velocity form reads and writes u and v only: 2 N d 8
-    bytes less nodal state than with the acceleration."""
+def test_position_form_drops_velocity_and_acceleration():
+    """The position form reads u and the previous displacement and writes u:
+    3 N d 8 bytes less nodal state than u, v and a read and written."""
     E, N, n, Q, d = 1000, 300, 4, 1, 3
-    _, with_a = bench.algorithmic_bytes(E, N, n, Q, d, 4)
-    _, leap = bench.algorithmic_bytes(E, N, n, Q, d, 4, leap=True)
-    assert with_a - leap == 2 * N * d * 8
+    _, with_va = bench.algorithmic_bytes(E, N, n, Q, d, 4)
+    _, position = bench.algorithmic_bytes(E, N, n, Q, d, 4, position=True)
+    assert with_va - position == 3 * N * d * 8
 
 
 def test_fused_mixed_against_fused_fp64():
-    """The mixed fused step differs from the FP64 one by the acceleration
-    (absent in the leap form), the per-node time step and, on triangles, the
+    """The mixed fused step differs from the FP64 one by the nodal state (the
+    position form: the previous displacement read in place of v and a), the
+    per-node time step and, on triangles, the
     32-byte record per evaluation in place of the element list, LDS slots,
     connectivity and coordinates (without records, the same streams as the
     FP64 step)."""
@@ -38,10 +39,10 @@ def test_fused_mixed_against_fused_fp64():
     fp64 = bench.fused_bytes(E, N, 3, d, ev, nblk)
     mixed = bench.fused_mixed_bytes(E, N, 3, d, ev, nblk, dt_per_node=True)
     assert mixed - fp64 == pytest.approx(ev * 32 - ev * (4 + 6) - E * 3 * 4 - N * d * 8
-                                         - 2 * N * d * 8 + N * 8)
+                                         - 3 * N * d * 8 + N * 8)
     gathers = bench.fused_mixed_bytes(E, N, 3, d, ev, nblk, dt_per_node=True, records=False)
-    assert gathers - fp64 == pytest.approx(-2 * N * d * 8 + N * 8)
+    assert gathers - fp64 == pytest.approx(-3 * N * d * 8 + N * 8)
     # tetrahedra keep the element lists and gather the coordinates
     fp64_t = bench.fused_bytes(E, N, 4, 3, ev, nblk)
     mixed_t = bench.fused_mixed_bytes(E, N, 4, 3, ev, nblk)
-    assert fp64_t - mixed_t == pytest.approx(2 * N * 3 * 8)
+    assert fp64_t - mixed_t == pytest.approx(3 * N * 3 * 8)
