@@ -74,7 +74,7 @@ void free_derived(akx_ctx* c) {
   dfree(c->tab64_d);
   dfree(c->node_inv_d); dfree(c->elem_inv_d);
   dfree(c->fel_off); dfree(c->fel); dfree(c->flslot); dfree(c->fch_off); dfree(c->fcbase);
-  dfree(c->u_alt); dfree(c->frec);
+  dfree(c->u_alt); dfree(c->pcoef); dfree(c->frec);
   c->fused_ok = false;
   c->patch_ok = false;
   dfree(c->slots); dfree(c->elem_mat_d); dfree(c->mats_d); dfree(c->damage); dfree(c->Yd);
@@ -373,6 +373,7 @@ NodeArgs node_args(akx_ctx* c, int mode, int precision) {
   A.u = c->u; A.v = c->v; A.a = c->a;
   A.u_out = nullptr;
   A.u_prev = nullptr;
+  A.pcoef = c->pcoef;
   A.fext = c->fext;
   A.mass = c->mass;
   A.blocked = c->blocked;
@@ -1016,6 +1017,7 @@ int akx_init(akx_ctx* c) {
   // the second displacement buffer: the mixed path's previous displacement
   // (position form), the fused step's output
   if (!c->u_alt) HIP_TRY(c, dalloc(&c->u_alt, size_t(N) * d));
+  HIP_TRY(c, dalloc(&c->pcoef, size_t(N)));
   // the fused mixed step on triangles streams one 32-byte record per
   // evaluation (built here, after the material indices it carries are on
   // the device; X is fixed after akx_init): node ids in 23 bits and material

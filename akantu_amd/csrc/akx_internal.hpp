@@ -85,6 +85,7 @@ struct NodeArgs {
   double* a;
   double* u_out;          // where the kernel writes u (nullptr: u itself)
   const double* u_prev;   // NODE_POSITION: the previous step's displacement
+  double* pcoef;          // NODE_POSITION: dt^2 / M per node, written by the first predictor
   const double* fext;
   const double* mass;     // one value per node
   const uint8_t* blocked; // bit0 blocked, bit1 per-step Dirichlet increment
@@ -152,10 +153,12 @@ enum NodeMode {
   // displacement u_{n-1} (NodeArgs::u_prev) is read instead:
   // a_n = (f_ext + f_int) / M, u_{n+1} = u_n + ((u_n - u_{n-1}) + dt (dt a_n))
   // -- the same central difference as the predictor / corrector pair with
-  // v_{n+1/2} = (u_{n+1} - u_n) / dt eliminated, two arrays less of traffic.
-  // The first predictor writes u_1 = u_0 + dt (v_0 + dt/2 a_0) to another
-  // buffer (NodeArgs::u_out) and keeps u_0; the last corrector returns
-  // v_n = (u_n - u_{n-1}) / dt + dt/2 a_n and a_n.
+  // v_{n+1/2} = (u_{n+1} - u_n) / dt eliminated, two arrays less of traffic;
+  // dt (dt a_n) is formed as (f_ext + f_int) dt^2 / M with dt^2 / M per node
+  // (NodeArgs::pcoef) in place of the mass and time step.  The first
+  // predictor writes u_1 = u_0 + dt (v_0 + dt/2 a_0) to another buffer
+  // (NodeArgs::u_out), keeps u_0 and writes dt^2 / M; the last corrector
+  // returns v_n = (u_n - u_{n-1}) / dt + dt/2 a_n and a_n.
   NODE_POSITION = 64
 };
 
@@ -383,7 +386,8 @@ struct akx_ctx {
   int64_t fused_blocks = 0, fused_elements = 0;
   uint32_t *fel_off = nullptr, *fel = nullptr, *fch_off = nullptr;
   uint16_t *flslot = nullptr, *fcbase = nullptr;
-  double* u_alt = nullptr;  // second displacement buffer of the fused step
+  double* u_alt = nullptr;  // second displacement buffer (position form, fused step)
+  double* pcoef = nullptr;  // position form: dt^2 / M per node
   uint4* frec = nullptr;     // per-evaluation records of the fused mixed step (triangles)
   akx::ElemTablesF tabf;
   akx::ElemTables* tab64_d = nullptr;

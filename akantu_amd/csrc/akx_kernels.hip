@@ -1327,28 +1327,32 @@ __device__ __forceinline__ void node_position_form(const NodeArgs& A, int64_t no
                                                    double* u, double* v, double* a) {
   const int mode = A.mode;
   const bool correct = mode & NODE_CORRECT, predict = mode & NODE_PREDICT;
-  const double M = correct ? A.mass[node] : 1.;
+  const bool mid = correct && predict;
+  // between the steps: dt^2 / M; the last corrector: M; the first predictor
+  // writes dt^2 / M for the steps after it
+  const double M = correct && !mid ? A.mass[node] : 1.;
+  const double k = mid ? A.pcoef[node] : 0.;
+  if (predict && !correct) A.pcoef[node] = dt * dt / A.mass[node];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     if (mode & NODE_STORE_FINT) A.fint_out[node * D + i] = fi[i];
     const bool free_dof = !(b[i] & 1);
-    double an = 0.;
+    double r = 0.;
     if (correct) {
-      double r = 0.;
       if constexpr (HAS_FEXT) r += fext[i];
       r += fi[i];
-      an = r / M;
     }
     if (predict) {
       double ui = u[i];
       if (b[i] & 2) ui += A.bc_inc[i];
       if (mode & NODE_SAVE_PREV) A.prev[node * D + i] = ui;
       if (free_dof) {
-        if (correct) ui += (u[i] - up[i]) + dt * (dt * an);
+        if (correct) ui += (u[i] - up[i]) + r * k;
         else ui += dt * (v[i] + 0.5 * dt * a[i]);
       }
       u[i] = ui;
     } else if (correct) {
+      const double an = r / M;
       if (free_dof) {
         v[i] = (u[i] - up[i]) / dt + 0.5 * dt * an;
         a[i] = an;
@@ -1460,7 +1464,8 @@ __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   __syncthreads();
   if (ln < cnt) {
     // one time step per node when independent bodies advance together
-    const double dt = A.dt_node ? A.dt_node[node] : A.dt;
+    // the position form between two steps needs no time step (dt^2 / M)
+    const double dt = pos_mid ? 0. : A.dt_node ? A.dt_node[node] : A.dt;
     if (gather) {
       double f[D], c[D];
 #pragma unroll
@@ -2289,7 +2294,7 @@ __global__ void __launch_bounds__(256) k_step_fused_mixed(ElemArgs EA, NodeArgs 
   // the previous displacement until this node overwrites it; between the
   // steps of a run neither v nor a is read or written
   const bool mid = mode & NODE_PREDICT;
-  const double dt = NA.dt_node ? NA.dt_node[node] : NA.dt;
+  const double dt = mid ? 0. : NA.dt_node ? NA.dt_node[node] : NA.dt;
   double u[d], up[d], v[d], acc[d], fx[d];
   uint8_t bl[d];
 #pragma unroll

@@ -71,7 +71,8 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
     partial's node-major position and the node kernel streams its partials
     (no entry list); position: the node kernel steps in the position form
     (mixed path) and reads u and the previous displacement and writes u,
-    neither v nor a."""
+    neither v nor a, and dt^2 / M per node in place of the mass and the
+    per-node time step."""
     state = (3 if position else 6) * N * d * 8
     local = (E * (4 if n == 3 else n) * 2) if packed else E * n * (2 + 1)
     if partials:
@@ -81,7 +82,8 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
                 + (pos if partials_node_major else 0))
         node = ((N + 1) * 4 + (0 if partials_node_major else pos) + patch_nodes * d * partials
                 + state + N * 8 + N * d
-                + (N * d * 8 if has_fext else 0) + (N * 8 if dt_per_node else 0))
+                + (N * d * 8 if has_fext else 0)
+                + (N * 8 if dt_per_node and not position else 0))
         return elem, node
     if patch_nodes is not None:
         elem = (local                                      # local connectivity, run ranks
@@ -103,7 +105,7 @@ def algorithmic_bytes(E, N, n, Q, d, pe, has_fext=False, tables=True, node_order
             + N * 8                  # lumped mass (one value per node)
             + N * d                  # blocked flags
             + (N * d * 8 if has_fext else 0)
-            + (N * 8 if dt_per_node else 0))
+            + (N * 8 if dt_per_node and not position else 0))
     return elem, node
 
 
@@ -123,7 +125,8 @@ def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_blo
     """Compulsory HBM bytes of one fused mixed step (slot layout 6 on the mixed
     path): as the FP64 one, but within a run the node update is the position
     form (the previous displacement read from the output buffer before it is
-    overwritten; neither v nor a), and triangles with
+    overwritten; neither v nor a; dt^2 / M per node in place of the mass and
+    the per-node time step), and triangles with
     records (mixed_recompute 1) stream one 32-byte record per evaluation
     (coordinate differences, node ids, LDS slots, material) instead of the
     element list, LDS slots, connectivity and coordinates."""
@@ -133,8 +136,7 @@ def fused_mixed_bytes(E, N, n, d, evaluations, nblk, dt_per_node=False, node_blo
         elements = evaluations * 32
     else:
         elements = evaluations * (4 + 2 * n) + E * n * 4 + N * d * 8
-    return (elements + tables + 3 * N * d * 8 + N * 8 + N * d
-            + (N * 8 if dt_per_node else 0))
+    return elements + tables + 3 * N * d * 8 + N * 8 + N * d
 
 
 def make_workload(n):
