@@ -107,6 +107,32 @@ def test_bar_c0_mixed_1000_steps(recompute, layout):
     assert abs(eg - ec) / ec <= 1e-5
 
 
+@pytest.mark.parametrize("layout", [5, 6])
+def test_mixed_state_across_runs(layout):
+    """The mixed path steps in the position form inside a run (u_{n-1} in
+    place of v and a) and returns the reference's state at the end of every
+    run: after runs of 1, 7, 13 and 29 steps, against the oracle after the
+    same 50 steps, u within 1e-6, v within 1e-5 and a within 1e-3 (a node's
+    net force is a small difference of FP32 element forces: 2.3e-4 with the
+    FP32 per-patch partials of layout 5, 1.3e-5 in the fused step), and
+    against one 50-step run u and v within 1e-7 and a within 1e-4 (only the
+    restart of the form differs: FP64 roundings of u, which an FP32 element
+    difference may carry one FP32 step further)."""
+    split = gh.bar_pair()
+    split.gpu.set_launch(slot_layout=layout)
+    for n in (1, 7, 13, 29):
+        split.step(n, akx.MIXED)
+    whole = gh.bar_pair()
+    whole.gpu.set_launch(slot_layout=layout)
+    whole.gpu.solve_step(50, akx.MIXED)
+    bounds = {"u": (1e-6, 1e-7), "v": (1e-5, 1e-7), "a": (1e-3, 1e-4)}
+    for field, name in (("u", "displacement"), ("v", "velocity"), ("a", "acceleration")):
+        g = split.gpu.get(name)
+        to_oracle, to_whole = bounds[field]
+        assert gh.rel(g, split.cpu.nodal(field)) <= to_oracle, field
+        assert gh.rel(g, whole.gpu.get(name)) <= to_whole, field
+
+
 def test_cube_tet4_fp64_bitwise_and_mixed():
     """reduced c1 (10^3 hexes -> 6,000 jittered tetrahedron_4), 200 steps; FP64
     with element slots and with the fused node-block step (layout 6)."""
