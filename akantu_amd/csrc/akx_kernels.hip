@@ -1252,7 +1252,8 @@ k_element_patch(ElemArgs A) {
 // f_int[node] = sum over the node's slots of -fe (DOFManager::
 // assembleElementalArrayLocalArray scale -1, dof_manager.cc:113-134).
 // FP64 slots: plain sum in slot order (bit-identical to the sequential
-// scatter).  FP32 slots: FP64 Neumaier-compensated sum.
+// scatter).  FP32 slots: FP64 Neumaier-compensated sum.  The mixed path
+// updates in the position form (node_position_form, NODE_POSITION).
 // ===========================================================================
 // One node's residual, lumped solve and corrector (NODE_CORRECT), with the
 // assembled internal force fi (sum of -fe) and the node's own state u, v, a
@@ -1373,7 +1374,8 @@ template <int D, int SRC, bool HAS_FEXT>
 __global__ void __launch_bounds__(256) k_node(NodeArgs A) {
   using TS = typename std::conditional<SRC == 1 || SRC == 3, float, double>::type;
   // The block's nodal state is staged through LDS: global loads and stores of
-  // u, v, a, f_ext and the blocked flags are contiguous runs of doubles (fully
+  // u, v, a, f_ext and the blocked flags (u and the previous displacement
+  // between the position form's steps) are contiguous runs of doubles (fully
   // coalesced, each array written once), while each thread works on one node.
   // The runs move as 16-byte pairs of doubles and 4-byte groups of flags:
   // the block's first entry n0 D is a multiple of 64 (block sizes are
